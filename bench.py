@@ -1,0 +1,171 @@
+"""Benchmark of the MLS-MPM transfer loop on one MI355X.
+
+Workload (two_boxes, 128^3 grid, dx 1/128, 1,048,576 particles, neo-Hookean
+E 5e4 nu 0.3 rho 1000, dt 5e-5): a step is one P2G -> grid update -> G2P pass
+over all particles with the particle state resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line: particle-steps/s of the tiled path (value), the plain
+GPU port measured in the same run, the roofline of the dominant kernel (the
+fused G2P+P2G tile kernel, timed with HIP events on its launch stream) and the
+CPU baseline (FP32 port of the same step on one host core, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_PARTICLE = 192   # x,v,C,F (96 B) read + written once per step
+BYTES_PER_NODE = 32        # grid (m,p) written once + (v) read once per active node
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU port leg")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    return ap.parse_args()
+
+
+def time_steps(ctx, steps, warmup):
+    import torch
+    ctx.step(warmup)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a.record(s)
+    ctx.step(steps)
+    b.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return a.elapsed_time(b) / 1e3, wall
+
+
+def cpu_baseline(scene, steps):
+    """FP32 port of the same step (oracle), one host core, `steps` steps of the full workload."""
+    import numpy as np
+    from oracle import oracle as O
+    prm = scene.oracle_params()
+    x = scene.x.copy()
+    t0 = time.perf_counter()
+    O.mls_run(prm, x, scene.v, None, None, scene.p_mass, scene.E, steps, dtype=np.float32)
+    dt = time.perf_counter() - t0
+    out = {"value": scene.n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+           "sample": f"{steps} steps of the full workload ({scene.n} particles, 128^3) with the FP32 oracle "
+                     f"(same MLS-MPM step), single thread, {dt:.1f} s"}
+    ref = None
+    if O.ref_available():
+        # the reference's own CPU solver (PIC semantics, no stress) on the same particles and grid
+        prm_r = O.ref_params(scene.dims, (scene.dx,) * 3, scene.origin, scene.dt, scene.gravity, 2)
+        m = np.full(scene.n, scene.p_mass, np.float32)
+        res = O.reference_run(prm_r, scene.x, scene.v, m, 1, want_grid=False)
+        ref = {"value": scene.n / res[-1], "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+               "sample": "1 step of MPMSolver<float,64>::step (reference PIC semantics) on the same "
+                         f"{scene.n} particles / 128^3 grid, single thread, {res[-1]:.2f} s"}
+    return out, ref
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (every box here has one GPU).",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from physgrad_amd import _lib
+    from physgrad_amd.scenes import two_boxes
+    from physgrad_amd.solver import MPMContext
+
+    _lib.load()
+    scene = two_boxes(n=args.particles)
+    N, K, W = scene.n, args.steps, args.warmup
+
+    ctx = MPMContext(scene.params(impl=_lib.IMPL_TILED))
+    ctx.set_particles(scene.x, scene.v)
+    t_tiled, wall = time_steps(ctx, K, W)
+    grid = ctx.get_grid()
+    active = int((grid["mass"] > 0).sum().item())
+    finite = bool(torch.isfinite(ctx.get_particles()["x"]).all().item())
+    # dominant kernel: fused G2P+P2G, timed with events on its launch stream
+    ctx.set_profiling(True)
+    kp = min(K, 50)
+    ctx.step(kp)
+    torch.cuda.synchronize()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    ctx.close()
+    g2p2g_ms, g2p2g_n = kt.get("tile_g2p2g", (float("nan"), 1))
+    avg_kernel_s = g2p2g_ms / g2p2g_n / 1e3
+    alg_bytes = N * BYTES_PER_PARTICLE + BYTES_PER_NODE * active
+    achieved = alg_bytes / avg_kernel_s
+    step_bytes_rate = alg_bytes * K / t_tiled
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_tile_g2p2g_two_boxes_128.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    plain = None
+    if not args.no_plain:
+        pctx = MPMContext(scene.params(impl=_lib.IMPL_PLAIN))
+        pctx.set_particles(scene.x, scene.v)
+        kpl = max(10, K // 4)
+        t_plain, _ = time_steps(pctx, kpl, min(W, 5))
+        pctx.close()
+        plain = {"value": N * kpl / t_plain, "steps": kpl, "ms_per_step": 1e3 * t_plain / kpl}
+
+    cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline(scene, args.cpu_steps)
+
+    out = {
+        "metric": "particle-steps/s (forward, 1M particles, 128^3 MLS-MPM)",
+        "value": N * K / t_tiled,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * t_tiled / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded two-box scene, generated in-process)",
+        "config": {"workload": "two_boxes_128: 128^3 grid, dx 1/128, 1,048,576 particles in two boxes at +-2 m/s, "
+                               "neo-Hookean E 5e4 nu 0.3 rho 1000, dt 5e-5, sticky walls 3 nodes",
+                   "particles": N, "grid": list(scene.dims), "impl": "tiled", "active_nodes": active},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "k_tile<3> (fused G2P+P2G)", "avg_launch_us": avg_kernel_s * 1e6,
+                     "alg_bytes_per_launch": alg_bytes},
+        "step_effective_GBps": step_bytes_rate / 1e9,
+        "kernel_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
+        "plain_port": plain,
+        "speedup_vs_plain": (N * K / t_tiled) / plain["value"] if plain else None,
+        "cpu_baseline": cpu,
+        "cpu_reference": cpu_ref,
+        "state_finite": finite,
+        "wall_s": wall,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

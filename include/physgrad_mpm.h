@@ -1,0 +1,129 @@
+/*
+ * physgrad_mpm.h - C-ABI of the MI355X MLS-MPM transfer loop.
+ *
+ * Every entry point returns an int status (PG_OK = 0); pg_mpm_last_error()
+ * gives the message of the last failure on a context (or, for NULL, of the
+ * last failure on this host thread).  Pointers are plain host or device
+ * pointers as the `memory` argument says; streams are hipStream_t passed as
+ * void* (NULL = the null stream).  A context is single-stream and not
+ * thread-safe; distinct contexts are independent.
+ *
+ * Reference interfaces these entry points replace (pedroscortes/physgrad):
+ *   pg_mpm_create / pg_mpm_destroy
+ *       MPMSolver<T,64>::MPMSolver(grid_dims, cell_size, origin, SimulationParams)
+ *           src/material_point_method.h:450-452
+ *       MPMSolver<T>::MPMSolver(const MPMSolverConfig&) / initialize() / cleanup()
+ *           src/mpm_solver.h:155-187, :506-532
+ *   pg_mpm_set_particles
+ *       MPMSolver<T,64>::addParticles   src/material_point_method.h:454-477
+ *       MPMSolver<T>::addParticles      src/mpm_solver.h:190-225
+ *   pg_mpm_step
+ *       MPMSolver<T,64>::step           src/material_point_method.h:479-500
+ *       MPMSolver<T>::step              src/mpm_solver.h:228-285
+ *       G2P2GKernelLauncher<T>::launchG2P2G  src/mpm_g2p2g_kernels.h:494-515
+ *   pg_mpm_get_particles / pg_mpm_get_grid / pg_mpm_get_time
+ *       getParticles/getGrid/getCurrentTime/getStepCount
+ *           src/material_point_method.h:503-506, src/mpm_solver.h:323-328
+ *   pg_mpm_forward_checkpointed / pg_mpm_backward
+ *       DifferentiableMPMSystem<T>::simulationStep / computeGradients
+ *           src/differentiable_mpm.h:748-783 (its backward is empty; this is the
+ *           working replacement), and the torch-side launchers
+ *           launch_mpm_timestep_forward/backward src/pytorch_autograd.cu:371-402
+ *   pg_spatial_hash_build
+ *       SpatialHashGrid<T,uint32_t>::buildHashTable src/sparse_data_structures.h:155-190
+ */
+#ifndef PHYSGRAD_MPM_H
+#define PHYSGRAD_MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_ERR_INVALID 1   /* bad argument or size */
+#define PG_ERR_HIP 2       /* HIP runtime error */
+#define PG_ERR_NOMEM 3     /* device allocation failed or over budget */
+#define PG_ERR_STATE 4     /* call out of order (e.g. backward without forward) */
+
+enum pg_mode { PG_MODE_REF_COMPAT = 0, PG_MODE_MLS_MPM = 1 };
+enum pg_material { PG_MAT_FIXED_COROTATED = 0, PG_MAT_NEO_HOOKEAN = 1 };
+enum pg_impl { PG_IMPL_PLAIN = 0, PG_IMPL_TILED = 1 };
+enum pg_memory { PG_HOST = 0, PG_DEVICE = 1 };
+
+typedef struct pg_mpm_ctx pg_mpm_ctx;
+
+/* Scene and solver parameters: a superset of the reference's
+ * MPMSolver::SimulationParams (src/material_point_method.h:440-448) and
+ * MPMSolverConfig (src/mpm_solver.h:29-66). */
+typedef struct pg_mpm_params {
+    int mode;                 /* pg_mode */
+    int impl;                 /* pg_impl (MLS-MPM only) */
+    int nx, ny, nz;           /* grid nodes per axis */
+    float dx, dy, dz;         /* cell size; MLS-MPM needs dx == dy == dz */
+    float ox, oy, oz;         /* grid origin */
+    float dt;                 /* time step */
+    float gx, gy, gz;         /* gravity */
+    int interpolation_order;  /* ref-compat: 1 linear, 2 quadratic, 3 cubic */
+    int material;             /* pg_material (MLS-MPM) */
+    float youngs_modulus;     /* E, used when no per-particle E is given */
+    float poisson_ratio;      /* nu */
+    float density;            /* rho: default particle mass = rho * particle_volume */
+    float particle_volume;    /* V0 */
+    int boundary_thickness;   /* sticky wall thickness in nodes (MLS-MPM) */
+    int max_particles;        /* capacity */
+} pg_mpm_params;
+
+/* Defaults reproduce the reference's SimulationParams: dt 1e-4, g (0,-9.81,0),
+ * quadratic interpolation, 64^3 nodes of 1/64, E 1e6, nu 0.3, rho 1000. */
+void pg_mpm_default_params(pg_mpm_params* p);
+
+int pg_mpm_create(const pg_mpm_params* p, pg_mpm_ctx** out);
+int pg_mpm_destroy(pg_mpm_ctx* ctx);
+const char* pg_mpm_last_error(const pg_mpm_ctx* ctx);
+
+/* Replace the particle set.  x, v: [3n] xyz-interleaved; C, F: [9n] row-major
+ * (NULL: C = 0, F = I); mass: [n] (NULL: density * particle_volume);
+ * E: [n] per-particle Young's modulus (NULL: params.youngs_modulus).
+ * ref-compat mode uses x, v and mass only. */
+int pg_mpm_set_particles(pg_mpm_ctx* ctx, int n, const float* x, const float* v, const float* C, const float* F,
+                         const float* mass, const float* E, int memory, void* stream);
+/* Read the particle set back in the caller's original particle order.  Any
+ * pointer may be NULL. */
+int pg_mpm_get_particles(pg_mpm_ctx* ctx, float* x, float* v, float* C, float* F, int memory, void* stream);
+int pg_mpm_num_particles(const pg_mpm_ctx* ctx);
+/* Grid of the last step: mass [G], momentum [3G] (ref-compat and plain MLS;
+ * pre-update momentum), velocity [3G] (post-update).  Any pointer may be NULL. */
+int pg_mpm_get_grid(pg_mpm_ctx* ctx, float* mass, float* momentum, float* velocity, int memory, void* stream);
+/* Advance nsteps steps on the stream. */
+int pg_mpm_step(pg_mpm_ctx* ctx, int nsteps, void* stream);
+/* Simulated time and step count (reference getCurrentTime/getStepCount). */
+int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around each
+ * kernel (replaces the cudaEventRecord timing of MPMSolver::step,
+ * src/mpm_solver.h:236-284, and its MPMPerformanceMetrics :72-101).
+ * pg_mpm_set_profiling resets the totals.  Kernel ids: 0 tiled P2G, 1 tiled
+ * fused G2P+P2G, 2 tiled G2P, 3 grid update, 4 binning, 5 plain P2G, 6 plain
+ * G2P, 7 grid clear, 8 ref-compat step. */
+int pg_mpm_set_profiling(pg_mpm_ctx* ctx, int on);
+int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long* launches);
+
+/* Spatial hash: cell ids (Morton code of the clamped truncated cell coordinate,
+ * modulo the cell count), stable sort of particle indices by cell id, per-cell
+ * start and count; bit-identical to SpatialHashGrid::buildHashTable.
+ * pos: device [3n]; res[a] = ceil((domain_max[a]-domain_min[a])/cell_size).
+ * Outputs are device arrays: cell_ids [n], particle_indices [n],
+ * cell_start [total], cell_count [total], total = res0*res1*res2.
+ * workspace: device scratch of pg_spatial_hash_workspace_size(n) bytes. */
+size_t pg_spatial_hash_workspace_size(unsigned n);
+int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                          const unsigned* res, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
+                          unsigned* cell_count, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+/*
+ * CPU oracle for the MLS-MPM particle-grid transfer loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this code, and only as the checker or
+ * the timed CPU baseline.  The product library (physgrad_amd) never calls it.
+ *
+ * Three parts:
+ *   1. ref-compat step (float): a restatement of the reference CPU solver
+ *      MPMSolver<float,64>::step (src/material_point_method.h:479-666) with the
+ *      same loop nesting and floating-point operation order, so its results are
+ *      bit-identical to the reference compiled with -ffp-contract=off.
+ *   2. MLS-MPM step (double and float): APIC transfers, deformation-gradient
+ *      update, fixed-corotated and neo-Hookean elasticity, sticky walls, plus the
+ *      hand-derived reverse (adjoint) step in double.  The reference has no
+ *      executable MLS-MPM, so this part is pinned by closed forms, invariants and
+ *      central finite differences (tests/test_oracle_mls.py).
+ *   3. cell keys + stable counting sort: a restatement of
+ *      SpatialHashGrid::mortonEncode3D/getCellID/buildHashTable
+ *      (src/sparse_data_structures.h:121-190).
+ */
+#ifndef PHYSGRAD_ORACLE_H
+#define PHYSGRAD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- ref-compat ---------------- */
+typedef struct {
+    int nx, ny, nz;
+    float dx, dy, dz;
+    float ox, oy, oz;
+    float dt;
+    float gx, gy, gz;
+    int order; /* 1 linear, 2 quadratic, 3 cubic (reference SimulationParams::interpolation_order) */
+} orc_ref_params;
+
+/* Runs nsteps reference steps in place.  x, v: [3n] xyz-interleaved; m: [n].
+ * grid_mass [G], grid_mom [3G], grid_vel [3G] receive the grid of the last step
+ * (node index k*nx*ny + j*nx + i, as MPMGrid::getNodeIndex). */
+void orc_ref_run(const orc_ref_params* p, int n, float* x, float* v, const float* m,
+                 int nsteps, float* grid_mass, float* grid_mom, float* grid_vel);
+
+/* ---------------- MLS-MPM ---------------- */
+enum { ORC_FIXED_COROTATED = 0, ORC_NEO_HOOKEAN = 1 };
+
+typedef struct {
+    int nx, ny, nz;       /* grid nodes per axis */
+    double dx;            /* cell size (isotropic) */
+    double ox, oy, oz;    /* grid origin */
+    double dt;
+    double gx, gy, gz;
+    int bc;               /* sticky wall thickness in nodes */
+    int material;         /* ORC_FIXED_COROTATED or ORC_NEO_HOOKEAN */
+    double nu;            /* Poisson ratio */
+    double p_vol;         /* initial particle volume V0 */
+} orc_mls_params;
+
+/* State: x,v [3n]; C,F [9n] row-major; mass [n]; E [n] (Young's modulus).
+ * grid_m [G], grid_v [3G] (optional, may be NULL) receive the last step's grid
+ * mass and post-update grid velocity. */
+void orc_mls_run_f64(const orc_mls_params* p, int n, double* x, double* v, double* C, double* F,
+                     const double* mass, const double* E, int nsteps,
+                     double* grid_m, double* grid_v);
+void orc_mls_run_f32(const orc_mls_params* p, int n, float* x, float* v, float* C, float* F,
+                     const float* mass, const float* E, int nsteps,
+                     float* grid_m, float* grid_v);
+
+/* Kirchhoff stress tau = P(F) F^T for one particle (double). */
+void orc_kirchhoff_f64(int material, double mu, double lam, const double* F, double* tau);
+/* Polar factor R of F (F = R S). Returns the Newton iteration count. */
+int orc_polar_f64(const double* F, double* R, double* S);
+
+/* One reverse step.  Inputs: the state (x,v,C,F) at the START of the step and
+ * the adjoints of the state at the END of the step (ax, av, aC, aF).  On return
+ * ax..aF hold the adjoints of the state at the start of the step and aE
+ * accumulates dL/dE. */
+void orc_mls_adjoint_step_f64(const orc_mls_params* p, int n,
+                              const double* x, const double* v, const double* C, const double* F,
+                              const double* mass, const double* E,
+                              double* ax, double* av, double* aC, double* aF, double* aE);
+
+/* ---------------- cell keys + counting sort ---------------- */
+unsigned orc_morton3(unsigned x, unsigned y, unsigned z);
+/* particle cell id = morton(clamped truncated coords) % total_cells */
+void orc_cell_ids(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                  const unsigned* res, unsigned* ids);
+/* stable counting sort (count, exclusive scan, scatter in index order) */
+void orc_counting_sort(const unsigned* ids, unsigned n, unsigned total_cells,
+                       unsigned* particle_indices, unsigned* cell_start, unsigned* cell_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+// Driver that instantiates the reference's own CPU MPM solver and spatial hash
+// from their headers where they lie (src/material_point_method.h,
+// src/sparse_data_structures.h) and exposes them through extern "C" for the
+// golden-vector script and the CPU baseline.  Built by oracle/Makefile into
+// oracle/_ref/ only.  TEST INFRASTRUCTURE ONLY.
+#include <cstring>
+#include <vector>
+#include <array>
+#include <chrono>
+#include "material_point_method.h"
+#include "sparse_data_structures.h"
+
+using physgrad::ConceptVector3D;
+
+namespace {
+void copy_out(const physgrad::mpm::MPMSolver<float, 64>& s, int n, float* x, float* v) {
+    const auto& P = s.getParticles();
+    for (int i = 0; i < n; ++i) {
+        auto p = P.getPosition(i);
+        auto u = P.getVelocity(i);
+        for (int a = 0; a < 3; ++a) { x[3 * i + a] = p[a]; v[3 * i + a] = u[a]; }
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Reference MPMSolver<float,64>: ctor (material_point_method.h:450-452),
+// addParticles (:454-477), step (:479-500), getParticles/getGrid (:503-504).
+int ref_mpm_run(int nx, int ny, int nz, float dx, float dy, float dz, float ox, float oy, float oz,
+                float dt, float gx, float gy, float gz, int order, int n, float* x, float* v,
+                const float* m, int nsteps, float* grid_mass, float* grid_mom, float* grid_vel,
+                double* seconds) {
+    using Solver = physgrad::mpm::MPMSolver<float, 64>;
+    Solver::SimulationParams prm;
+    prm.timestep = dt;
+    prm.gravity = ConceptVector3D<float>(gx, gy, gz);
+    prm.interpolation_order = order;
+    int3 dims(nx, ny, nz);
+    Solver s(dims, ConceptVector3D<float>(dx, dy, dz), ConceptVector3D<float>(ox, oy, oz), prm);
+    std::vector<ConceptVector3D<float>> P(n), V(n);
+    std::vector<float> M(m, m + n), Vol(n, 1.0f);
+    for (int i = 0; i < n; ++i) {
+        P[i] = ConceptVector3D<float>(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        V[i] = ConceptVector3D<float>(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+    }
+    s.addParticles(P, V, M, Vol);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < nsteps; ++k) s.step();
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    copy_out(s, n, x, v);
+    const auto& G = s.getGrid();
+    size_t tot = (size_t)nx * ny * nz;
+    for (size_t i = 0; i < tot; ++i) {
+        if (grid_mass) grid_mass[i] = G.mass[i];
+        for (int a = 0; a < 3; ++a) {
+            if (grid_mom) grid_mom[3 * i + a] = G.momentum[i][a];
+            if (grid_vel) grid_vel[3 * i + a] = G.velocity[i][a];
+        }
+    }
+    return 0;
+}
+
+// Reference SpatialHashGrid<float,uint32_t>::buildHashTable
+// (sparse_data_structures.h:56-82 ctor, :155-190 build, :284-287 accessors).
+// cell ids are recovered from the sorted index list and the per-cell counts.
+int ref_spatial_hash(const float* pos, unsigned n, float cell_size, const float* dmin, const float* dmax,
+                     unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
+                     unsigned* cell_count, unsigned* total_cells_out) {
+    std::array<float, 3> lo{dmin[0], dmin[1], dmin[2]}, hi{dmax[0], dmax[1], dmax[2]};
+    physgrad::sparse::SpatialHashGrid<float, uint32_t> h(cell_size, lo, hi, n);
+    std::vector<float> p(pos, pos + 3 * (size_t)n);
+    h.buildHashTable(p, n);
+    unsigned tot = h.getTotalCells();
+    if (total_cells_out) *total_cells_out = tot;
+    const auto& idx = h.getParticleIndices();
+    const auto& st = h.getCellStartIndices();
+    const auto& ct = h.getCellParticleCounts();
+    for (unsigned i = 0; i < n; ++i) particle_indices[i] = idx[i];
+    for (unsigned c = 0; c < tot; ++c) {
+        cell_start[c] = st[c];
+        cell_count[c] = ct[c];
+        for (unsigned k = 0; k < ct[c]; ++k) cell_ids[idx[st[c] + k]] = c;
+    }
+    return 0;
+}
+
+}  // extern "C"

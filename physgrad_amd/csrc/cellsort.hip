@@ -1,0 +1,205 @@
+// Spatial hash: cell ids and a stable LSD radix sort of particle indices by
+// cell id, bit-identical to SpatialHashGrid::buildHashTable
+// (src/sparse_data_structures.h:121-190), whose host counting sort scatters
+// in index order, i.e. is stable; any stable sort on the same keys yields the
+// same permutation.
+//
+// Radix sort: 8-bit digits, 4096 keys per workgroup (256 threads x 16 rounds).
+// Per pass: per-block digit histogram, one-workgroup exclusive scan of the
+// digit-major histogram table, then a stable scatter in which each 256-key
+// round ranks keys inside a wave64 with eight ballots (lanes with equal digit
+// form a peer mask) and across the four waves with an LDS table.  Cell start
+// and count come from a binary search of the sorted ids for every cell, which
+// also defines start for empty cells as the reference's prefix sum does.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/physgrad_mpm.h"
+
+namespace pg {
+namespace {
+
+constexpr int RS_T = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_T * RS_ROUNDS;
+constexpr int RADIX = 256;
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// getGridCoords + getCellID (src/sparse_data_structures.h:134-152)
+__global__ void k_cell_ids(const float* __restrict__ pos, unsigned n, float cell, float m0, float m1, float m2,
+                           unsigned r0, unsigned r1, unsigned r2, uint32_t* __restrict__ ids,
+                           uint32_t* __restrict__ idx) {
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned c0 = (unsigned)((pos[3 * (size_t)i] - m0) / cell);
+    unsigned c1 = (unsigned)((pos[3 * (size_t)i + 1] - m1) / cell);
+    unsigned c2 = (unsigned)((pos[3 * (size_t)i + 2] - m2) / cell);
+    c0 = c0 < r0 - 1 ? c0 : r0 - 1;
+    c1 = c1 < r1 - 1 ? c1 : r1 - 1;
+    c2 = c2 < r2 - 1 ? c2 : r2 - 1;
+    uint32_t code = expand_bits(c0) | (expand_bits(c1) << 1) | (expand_bits(c2) << 2);
+    ids[i] = code % (r0 * r1 * r2);
+    idx[i] = i;
+}
+
+__global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t* __restrict__ keys, unsigned n, int shift,
+                                                  uint32_t* __restrict__ hist, unsigned nb) {
+    __shared__ uint32_t h[RADIX];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        size_t i = base + (size_t)r * RS_T + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of `total` entries in place, one workgroup
+__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ a, unsigned total) {
+    __shared__ uint32_t s[1024];
+    const unsigned per = (total + 1023) / 1024;
+    const unsigned lo = threadIdx.x * per;
+    const unsigned hi = min(lo + per, total);
+    uint32_t sum = 0;
+    for (unsigned i = lo; i < hi; ++i) sum += a[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (unsigned i = lo; i < hi; ++i) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                     unsigned n, int shift, const uint32_t* __restrict__ hist,
+                                                     unsigned nb) {
+    __shared__ uint32_t base[RADIX];
+    __shared__ uint32_t wc[RS_T / 64][RADIX];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    base[tid] = hist[(size_t)tid * nb + blockIdx.x];
+    for (int w = 0; w < RS_T / 64; ++w) wc[w][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const size_t b0 = (size_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        size_t i = b0 + (size_t)r * RS_T + tid;
+        bool act = i < n;
+        uint32_t k = act ? kin[i] : 0u;
+        uint32_t v = act ? vin[i] : 0u;
+        uint32_t d = (k >> shift) & (RADIX - 1);
+        uint64_t peers = __ballot(act);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            uint64_t bm = __ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? bm : ~bm;
+        }
+        uint32_t pre = __popcll(peers & lt);
+        uint32_t cnt = __popcll(peers);
+        if (act && pre == 0) wc[wave][d] = cnt;
+        __syncthreads();
+        if (act) {
+            uint32_t off = base[d] + pre;
+            for (int w = 0; w < wave; ++w) off += wc[w][d];
+            kout[off] = k;
+            vout[off] = v;
+        }
+        __syncthreads();
+        {
+            uint32_t tot = 0;
+            for (int w = 0; w < RS_T / 64; ++w) { tot += wc[w][tid]; wc[w][tid] = 0; }
+            base[tid] += tot;
+        }
+        __syncthreads();
+    }
+}
+
+// cell_start[c] = lower_bound(ids, c); cell_count[c] = lower_bound(ids, c+1) - cell_start[c]
+__global__ void k_cell_ranges(const uint32_t* __restrict__ ids, unsigned n, unsigned total,
+                              uint32_t* __restrict__ start, uint32_t* __restrict__ count) {
+    unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    auto lb = [&](uint32_t key) {
+        unsigned lo = 0, hi = n;
+        while (lo < hi) {
+            unsigned mid = (lo + hi) >> 1;
+            if (ids[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    unsigned a = lb(c), b = lb(c + 1);
+    start[c] = a;
+    count[c] = b - a;
+}
+
+inline size_t rs_blocks(unsigned n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+}  // namespace
+}  // namespace pg
+
+extern "C" size_t pg_spatial_hash_workspace_size(unsigned n) {
+    size_t nb = pg::rs_blocks(n ? n : 1);
+    size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
+    return 3 * a + (((size_t)pg::RADIX * nb * 4 + 255) & ~(size_t)255);
+}
+
+extern "C" int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                                     const unsigned* res, unsigned* cell_ids, unsigned* particle_indices,
+                                     unsigned* cell_start, unsigned* cell_count, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+    using namespace pg;
+    if (!domain_min || !res || !(cell_size > 0.f)) return PG_ERR_INVALID;
+    if (res[0] == 0 || res[1] == 0 || res[2] == 0) return PG_ERR_INVALID;
+    const uint64_t total64 = (uint64_t)res[0] * res[1] * res[2];
+    if (total64 > 0xFFFFFFFFull) return PG_ERR_INVALID;
+    const unsigned total = (unsigned)total64;
+    if (n > 0 && (!pos || !cell_ids || !particle_indices)) return PG_ERR_INVALID;
+    if (!cell_start || !cell_count) return PG_ERR_INVALID;
+    if (workspace_bytes < pg_spatial_hash_workspace_size(n) || (n > 0 && !workspace)) return PG_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
+    uint32_t* kb[2] = {(uint32_t*)workspace, (uint32_t*)((char*)workspace + a)};
+    uint32_t* v2 = (uint32_t*)((char*)workspace + 2 * a);
+    uint32_t* hist = (uint32_t*)((char*)workspace + 3 * a);
+    const uint32_t* sorted = cell_ids;
+    if (n > 0) {
+        k_cell_ids<<<(n + 255) / 256, 256, 0, st>>>(pos, n, cell_size, domain_min[0], domain_min[1], domain_min[2],
+                                                   res[0], res[1], res[2], cell_ids, particle_indices);
+        int bits = 0;
+        while (bits < 32 && ((total - 1) >> bits)) ++bits;
+        int passes = (bits + 7) / 8;
+        if (passes < 2) passes = 2;
+        if (passes & 1) ++passes;  // an even pass count ends in particle_indices
+        const unsigned nb = (unsigned)rs_blocks(n);
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t* ki = p == 0 ? cell_ids : kb[(p - 1) & 1];
+            const uint32_t* vi = (p & 1) ? v2 : particle_indices;
+            uint32_t* ko = kb[p & 1];
+            uint32_t* vo = (p & 1) ? particle_indices : v2;
+            const int shift = 8 * p;  // passes <= 4, digits above the key width are zero
+            k_rs_hist<<<nb, RS_T, 0, st>>>(ki, n, shift, hist, nb);
+            k_rs_scan<<<1, 1024, 0, st>>>(hist, RADIX * nb);
+            k_rs_scatter<<<nb, RS_T, 0, st>>>(ki, vi, ko, vo, n, shift, hist, nb);
+            sorted = ko;
+        }
+    }
+    k_cell_ranges<<<(total + 255) / 256, 256, 0, st>>>(sorted, n, total, cell_start, cell_count);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PG_OK : PG_ERR_HIP;
+}

@@ -1,0 +1,120 @@
+// Host-side launch functions of the MPM kernels (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace pg {
+
+// ref-compat geometry (reference MPMGrid: dims, cell size, origin; order 1..3)
+struct RefGeom {
+    int nx, ny, nz;
+    float o[3];
+    float h[3];
+    int order;
+};
+
+// MLS-MPM scene description used by every MLS kernel.
+struct MlsGeom {
+    int nx, ny, nz;
+    float dx, inv_dx;
+    float o[3];
+    float dt;
+    float g[3];
+    int bc;          // sticky wall thickness (nodes)
+    int material;    // MAT_FIXED_COROTATED / MAT_NEO_HOOKEAN
+    float nu;
+    float E;         // uniform Young's modulus (used when no per-particle E)
+    float p_vol;     // V0
+    float p_mass;    // uniform mass (used when no per-particle mass)
+};
+
+struct KernelProf;
+hipError_t refcompat_step(const RefGeom& g, int n, float* x, float* v, const float* m, float* gm, float* gp, float* gv,
+                          float dt, const float* grav, hipStream_t st, KernelProf* prof);
+
+// ---- MLS-MPM plain port: one thread per particle, global float atomics, dense grid
+hipError_t mls_plain_step(const MlsGeom& g, int n, size_t cap, float4* S, const float* mass, const float* E,
+                          float4* mp, float4* gv, hipStream_t st, KernelProf* prof);
+
+// ---- layout conversion between user AoS arrays and the six float4 planes
+// pid (optional) maps storage slot -> user index; NULL means identity.
+hipError_t pack_state(int n, size_t cap, float4* S, const float* x, const float* v, const float* C, const float* F,
+                      hipStream_t st);
+hipError_t unpack_state(int n, size_t cap, const float4* S, const uint32_t* pid, float* x, float* v, float* C, float* F,
+                        hipStream_t st);
+hipError_t unpermute_f32(int n, const float* src, const uint32_t* pid, float* dst, hipStream_t st);
+hipError_t fill_f4(float4* p, size_t n, float4 v, hipStream_t st);
+hipError_t fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+hipError_t iota_u32(uint32_t* p, size_t n, hipStream_t st);
+hipError_t grid_split(size_t G, const float4* mp, const float4* gv, float* mass, float* mom, float* vel, hipStream_t st);
+
+// ---- per-kernel timing with HIP events on the launch stream (opt-in)
+enum ProfKernel {
+    PK_TILE_P2G = 0,   // tiled P2G only (first step of a call)
+    PK_TILE_G2P2G = 1, // tiled fused G2P(n) + P2G(n+1)
+    PK_TILE_G2P = 2,   // tiled G2P only (last step of a call)
+    PK_GRID = 3,       // grid update (tiled: flagged blocks; plain: dense)
+    PK_BIN = 4,        // tile keys scan + scatter
+    PK_PLAIN_P2G = 5,
+    PK_PLAIN_G2P = 6,
+    PK_CLEAR = 7,
+    PK_REF_STEP = 8,
+    PK_COUNT = 16
+};
+
+struct KernelProf {
+    bool on = false;
+    double total_ms[PK_COUNT] = {};
+    long long count[PK_COUNT] = {};
+    struct Rec { int k; hipEvent_t a, b; };
+    Rec recs[4096];
+    int nrec = 0;
+    hipEvent_t pool[8192];
+    int npool = 0;
+    hipEvent_t get();
+    void begin(int k, hipStream_t st, hipEvent_t* out);
+    void end(int k, hipEvent_t a, hipStream_t st);
+    hipError_t flush();  // synchronizes recorded events and accumulates
+};
+
+// RAII helper: times the launches issued in its scope when prof is on.
+struct ProfScope {
+    KernelProf* p;
+    int k;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    ProfScope(KernelProf* p_, int k_, hipStream_t s) : p(p_), k(k_), st(s) {
+        if (p && p->on) p->begin(k, st, &a);
+    }
+    ~ProfScope() {
+        if (p && p->on && a) p->end(k, a, st);
+    }
+};
+
+// ---- MLS-MPM tiled path (cell-binned particles, LDS-staged transfers)
+struct TiledWork {
+    KernelProf* prof;      // optional per-kernel timing
+    int T;                 // tile edge in cells
+    int ntx, nty, ntz;     // tiles per axis
+    int tile_slots;        // number of tile ids (Morton range)
+    int gbx, gby, gbz;     // 8^3 grid blocks per axis
+    uint32_t* count;       // [tile_slots]
+    uint32_t* start;       // [tile_slots]
+    uint32_t* active;      // [tile_slots] active tile list
+    uint32_t* nactive;     // [1]
+    uint32_t* key;         // [cap] tile id per storage slot
+    uint32_t* rank;        // [cap] rank within tile
+    uint32_t* perm;        // [cap] sorted position -> storage slot
+    uint32_t* flags;       // [gbx*gby*gbz] touched grid blocks
+    int nblocks_launch;    // persistent grid size of the tile kernels
+};
+
+hipError_t tiled_init_keys(const MlsGeom& g, TiledWork& w, int n, size_t cap, const float4* S, hipStream_t st);
+// Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on return
+// the current buffers are in S (swapped by the caller as reported by *swapped).
+hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
+                       uint32_t*& pid2, float*& E, float*& E2, float*& mass, float*& mass2, float4* mp, float4* gv,
+                       int nsteps, hipStream_t st);
+
+}  // namespace pg

@@ -1,0 +1,103 @@
+"""Synthetic workloads of the MLS-MPM transfer loop (seeded, reproducible).
+
+    single_box        config 1: 64^3, dx 1/64, 32,768 particles uniform in
+                      [0.3,0.5]^3 (seed 1), fixed-corotated E 1e4, nu 0.2, rho 1,
+                      v (0,-1,0), g -9.8, dt 1e-4, sticky walls 3 nodes, 200 steps
+    two_boxes         config 2: 128^3, dx 1/128, 1,048,576 particles, half in
+                      [0.2,0.45]^3 and half in [0.55,0.8]x[0.2,0.45]^2 (seed 2),
+                      v = +-2 along x (towards each other), neo-Hookean E 5e4,
+                      nu 0.3, dt 5e-5, 1000 steps.  The density is not given by
+                      the workload; rho = 1000 keeps the elastic CFL number
+                      c dt/dx = 0.05 (rho = 1 would give 1.66, an unstable run).
+    rollout           config 3: 64^3, 262,144 particles in a seeded random box
+                      (seed 3), fixed-corotated, nu 0.2, per-particle E
+                      log-uniform in [5e3,5e4], rho 1000 (unstated; CFL 0.05),
+                      dt 1e-4, 512 steps; loss = |mean x - (0.6,0.3,0.5)|^2.
+Particle volume V0 = box volume / N; mass = rho V0.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class Scene:
+    name: str
+    dims: tuple
+    dx: float
+    dt: float
+    steps: int
+    material: int           # 0 fixed-corotated, 1 neo-Hookean
+    E: float
+    nu: float
+    rho: float
+    p_vol: float
+    x: np.ndarray
+    v: np.ndarray
+    gravity: tuple = (0.0, -9.8, 0.0)
+    bc: int = 3
+    origin: tuple = (0.0, 0.0, 0.0)
+    E_particles: Optional[np.ndarray] = None
+    target: Optional[tuple] = None
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+    @property
+    def p_mass(self) -> float:
+        return self.rho * self.p_vol
+
+    def params(self, impl: int = 1, max_particles: Optional[int] = None):
+        from .solver import make_params
+        from . import _lib
+        return make_params(mode=_lib.MODE_MLS_MPM, impl=impl, dims=self.dims, dx=self.dx, origin=self.origin,
+                           dt=self.dt, gravity=self.gravity, material=self.material, youngs_modulus=self.E,
+                           poisson_ratio=self.nu, density=self.rho, particle_volume=self.p_vol,
+                           boundary_thickness=self.bc, max_particles=int(max_particles or self.n))
+
+    def oracle_params(self):
+        from oracle import oracle as O  # test infrastructure
+        return O.mls_params(self.dims, self.dx, self.origin, self.dt, self.gravity, self.bc, self.material,
+                            self.nu, self.p_vol)
+
+
+def _uniform(rng, n, lo, hi):
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    return (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+
+
+def single_box(n: int = 32768, seed: int = 1) -> Scene:
+    rng = np.random.default_rng(seed)
+    x = _uniform(rng, n, (0.3,) * 3, (0.5,) * 3)
+    v = np.tile(np.array([0.0, -1.0, 0.0], np.float32), (n, 1))
+    return Scene("single_box_64", (64, 64, 64), 1 / 64, 1e-4, 200, 0, 1e4, 0.2, 1.0, 0.2 ** 3 / n, x, v)
+
+
+def two_boxes(n: int = 1 << 20, seed: int = 2, grid: int = 128) -> Scene:
+    rng = np.random.default_rng(seed)
+    h = n // 2
+    a = _uniform(rng, h, (0.2, 0.2, 0.2), (0.45, 0.45, 0.45))
+    b = _uniform(rng, n - h, (0.55, 0.2, 0.2), (0.8, 0.45, 0.45))
+    x = np.concatenate([a, b])
+    v = np.zeros((n, 3), np.float32)
+    v[:h, 0] = 2.0
+    v[h:, 0] = -2.0
+    vol = 2 * 0.25 ** 3
+    return Scene("two_boxes_128", (grid,) * 3, 1 / grid, 5e-5, 1000, 1, 5e4, 0.3, 1000.0, vol / n, x, v)
+
+
+def rollout(n: int = 262144, seed: int = 3) -> Scene:
+    rng = np.random.default_rng(seed)
+    size = 0.15 + 0.1 * rng.random(3)
+    lo = 0.25 + (0.5 - size) * rng.random(3) * 0.5
+    x = _uniform(rng, n, lo, lo + size)
+    v = np.zeros((n, 3), np.float32)
+    E = np.exp(rng.uniform(np.log(5e3), np.log(5e4), n)).astype(np.float32)
+    s = Scene("rollout_64", (64, 64, 64), 1 / 64, 1e-4, 512, 0, 1e4, 0.2, 1000.0, float(np.prod(size)) / n, x, v,
+              E_particles=E, target=(0.6, 0.3, 0.5))
+    return s

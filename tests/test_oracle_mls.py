@@ -1,0 +1,139 @@
+"""Pins of the MLS-MPM oracle (the reference has no executable MLS-MPM).
+
+Each check is independent of the oracle's formulas: closed-form stresses,
+a library polar decomposition (numpy SVD), exact reproduction of affine
+velocity fields by APIC (catches a transposed C or a sign error in x_i - x_p),
+conservation of mass and linear momentum, sticky walls, and central finite
+differences for the reverse step.
+"""
+import numpy as np
+import pytest
+
+DIMS = (16, 16, 16)
+DX = 1.0 / 16
+
+
+def scene(O, rng, n=64, material=0, g=(0, -9.8, 0), bc=3, lo=0.35, span=0.3):
+    x = lo + span * rng.random((n, 3))
+    prm = O.mls_params(DIMS, DX, (0, 0, 0), 1e-3, g, bc, material, 0.3, 1e-4)
+    return prm, x
+
+
+@pytest.mark.parametrize("s", [0.8, 1.0, 1.3])
+def test_kirchhoff_uniform_stretch_closed_form(oracle_lib, s):
+    O = oracle_lib
+    mu, lam = 1000.0, 1500.0
+    F = s * np.eye(3)
+    nh = O.kirchhoff(1, mu, lam, F)
+    np.testing.assert_allclose(nh, (mu * (s * s - 1) + lam * 3 * np.log(s)) * np.eye(3), rtol=1e-12, atol=1e-9)
+    fc = O.kirchhoff(0, mu, lam, F)
+    np.testing.assert_allclose(fc, (2 * mu * (s - 1) * s + lam * (s ** 3 - 1) * s ** 3) * np.eye(3),
+                               rtol=1e-12, atol=1e-9)
+
+
+def test_kirchhoff_zero_for_rotation(oracle_lib):
+    O = oracle_lib
+    th = 0.7
+    Q = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    for mat in (0, 1):
+        np.testing.assert_allclose(O.kirchhoff(mat, 1e3, 2e3, Q), 0.0, atol=1e-9)
+
+
+def test_polar_matches_svd(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        F = np.eye(3) + 0.4 * rng.normal(size=(3, 3))
+        if np.linalg.det(F) <= 0.05:
+            continue
+        R, S, it = O.polar(F)
+        U, _, Vt = np.linalg.svd(F)
+        np.testing.assert_allclose(R, U @ Vt, atol=1e-12)
+        np.testing.assert_allclose(R @ S, F, atol=1e-12)
+        assert np.all(np.linalg.eigvalsh(S) > 0)
+
+
+def test_affine_field_reproduced_exactly(oracle_lib):
+    """Particles carrying v = v0 + M x with C = M (M not symmetric), no stress,
+    no gravity: APIC transfers give back v' = v0 + M x and C' = M exactly."""
+    O = oracle_lib
+    rng = np.random.default_rng(1)
+    n = 400
+    prm, x = scene(O, rng, n=n, g=(0, 0, 0), bc=0)
+    M = np.array([[0.1, 0.7, -0.2], [-0.3, 0.05, 0.4], [0.25, -0.6, -0.15]])
+    v0 = np.array([0.2, -0.5, 0.3])
+    v = v0 + x @ M.T
+    C = np.tile(M, (n, 1, 1))
+    out = O.mls_run(prm, x, v, C, None, 1e-3, 0.0, 1)
+    np.testing.assert_allclose(out["v"], v, atol=1e-10)
+    np.testing.assert_allclose(out["C"], C, atol=1e-9)
+    # and x' = x + dt v'
+    np.testing.assert_allclose(out["x"], x + 1e-3 * out["v"], atol=1e-14)
+
+
+def test_mass_and_momentum_conservation(oracle_lib):
+    """Partition of unity conserves grid mass; with no gravity and particles
+    away from the walls linear momentum is conserved through P2G -> G2P, stress
+    included (sum_i w_ip (x_i - x_p) = 0)."""
+    O = oracle_lib
+    rng = np.random.default_rng(2)
+    n = 300
+    for mat in (0, 1):
+        prm, x = scene(O, rng, n=n, material=mat, g=(0, 0, 0), bc=0)
+        v = rng.normal(0, 0.3, (n, 3))
+        F = np.eye(3) + 0.05 * rng.normal(size=(n, 3, 3))
+        mass = 1e-3 * (0.5 + rng.random(n))
+        out = O.mls_run(prm, x, v, None, F, mass, 1e4, 1, want_grid=True)
+        assert abs(out["grid_m"].sum() - mass.sum()) < 1e-14
+        np.testing.assert_allclose((mass[:, None] * out["v"]).sum(0), (mass[:, None] * v).sum(0), atol=1e-13)
+
+
+def test_free_fall_and_sticky_walls(oracle_lib):
+    O = oracle_lib
+    rng = np.random.default_rng(3)
+    n = 50
+    prm, x = scene(O, rng, n=n, g=(0, -9.8, 0), bc=3)
+    v = np.tile([0.1, -1.0, 0.2], (n, 1))
+    out = O.mls_run(prm, x, v, None, None, 1e-3, 0.0, 1, want_grid=True)
+    np.testing.assert_allclose(out["v"], v + np.array([0, -9.8e-3, 0]), atol=1e-12)
+    np.testing.assert_allclose(out["F"], np.tile(np.eye(3), (n, 1, 1)), atol=1e-12)
+    # a particle inside the 3-node wall band: its band nodes have zero velocity
+    xw = np.array([[2.2 * DX, 0.5, 0.5]])
+    out = O.mls_run(prm, xw, np.array([[-1.0, 0, 0]]), None, None, 1e-3, 0.0, 1, want_grid=True)
+    gv = out["grid_v"].reshape(16, 16, 16, 3)  # [k][j][i]
+    assert np.all(gv[:, :, :3] == 0.0)
+    assert np.any(gv[:, :, 3:] != 0.0)
+
+
+@pytest.mark.parametrize("material", [0, 1])
+def test_adjoint_step_matches_central_differences(oracle_lib, material):
+    O = oracle_lib
+    rng = np.random.default_rng(10 + material)
+    n = 48
+    prm, x = scene(O, rng, n=n, material=material)
+    v = rng.normal(0, 0.5, (n, 3))
+    C = rng.normal(0, 0.3, (n, 3, 3))
+    F = np.eye(3) + rng.normal(0, 0.05, (n, 3, 3))
+    mass = np.full(n, 1e-3)
+    E = 1e3 * np.exp(rng.random(n))
+    W = [rng.normal(size=s) for s in ((n, 3), (n, 3), (n, 3, 3), (n, 3, 3))]
+
+    def loss(x_, v_, C_, F_, E_):
+        o = O.mls_run(prm, x_, v_, C_, F_, mass, E_, 1)
+        return sum((w * o[k]).sum() for w, k in zip(W, ("x", "v", "C", "F")))
+
+    adj = O.mls_adjoint_step(prm, x, v, C, F, mass, E, *W)
+    args = {"x": x, "v": v, "C": C, "F": F, "E": E}
+    for pi, name in enumerate(("x", "v", "C", "F", "E")):
+        for _ in range(4):
+            arr = args[name]
+            idx = tuple(int(rng.integers(0, s)) for s in arr.shape)
+            h = 1e-6 * max(1.0, abs(arr[idx]))
+            ap, am = arr.copy(), arr.copy()
+            ap[idx] += h
+            am[idx] -= h
+            kp = {k + "_": (ap if k == name else a) for k, a in args.items()}
+            km = {k + "_": (am if k == name else a) for k, a in args.items()}
+            fd = (loss(**kp) - loss(**km)) / (2 * h)
+            got = adj[pi][idx]
+            assert abs(fd - got) <= 1e-5 * max(1.0, abs(fd)), (name, idx, fd, got)
