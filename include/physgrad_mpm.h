@@ -102,6 +102,22 @@ int pg_mpm_step(pg_mpm_ctx* ctx, int nsteps, void* stream);
 /* Simulated time and step count (reference getCurrentTime/getStepCount). */
 int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);
 
+/* Differentiable rollout (MLS-MPM, tiled implementation).
+ * pg_mpm_forward_checkpointed advances nsteps steps like pg_mpm_step and keeps
+ * the state every checkpoint_every steps in device memory (the reference's
+ * CheckpointStrategy::UNIFORM, src/differentiable_mpm.h:63-68).
+ * pg_mpm_backward then runs the exact reverse of those steps, segment by
+ * segment from the last checkpoint: each segment is recomputed into a tape,
+ * then reversed (G2P, grid update with sticky walls, P2G with elasticity).
+ * Inputs: adjoints dL/d(final x, v, C, F) in the caller's particle order
+ * (device arrays [3n],[3n],[9n],[9n]; NULL = zero).  Outputs: dL/d(initial
+ * x, v, C, F) and dL/dE per particle ([n]); NULL outputs are skipped.  The
+ * reference's computeGradients (src/differentiable_mpm.h:772-783) reverses an
+ * empty tape; this is the working replacement. */
+int pg_mpm_forward_checkpointed(pg_mpm_ctx* ctx, int nsteps, int checkpoint_every, void* stream);
+int pg_mpm_backward(pg_mpm_ctx* ctx, const float* ax, const float* av, const float* aC, const float* aF, float* gx,
+                    float* gv, float* gC, float* gF, float* gE, void* stream);
+
 /* Per-kernel timing with HIP events recorded on the launch stream around each
  * kernel (replaces the cudaEventRecord timing of MPMSolver::step,
  * src/mpm_solver.h:236-284, and its MPMPerformanceMetrics :72-101).

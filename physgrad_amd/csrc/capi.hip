@@ -9,6 +9,7 @@
 #include <new>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/physgrad_mpm.h"
 #include "pg_kernels.h"
@@ -37,8 +38,15 @@ struct pg_mpm_ctx {
     float4 *mp = nullptr, *gv = nullptr;
     pg::TiledWork tw{};
     pg::KernelProf prof;
-    void* allocs[32] = {};
-    int nallocs = 0;
+    std::vector<void*> allocs;
+    // differentiation: checkpoints every ck_every steps, a segment tape, adjoint buffers
+    int ck_every = 0, ck_steps = 0, ck_cap = 0, tape_len = 0;
+    long long version = 0, ck_version = -1;
+    std::vector<pg::TapeStep> ckpt;   // S, pid, E, mass used
+    std::vector<pg::TapeStep> tape;
+    std::vector<void*> ck_allocs, tape_allocs;
+    float4 *abar = nullptr, *part = nullptr, *agv = nullptr, *apam = nullptr;
+    float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
 };
 
 namespace {
@@ -62,7 +70,7 @@ int dalloc(pg_mpm_ctx* c, T** out, size_t count) {
         (void)hipGetLastError();
         return fail(c, PG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     }
-    c->allocs[c->nallocs++] = p;
+    c->allocs.push_back(p);
     *out = static_cast<T*>(p);
     return PG_OK;
 }
@@ -194,7 +202,8 @@ int pg_mpm_create(const pg_mpm_params* p, pg_mpm_ctx** out) {
                        (rc = dalloc(c, &w.active, 4 * (size_t)w.tile_slots)) || (rc = dalloc(c, &w.nactive, 1)) ||
                        (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
                        (rc = dalloc(c, &w.perm, c->cap)) ||
-                       (rc = dalloc(c, &w.flags, (size_t)w.gbx * w.gby * w.gbz))))) {
+                       (rc = dalloc(c, &w.flags, (size_t)w.gbx * w.gby * w.gbz)) ||
+                       (rc = dalloc(c, &w.bsum, 2 * (size_t)((w.tile_slots + 1023) / 1024)))))) {
             std::string m = c->err;
             pg_mpm_destroy(c);
             return fail(nullptr, rc, m);
@@ -221,7 +230,9 @@ int pg_mpm_destroy(pg_mpm_ctx* c) {
     hipDeviceSynchronize();
     c->prof.flush();
     for (int i = 0; i < c->prof.npool; ++i) hipEventDestroy(c->prof.pool[i]);
-    for (int i = 0; i < c->nallocs; ++i) hipFree(c->allocs[i]);
+    for (void* p : c->allocs) hipFree(p);
+    for (void* p : c->ck_allocs) hipFree(p);
+    for (void* p : c->tape_allocs) hipFree(p);
     delete c;
     return PG_OK;
 }
@@ -248,6 +259,7 @@ int pg_mpm_set_particles(pg_mpm_ctx* c, int n, const float* x, const float* v, c
     c->n = n;
     c->time = 0.0;
     c->steps = 0;
+    c->version++;
     if (c->p.mode == PG_MODE_REF_COMPAT) {
         if ((rc = put(c, c->rx, x, 12 * (size_t)n, memory, st)) || (rc = put(c, c->rv, v, 12 * (size_t)n, memory, st)))
             return rc;
@@ -418,6 +430,7 @@ int pg_mpm_step(pg_mpm_ctx* c, int nsteps, void* stream) {
     if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_step");
     c->time += (double)c->p.dt * nsteps;
     c->steps += nsteps;
+    c->version++;
     return PG_OK;
 }
 
@@ -436,6 +449,194 @@ int pg_mpm_kernel_time(pg_mpm_ctx* c, int kernel, double* total_ms, long long* l
     if (e != hipSuccess) return hip_fail(c, e, "profiling flush");
     if (total_ms) *total_ms = c->prof.total_ms[kernel];
     if (launches) *launches = c->prof.count[kernel];
+    return PG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- differentiation
+namespace {
+
+template <typename T>
+int valloc(pg_mpm_ctx* c, std::vector<void*>& owner, T** out, size_t count, bool zero) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, PG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes for the differentiation "
+                                     "buffers failed: " + hipGetErrorString(e));
+    }
+    if (zero && hipMemset(p, 0, bytes) != hipSuccess) return fail(c, PG_ERR_HIP, "hipMemset failed");
+    owner.push_back(p);
+    *out = static_cast<T*>(p);
+    return PG_OK;
+}
+
+void free_all(std::vector<void*>& v) {
+    for (void* p : v) hipFree(p);
+    v.clear();
+}
+
+hipError_t copy_planes(float4* dst, const float4* src, size_t cap, int n, int planes, hipStream_t st) {
+    for (int p = 0; p < planes; ++p) {
+        hipError_t e = hipMemcpyAsync(dst + p * cap, src + p * cap, (size_t)n * sizeof(float4),
+                                      hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t copy_u32(uint32_t* dst, const uint32_t* src, int n, hipStream_t st) {
+    return hipMemcpyAsync(dst, src, (size_t)n * 4, hipMemcpyDeviceToDevice, st);
+}
+
+int ensure_checkpoints(pg_mpm_ctx* c, int ncp) {
+    if ((int)c->ckpt.size() >= ncp && (!c->have_E || c->ckpt[0].E) && (!c->have_mass || c->ckpt[0].mass))
+        return PG_OK;
+    hipDeviceSynchronize();
+    free_all(c->ck_allocs);
+    c->ckpt.assign(ncp, pg::TapeStep{});
+    int rc;
+    for (auto& k : c->ckpt) {
+        if ((rc = valloc(c, c->ck_allocs, &k.S, 6 * c->cap, false)) ||
+            (rc = valloc(c, c->ck_allocs, &k.pid, c->cap, false)) ||
+            (c->have_E && (rc = valloc(c, c->ck_allocs, &k.E, c->cap, false))) ||
+            (c->have_mass && (rc = valloc(c, c->ck_allocs, &k.mass, c->cap, false)))) {
+            c->ckpt.clear();
+            free_all(c->ck_allocs);
+            return rc;
+        }
+    }
+    return PG_OK;
+}
+
+int ensure_tape(pg_mpm_ctx* c, int K) {
+    if (c->tape_len >= K + 1 && c->abar && (!c->have_E || c->tape[0].E) && (!c->have_mass || c->tape[0].mass))
+        return PG_OK;
+    hipDeviceSynchronize();
+    free_all(c->tape_allocs);
+    c->tape.assign(K + 1, pg::TapeStep{});
+    c->tape_len = 0;
+    const size_t G = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    const pg::TiledWork& w = c->tw;
+    const size_t nblk = (size_t)w.gbx * w.gby * w.gbz;
+    auto& A = c->tape_allocs;
+    int rc;
+    for (auto& t : c->tape) {
+        if ((rc = valloc(c, A, &t.S, 6 * c->cap, false)) || (rc = valloc(c, A, &t.pid, c->cap, false)) ||
+            (c->have_E && (rc = valloc(c, A, &t.E, c->cap, false))) ||
+            (c->have_mass && (rc = valloc(c, A, &t.mass, c->cap, false))) ||
+            (rc = valloc(c, A, &t.perm, c->cap, false)) || (rc = valloc(c, A, &t.src, c->cap, false)) ||
+            (rc = valloc(c, A, &t.alist, 4 * (size_t)w.tile_slots, false)) ||
+            (rc = valloc(c, A, &t.nactive, 1, true)) || (rc = valloc(c, A, &t.mp, G, true)) ||
+            (rc = valloc(c, A, &t.gv, G, true)) || (rc = valloc(c, A, &t.flags, nblk, true)))
+            goto fail;
+    }
+    if ((rc = valloc(c, A, &c->abar, 6 * c->cap, false)) || (rc = valloc(c, A, &c->part, 3 * c->cap, false)) ||
+        (rc = valloc(c, A, &c->agv, G, true)) || (rc = valloc(c, A, &c->apam, G, true)) ||
+        (rc = valloc(c, A, &c->aE, c->cap, false)) || (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
+        (rc = valloc(c, A, &c->uv, 3 * c->cap, false)) || (rc = valloc(c, A, &c->uC, 9 * c->cap, false)) ||
+        (rc = valloc(c, A, &c->uF, 9 * c->cap, false)))
+        goto fail;
+    c->tape_len = K + 1;
+    return PG_OK;
+fail:
+    c->tape.clear();
+    free_all(c->tape_allocs);
+    c->abar = c->part = c->agv = c->apam = nullptr;
+    c->aE = c->ux = c->uv = c->uC = c->uF = nullptr;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
+        return fail(c, PG_ERR_INVALID, "checkpointed rollouts need mode MLS_MPM with the tiled implementation");
+    if (nsteps <= 0 || checkpoint_every <= 0) return fail(c, PG_ERR_INVALID, "nsteps and checkpoint_every must be > 0");
+    const int K = checkpoint_every;
+    const int ncp = (nsteps + K - 1) / K;
+    int rc;
+    if ((rc = ensure_checkpoints(c, ncp)) || (rc = ensure_tape(c, K))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    pg::MlsGeom g = mls_geom(c);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < ncp && e == hipSuccess; ++k) {
+        pg::TapeStep& ck = c->ckpt[k];
+        e = copy_planes(ck.S, c->S, c->cap, c->n, 6, st);
+        if (e == hipSuccess) e = copy_u32(ck.pid, c->pid, c->n, st);
+        if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(ck.E, c->E, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && c->have_mass)
+            e = hipMemcpyAsync(ck.mass, c->mass, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+        const int L = std::min(K, nsteps - k * K);
+        float* E = c->have_E ? c->E : nullptr;
+        float* E2 = c->have_E ? c->E2 : nullptr;
+        float* m = c->have_mass ? c->mass : nullptr;
+        float* m2 = c->have_mass ? c->mass2 : nullptr;
+        if (e == hipSuccess)
+            e = pg::tiled_steps(g, c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv, L, st);
+        if (c->have_E) { c->E = E; c->E2 = E2; }
+        if (c->have_mass) { c->mass = m; c->mass2 = m2; }
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_forward_checkpointed");
+    c->time += (double)c->p.dt * nsteps;
+    c->steps += nsteps;
+    c->version++;
+    c->ck_version = c->version;
+    c->ck_steps = nsteps;
+    c->ck_every = K;
+    return PG_OK;
+}
+
+int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float* aC, const float* aF, float* gx,
+                    float* gv, float* gC, float* gF, float* gE, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->ck_version != c->version || c->ck_steps <= 0)
+        return fail(c, PG_ERR_STATE, "pg_mpm_backward needs the state of the last pg_mpm_forward_checkpointed "
+                                     "(the particles were changed or stepped since)");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = c->n, K = c->ck_every;
+    const int ncp = (c->ck_steps + K - 1) / K;
+    pg::MlsGeom g = mls_geom(c);
+    hipError_t e = hipSuccess;
+    auto seed = [&](float* dst, const float* src, size_t count) {
+        if (e != hipSuccess) return;
+        if (src) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToDevice, st);
+        else e = pg::fill_u32(reinterpret_cast<uint32_t*>(dst), count, 0u, st);
+    };
+    seed(c->ux, ax, 3 * (size_t)n);
+    seed(c->uv, av, 3 * (size_t)n);
+    seed(c->uC, aC, 9 * (size_t)n);
+    seed(c->uF, aF, 9 * (size_t)n);
+    seed(c->aE, nullptr, (size_t)n);
+    for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
+        const int L = std::min(K, c->ck_steps - k * K);
+        pg::TapeStep* T = c->tape.data();
+        const pg::TapeStep& ck = c->ckpt[k];
+        e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
+        if (e == hipSuccess) e = copy_u32(T[0].pid, ck.pid, n, st);
+        if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(T[0].E, ck.E, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && c->have_mass)
+            e = hipMemcpyAsync(T[0].mass, ck.mass, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = pg::tiled_tape_forward(g, c->tw, n, c->cap, T, L, st);
+        if (e == hipSuccess) e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, st);
+        if (e == hipSuccess)
+            e = pg::adjoint_segment(g, c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, st);
+        if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, st);
+    }
+    auto out = [&](float* dst, const float* src, size_t count) {
+        if (e == hipSuccess && dst) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToDevice, st);
+    };
+    out(gx, c->ux, 3 * (size_t)n);
+    out(gv, c->uv, 3 * (size_t)n);
+    out(gC, c->uC, 9 * (size_t)n);
+    out(gF, c->uF, 9 * (size_t)n);
+    out(gE, c->aE, (size_t)n);
+    if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_backward");
     return PG_OK;
 }
 

@@ -1,62 +1,39 @@
 // MLS-MPM tiled path: cell-binned particles with LDS-staged transfers.
 //
-// Particles are binned every step by the tile (T^3 block of base cells) of
-// their stencil base; a counting sort (per-tile atomic rank, one-workgroup
-// scan, scatter) gives perm = sorted position -> storage slot.  One workgroup
-// processes one tile at a time:
-//   G2P  reads the tile's (T+2)^3 node window of grid velocity into LDS and
-//        gathers from it;
-//   P2G  accumulates the updated particles into an LDS window with LDS float
-//        atomics, then flushes the window with one global atomic per node and
-//        channel (instead of one per particle, node and channel).
-// G2P of step n and P2G of step n+1 are fused (the particle's new state never
-// leaves registers).  The updated state is written at the particle's sorted
-// position, so storage stays sorted by tile and the next step's gathers are
-// nearly coalesced; pid carries the user's particle index along.  A particle
-// whose stencil leaves the tile window (it crossed a tile face this step)
-// falls back to global atomics, so correctness never depends on the binning.
+// Binning.  Particles are binned every step by the tile (TT^3 block of
+// stencil-base cells, Morton-numbered) of their stencil base.  The fused
+// kernel that moves a particle also counts it into its new tile: per
+// workgroup, counts go to 27 LDS counters (one per neighbour direction) and
+// one global atomic per (tile, direction) reserves the ranks, so there is no
+// per-particle global atomic.  A two-kernel scan turns the tile counts into
+// starts and the list of non-empty tiles, and a scatter builds
+// perm = sorted position -> storage slot.
 //
-// The grid update runs over 8^3 node blocks flagged by the P2G flush; it
-// writes (v, m) and clears the accumulator, so there is no dense grid clear.
-#include "pg_device.h"
-#include "pg_kernels.h"
+// Transfers.  One workgroup processes one tile at a time.  Inside the tile
+// the particles are ordered by stencil-base cell with an LDS counting sort
+// (64 cells), then processed 256 at a time:
+//   G2P  gathers the 27 node velocities from the tile's (TT+2)^3 window of
+//        grid velocity held in LDS;
+//   P2G  writes the particle's contributions to a per-wave scratch and
+//        reduces them across the wave (wave_slice_reduce): runs of particles
+//        sharing a cell become one LDS atomic per node and channel, and the
+//        window is flushed with one global atomic per node and channel.
+// G2P of step n and P2G of step n+1 are fused (the new state never leaves
+// registers) and the new state is written at the particle's cell-sorted
+// position, so storage stays sorted and the next step's gathers coalesce;
+// pid carries the user's particle index.  A particle whose stencil leaves the
+// window (it crossed a tile face this step) scatters with global atomics, so
+// correctness never depends on the binning.
+//
+// Grid update.  Runs over 8^3 node blocks flagged by the P2G flush; writes
+// (v, m) and clears the accumulator, so there is no dense grid clear.
+#include "pg_tile.h"
 
 namespace pg {
 namespace {
 
-constexpr int TT = 4;            // tile edge (cells)
-constexpr int WN = TT + 2;       // window edge (nodes)
-constexpr int WV = WN * WN * WN; // window volume (216)
-constexpr int TB = 256;          // threads per tile workgroup
-constexpr int GB = 8;            // grid-update block edge (nodes)
-
-__device__ __forceinline__ uint32_t expand3(uint32_t v) {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
-__device__ __forceinline__ uint32_t compact3(uint32_t v) {
-    v &= 0x49249249u;
-    v = (v ^ (v >> 2)) & 0xC30C30C3u;
-    v = (v ^ (v >> 4)) & 0x0F00F00Fu;
-    v = (v ^ (v >> 8)) & 0xFF0000FFu;
-    v = (v ^ (v >> 16)) & 0x000003FFu;
-    return v;
-}
-
-__device__ __forceinline__ uint32_t tile_key(const float* x, const MlsGeom& g, int ntx, int nty, int ntz) {
-    int t[3];
-    const int nt[3] = {ntx, nty, ntz};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        int b = (int)floorf((x[a] - g.o[a]) * g.inv_dx - 0.5f);
-        int tc = b >= 0 ? b / TT : 0;
-        t[a] = tc < nt[a] ? tc : nt[a] - 1;
-    }
-    return expand3((uint32_t)t[0]) | (expand3((uint32_t)t[1]) << 1) | (expand3((uint32_t)t[2]) << 2);
-}
+constexpr int PASS = 2048;       // particles sorted per LDS pass of a tile
+constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 
 struct TileArgs {
     const float4* Sin;
@@ -73,6 +50,7 @@ struct TileArgs {
     uint32_t* key_out;
     uint32_t* rank_out;
     uint32_t* count;
+    uint32_t* src_out;       // output position -> input slot (segment tapes only)
     const float4* gv;
     float4* mp;
     uint32_t* flags;
@@ -85,178 +63,291 @@ __device__ __forceinline__ void flag_node(uint32_t* flags, const TileArgs& a, in
     flags[((gk / GB) * a.gby + gj / GB) * a.gbx + gi / GB] = 1u;
 }
 
-// MODE bit 1: G2P (reads x, F; writes the new state at the sorted position)
-// MODE bit 2: P2G (scatters the current/updated state into the grid accumulator)
+__device__ __forceinline__ uint32_t morton_tile(int tx, int ty, int tz) {
+    return expand3((uint32_t)tx) | (expand3((uint32_t)ty) << 1) | (expand3((uint32_t)tz) << 2);
+}
+
+// tile coordinates of a position, clamped like tile_key
+__device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, int ntx, int nty, int ntz, int* t) {
+    const int nt[3] = {ntx, nty, ntz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int b = (int)floorf((x[a] - g.o[a]) * g.inv_dx - 0.5f);
+        int tc = b >= 0 ? b / TT : 0;
+        t[a] = tc < nt[a] ? tc : nt[a] - 1;
+    }
+}
+
+// MODE bit 1: G2P (reads x, F; writes the new state at the cell-sorted position)
+// MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state)
 template <int MODE>
 __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
     constexpr bool G2P = (MODE & 1) != 0;
     constexpr bool P2G = (MODE & 2) != 0;
-    __shared__ float sv[3][WV];
-    __shared__ float sm[4][WV];
-    const uint32_t nact = *a.nactive;
-    // XCD-aware assignment: workgroups are dispatched round-robin over 8 XCDs;
-    // give each XCD a contiguous (Morton-ordered, hence compact) chunk of tiles
-    // so tiles sharing window nodes share an L2.
-    const uint32_t nx8 = 8;
-    const uint32_t xcd = blockIdx.x % nx8;
-    const uint32_t per_xcd = (gridDim.x + nx8 - 1) / nx8;
-    const uint32_t lane = blockIdx.x / nx8;
-    const uint32_t c0 = (uint32_t)(((uint64_t)nact * xcd) / nx8);
-    const uint32_t c1 = (uint32_t)(((uint64_t)nact * (xcd + 1)) / nx8);
+    constexpr bool SORT = P2G;           // local cell order matters for the P2G reduction
+    constexpr int NP = 36;               // 9 nodes x 4 channels per slice
+    __shared__ float4 sv[WV];            // grid (v, m) window
+    __shared__ float sm[4 * WV];         // P2G accumulation window, channel-major
+    __shared__ float scr[TB / 64][64 * NP];
+    __shared__ uint16_t lperm[PASS];     // local cell-sorted order within the pass
+    __shared__ uint16_t ldir[PASS];      // (direction << 11 | rank) of the new tile, or FAR
+    __shared__ uint32_t lcnt[65], lstart[65];
+    __shared__ uint32_t dcnt[27], dbase[27];
+
+    const XcdChunk ch(*a.nactive);
     const float s4 = 4.f * g.inv_dx * g.inv_dx;
     const float kscale = -g.dt * g.p_vol * s4;
-    const int tid = threadIdx.x;
+    const float emu = 1.f / (2.f * (1.f + g.nu));
+    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float* myscr = scr[wave];
 
-    for (uint32_t ti = c0 + lane; ti < c1; ti += per_xcd) {
+    for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
         const uint32_t tile = te.x, beg = te.y, cnt = te.z;
-        const int t0[3] = {(int)compact3(tile) * TT, (int)compact3(tile >> 1) * TT, (int)compact3(tile >> 2) * TT};
+        int t0[3];
+        tile_origin(tile, t0);
+        const int tc[3] = {t0[0] / TT, t0[1] / TT, t0[2] / TT};
         for (int l = tid; l < WV; l += TB) {
             if (G2P) {
                 int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
                 float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gi < g.nx && gj < g.ny && gk < g.nz) u = a.gv[((size_t)gk * g.ny + gj) * g.nx + gi];
-                sv[0][l] = u.x; sv[1][l] = u.y; sv[2][l] = u.z;
+                if (gi < g.nx && gj < g.ny && gk < g.nz) u = a.gv[node_id(g, gi, gj, gk)];
+                sv[l] = u;
             }
-            if (P2G) { sm[0][l] = 0.f; sm[1][l] = 0.f; sm[2][l] = 0.f; sm[3][l] = 0.f; }
+            if (P2G) { sm[l] = 0.f; sm[WV + l] = 0.f; sm[2 * WV + l] = 0.f; sm[3 * WV + l] = 0.f; }
         }
-        __syncthreads();
 
-        for (uint32_t r = tid; r < cnt; r += TB) {
-            const uint32_t i = beg + r;
-            const uint32_t s = a.perm[i];
-            float x[3], F[9], v[3], C[9];
-            float Eq = g.E, mq = g.p_mass;
-            if (G2P) {
-                load_xF(a.Sin, a.cap, s, x, F);
-                Stencil st;
-                make_stencil(x, g.o, g.inv_dx, st);
-                const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-                const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
+        for (uint32_t pb = 0; pb < cnt; pb += PASS) {
+            const uint32_t pc = min((uint32_t)PASS, cnt - pb);
+            // ---- local counting sort of the pass by stencil-base cell (64 cells + "outside")
+            if (SORT) {
+                if (tid < 65) lcnt[tid] = 0;
+                if (tid < 27) dcnt[tid] = 0;
+                __syncthreads();
+                uint32_t key[PASS / TB];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = 0.f;
+                for (int k = 0; k < PASS / TB; ++k) {
+                    const uint32_t r = tid + k * TB;
+                    key[k] = 0xFFFFFFFFu;
+                    if (r < pc) {
+                        const float4 q = a.Sin[a.perm[beg + pb + r]];
+                        const float xx[3] = {q.x, q.y, q.z};
+                        int cell = 64;
+                        int l3[3];
+                        bool in = true;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) C[c] = 0.f;
+                        for (int d = 0; d < 3; ++d) {
+                            l3[d] = (int)floorf((xx[d] - g.o[d]) * g.inv_dx - 0.5f) - t0[d];
+                            in = in && l3[d] >= 0 && l3[d] < TT;
+                        }
+                        if (in) cell = l3[0] + TT * l3[1] + TT * TT * l3[2];
+                        const uint32_t rk = atomicAdd(&lcnt[cell], 1u);
+                        key[k] = ((uint32_t)cell << 16) | rk;
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    uint32_t run = 0;
+                    for (int c = 0; c < 65; ++c) {
+                        lstart[c] = run;
+                        run += lcnt[c];
+                    }
+                }
+                __syncthreads();
 #pragma unroll
-                for (int ii = 0; ii < 3; ++ii)
+                for (int k = 0; k < PASS / TB; ++k) {
+                    const uint32_t r = tid + k * TB;
+                    if (r < pc) lperm[lstart[key[k] >> 16] + (key[k] & 0xFFFFu)] = (uint16_t)r;
+                }
+            }
+            __syncthreads();
+
+            // ---- transfers, 256 particles at a time in cell order
+            for (uint32_t c0 = 0; c0 < pc; c0 += TB) {
+                const uint32_t j = c0 + tid;
+                const bool act = j < pc;
+                const uint32_t r = act ? (SORT ? (uint32_t)lperm[j] : j) : 0u;
+                const uint32_t s = act ? a.perm[beg + pb + r] : 0u;
+                const uint32_t o = beg + pb + j;
+                float x[3] = {0.f, 0.f, 0.f}, F[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+                float v[3] = {0.f, 0.f, 0.f}, C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                float Eq = g.E, mq = g.p_mass;
+                if (act) {
+                    if (a.E_in) Eq = a.E_in[s];
+                    if (a.m_in) mq = a.m_in[s];
+                }
+                if (G2P) {
+                    if (act) {
+                        load_xF(a.Sin, a.cap, s, x, F);
+                        Stencil st;
+                        make_stencil(x, g.o, g.inv_dx, st);
+                        const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
+                        const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
 #pragma unroll
-                    for (int jj = 0; jj < 3; ++jj)
+                        for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
-                        for (int kk = 0; kk < 3; ++kk) {
-                            float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                            float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
-                                          ((float)kk - st.f[2]) * g.dx};
-                            float u[3];
-                            if (inwin) {
-                                int l = ((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii);
-                                u[0] = sv[0][l]; u[1] = sv[1][l]; u[2] = sv[2][l];
-                            } else {
-                                int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                                u[0] = u[1] = u[2] = 0.f;
-                                if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) {
-                                    float4 q = a.gv[((size_t)gk * g.ny + gj) * g.nx + gi];
-                                    u[0] = q.x; u[1] = q.y; u[2] = q.z;
+                            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                                for (int kk = 0; kk < 3; ++kk) {
+                                    const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
+                                    const float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
+                                                        ((float)kk - st.f[2]) * g.dx};
+                                    float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
+                                    if (inwin) {
+                                        u = sv[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
+                                    } else {
+                                        const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
+                                        if (in_grid(g, gi, gj, gk)) u = a.gv[node_id(g, gi, gj, gk)];
+                                    }
+                                    const float uv[3] = {u.x, u.y, u.z};
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) {
+                                        const float wu = w * uv[c];
+                                        v[c] += wu;
+#pragma unroll
+                                        for (int e = 0; e < 3; ++e) C[3 * c + e] += wu * d[e];
+                                    }
                                 }
-                            }
 #pragma unroll
-                            for (int c = 0; c < 3; ++c) {
-                                float wu = w * u[c];
-                                v[c] += wu;
+                        for (int c = 0; c < 9; ++c) C[c] *= s4;
+                        float Fn[9];
 #pragma unroll
-                                for (int e = 0; e < 3; ++e) C[3 * c + e] += wu * d[e];
+                        for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                            for (int jj = 0; jj < 3; ++jj)
+                                Fn[3 * ii + jj] = F[3 * ii + jj] + g.dt * (C[3 * ii] * F[jj] + C[3 * ii + 1] * F[3 + jj] +
+                                                                           C[3 * ii + 2] * F[6 + jj]);
+#pragma unroll
+                        for (int c = 0; c < 9; ++c) F[c] = Fn[c];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
+                        store_state(a.Sout, a.cap, o, x, F, v, C);
+                        a.pid_out[o] = a.pid_in[s];
+                        if (a.E_out && a.E_in) a.E_out[o] = Eq;
+                        if (a.m_out && a.m_in) a.m_out[o] = mq;
+                        if (a.src_out) a.src_out[o] = s;
+                        if (P2G) {
+                            int nt[3];
+                            tile_coords(x, g, a.ntx, a.nty, a.ntz, nt);
+                            const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
+                            const uint32_t k = morton_tile(nt[0], nt[1], nt[2]);
+                            a.key_out[o] = k;
+                            if (dx_ >= -1 && dx_ <= 1 && dy_ >= -1 && dy_ <= 1 && dz_ >= -1 && dz_ <= 1) {
+                                const int dir = (dx_ + 1) + 3 * (dy_ + 1) + 9 * (dz_ + 1);
+                                const uint32_t rk = atomicAdd(&dcnt[dir], 1u);
+                                ldir[j] = (uint16_t)((dir << 11) | rk);
+                            } else {
+                                ldir[j] = FAR;
+                                a.rank_out[o] = atomicAdd(a.count + k, 1u);
                             }
                         }
-#pragma unroll
-                for (int c = 0; c < 9; ++c) C[c] *= s4;
-                float Fn[9];
-#pragma unroll
-                for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-                    for (int jj = 0; jj < 3; ++jj)
-                        Fn[3 * ii + jj] = F[3 * ii + jj] + g.dt * (C[3 * ii] * F[jj] + C[3 * ii + 1] * F[3 + jj] +
-                                                                   C[3 * ii + 2] * F[6 + jj]);
-#pragma unroll
-                for (int c = 0; c < 9; ++c) F[c] = Fn[c];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
-                store_state(a.Sout, a.cap, i, x, F, v, C);
-                a.pid_out[i] = a.pid_in[s];
-                if (a.E_in) { Eq = a.E_in[s]; a.E_out[i] = Eq; }
-                if (a.m_in) { mq = a.m_in[s]; a.m_out[i] = mq; }
-                if (P2G) {
-                    uint32_t k = tile_key(x, g, a.ntx, a.nty, a.ntz);
-                    a.key_out[i] = k;
-                    a.rank_out[i] = atomicAdd(a.count + k, 1u);
+                    }
+                } else if (act) {
+                    load_state(a.Sin, a.cap, s, x, F, v, C);
                 }
-            } else {
-                load_state(a.Sin, a.cap, s, x, F, v, C);
-                if (a.E_in) Eq = a.E_in[s];
-                if (a.m_in) mq = a.m_in[s];
-            }
-            if (P2G) {
-                const float mu = Eq / (2.f * (1.f + g.nu));
-                const float lam = Eq * g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
-                float tau[9], A[9];
-                kirchhoff(g.material, mu, lam, F, tau);
+                if (P2G) {
+                    // scatter the (new) state: m_i += w m, p_i += w (m v + A (x_i - x_p))
+                    float A[9];
+                    {
+                        float tau[9];
+                        kirchhoff(g.material, Eq * emu, Eq * elam, F, tau);
 #pragma unroll
-                for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
-                Stencil st;
-                make_stencil(x, g.o, g.inv_dx, st);
-                const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-                const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
-                const float mv[3] = {mq * v[0], mq * v[1], mq * v[2]};
+                        for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
+                    }
+                    Stencil st;
+                    make_stencil(x, g.o, g.inv_dx, st);
+                    const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
+                    const bool inwin = act && l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
+                    // contribution at offset d: w(d) (m, q + a0 d0 + a1 d1 + a2 d2),
+                    // q = m v - dx A f, a_j = dx A[:, j]
+                    float q3[3], a0[3], a1[3], a2[3];
 #pragma unroll
-                for (int ii = 0; ii < 3; ++ii)
+                    for (int c = 0; c < 3; ++c) {
+                        q3[c] = mq * v[c] - g.dx * (A[3 * c] * st.f[0] + A[3 * c + 1] * st.f[1] + A[3 * c + 2] * st.f[2]);
+                        a0[c] = g.dx * A[3 * c];
+                        a1[c] = g.dx * A[3 * c + 1];
+                        a2[c] = g.dx * A[3 * c + 2];
+                    }
+                    if (act && !inwin) {
 #pragma unroll
-                    for (int jj = 0; jj < 3; ++jj)
+                        for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
-                        for (int kk = 0; kk < 3; ++kk) {
-                            float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                            float d0 = ((float)ii - st.f[0]) * g.dx, d1 = ((float)jj - st.f[1]) * g.dx,
-                                  d2 = ((float)kk - st.f[2]) * g.dx;
-                            float c0v = w * mq;
-                            float c1v = w * (mv[0] + A[0] * d0 + A[1] * d1 + A[2] * d2);
-                            float c2v = w * (mv[1] + A[3] * d0 + A[4] * d1 + A[5] * d2);
-                            float c3v = w * (mv[2] + A[6] * d0 + A[7] * d1 + A[8] * d2);
-                            if (inwin) {
-                                int l = ((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii);
-                                atomicAdd(&sm[0][l], c0v);
-                                atomicAdd(&sm[1][l], c1v);
-                                atomicAdd(&sm[2][l], c2v);
-                                atomicAdd(&sm[3][l], c3v);
-                            } else {
-                                int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                                if (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) {
-                                    float* cp = reinterpret_cast<float*>(a.mp + ((size_t)gk * g.ny + gj) * g.nx + gi);
-                                    atomic_add_f32(cp + 0, c0v);
-                                    atomic_add_f32(cp + 1, c1v);
-                                    atomic_add_f32(cp + 2, c2v);
-                                    atomic_add_f32(cp + 3, c3v);
+                            for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                                for (int kk = 0; kk < 3; ++kk) {
+                                    const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
+                                    if (!in_grid(g, gi, gj, gk)) continue;
+                                    const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
+                                    float* cp = reinterpret_cast<float*>(a.mp + node_id(g, gi, gj, gk));
+                                    atomic_add_f32(cp + 0, w * mq);
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c)
+                                        atomic_add_f32(cp + 1 + c, w * (q3[c] + a0[c] * ii + a1[c] * jj + a2[c] * kk));
                                     flag_node(a.flags, a, gi, gj, gk);
                                 }
+                    }
+                    const int base = inwin ? (l2 * WN + l1) * WN + l0 : -1;
+#pragma unroll
+                    for (int ii = 0; ii < 3; ++ii) {
+#pragma unroll
+                        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+                            for (int kk = 0; kk < 3; ++kk) {
+                                float4 cv = make_float4(0.f, 0.f, 0.f, 0.f);
+                                if (inwin) {
+                                    const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
+                                    cv.x = w * mq;
+                                    cv.y = w * (q3[0] + a0[0] * ii + a1[0] * jj + a2[0] * kk);
+                                    cv.z = w * (q3[1] + a0[1] * ii + a1[1] * jj + a2[1] * kk);
+                                    cv.w = w * (q3[2] + a0[2] * ii + a1[2] * jj + a2[2] * kk);
+                                }
+                                *reinterpret_cast<float4*>(&myscr[lane * NP + 4 * (3 * jj + kk)]) = cv;
                             }
-                        }
+                        __builtin_amdgcn_wave_barrier();
+                        wave_slice_reduce<4>(myscr, base, ii, sm);
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- ranks in the new tiles: one global atomic per (tile, direction)
+            if (G2P && P2G) {
+                if (tid < 27) {
+                    const uint32_t nd = dcnt[tid];
+                    if (nd) {
+                        const int dx_ = tid % 3 - 1, dy_ = (tid / 3) % 3 - 1, dz_ = tid / 9 - 1;
+                        const uint32_t k = morton_tile(tc[0] + dx_, tc[1] + dy_, tc[2] + dz_);
+                        dbase[tid] = atomicAdd(a.count + k, nd);
+                    }
+                }
+                __syncthreads();
+                for (uint32_t jj = tid; jj < pc; jj += TB) {
+                    const uint16_t dr = ldir[jj];
+                    if (dr != FAR) a.rank_out[beg + pb + jj] = dbase[dr >> 11] + (dr & 0x7FFu);
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
+
         if (P2G) {
             for (int l = tid; l < WV; l += TB) {
-                float m = sm[0][l];
+                const float m = sm[l];
                 if (m != 0.f) {
-                    int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
+                    const int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
                     if (gi < g.nx && gj < g.ny && gk < g.nz) {
-                        float* cp = reinterpret_cast<float*>(a.mp + ((size_t)gk * g.ny + gj) * g.nx + gi);
+                        float* cp = reinterpret_cast<float*>(a.mp + node_id(g, gi, gj, gk));
                         atomic_add_f32(cp + 0, m);
-                        atomic_add_f32(cp + 1, sm[1][l]);
-                        atomic_add_f32(cp + 2, sm[2][l]);
-                        atomic_add_f32(cp + 3, sm[3][l]);
+                        atomic_add_f32(cp + 1, sm[WV + l]);
+                        atomic_add_f32(cp + 2, sm[2 * WV + l]);
+                        atomic_add_f32(cp + 3, sm[3 * WV + l]);
                     }
                 }
             }
             if (tid < 8) {
-                int bx = t0[0] / GB + (tid & 1), by = t0[1] / GB + ((tid >> 1) & 1), bz = t0[2] / GB + (tid >> 2);
+                const int bx = t0[0] / GB + (tid & 1), by = t0[1] / GB + ((tid >> 1) & 1), bz = t0[2] / GB + (tid >> 2);
                 // the window spans nodes [t0, t0 + WN); flag each 8^3 block it touches
-                bool touch = bx * GB < t0[0] + WN && by * GB < t0[1] + WN && bz * GB < t0[2] + WN;
+                const bool touch = bx * GB < t0[0] + WN && by * GB < t0[1] + WN && bz * GB < t0[2] + WN;
                 if (touch && bx < a.gbx && by < a.gby && bz < a.gbz) a.flags[(bz * a.gby + by) * a.gbx + bx] = 1u;
             }
         }
@@ -264,8 +355,11 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
     }
 }
 
-// Grid update over flagged 8^3 blocks: gv = (p/m + dt g, m) with sticky walls,
-// accumulator cleared for the next P2G.
+// Grid update over flagged 8^3 blocks: gv = (p/m + dt g, m) with sticky walls.
+// Forward: the accumulator and the flag are cleared for the next P2G.  TAPE
+// (segment recompute for the adjoint): both are kept for the reverse pass,
+// which clears them after use.
+template <bool TAPE>
 __global__ void __launch_bounds__(512) k_grid_tiled(MlsGeom g, float4* __restrict__ mp, float4* __restrict__ gv,
                                                     uint32_t* __restrict__ flags, int gbx, int gby) {
     const int b = blockIdx.x;
@@ -273,94 +367,169 @@ __global__ void __launch_bounds__(512) k_grid_tiled(MlsGeom g, float4* __restric
     const int bx = b % gbx, by = (b / gbx) % gby, bz = b / (gbx * gby);
     const int i = bx * GB + (threadIdx.x & 7), j = by * GB + ((threadIdx.x >> 3) & 7), k = bz * GB + (threadIdx.x >> 6);
     if (i < g.nx && j < g.ny && k < g.nz) {
-        size_t id = ((size_t)k * g.ny + j) * g.nx + i;
-        float4 a = mp[id];
-        bool sticky = i < g.bc || i >= g.nx - g.bc || j < g.bc || j >= g.ny - g.bc || k < g.bc || k >= g.nz - g.bc;
-        float4 r = make_float4(0.f, 0.f, 0.f, a.x);
-        if (a.x > 0.f && !sticky) {
-            float im = 1.f / a.x;
-            r.x = a.y * im + g.dt * g.g[0];
-            r.y = a.z * im + g.dt * g.g[1];
-            r.z = a.w * im + g.dt * g.g[2];
+        const size_t id = node_id(g, i, j, k);
+        const float4 q = mp[id];
+        float4 r = make_float4(0.f, 0.f, 0.f, q.x);
+        if (q.x > 0.f && !is_sticky(g, i, j, k)) {
+            const float im = 1.f / q.x;
+            r.x = q.y * im + g.dt * g.g[0];
+            r.y = q.z * im + g.dt * g.g[1];
+            r.z = q.w * im + g.dt * g.g[2];
         }
         gv[id] = r;
-        mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!TAPE) mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) flags[b] = 0u;
+    if (!TAPE) {
+        __syncthreads();
+        if (threadIdx.x == 0) flags[b] = 0u;
+    }
 }
 
-__global__ void k_init_keys(int n, size_t cap, const float4* __restrict__ S, MlsGeom g, int ntx, int nty, int ntz,
+// First binning of a call: tile key per storage slot, ranks from one global
+// atomic per distinct key in each wave (storage is normally tile-sorted
+// already, so a wave holds one or two keys).
+__global__ void k_init_keys(int n, const float4* __restrict__ S, MlsGeom g, int ntx, int nty, int ntz,
                             uint32_t* __restrict__ key, uint32_t* __restrict__ rank, uint32_t* __restrict__ count) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    float4 q = S[s];
-    float x[3] = {q.x, q.y, q.z};
-    uint32_t k = tile_key(x, g, ntx, nty, ntz);
-    key[s] = k;
-    rank[s] = atomicAdd(count + k, 1u);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = s < n;
+    uint32_t k = 0xFFFFFFFFu;
+    if (valid) {
+        const float4 q = S[s];
+        const float x[3] = {q.x, q.y, q.z};
+        k = tile_key(x, g, ntx, nty, ntz);
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint64_t todo = __ballot(valid);
+    uint32_t rk = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t kl = __builtin_amdgcn_readlane(k, leader);
+        const uint64_t peers = __ballot(k == kl) & todo;
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(count + kl, (uint32_t)__popcll(peers));
+        b = __builtin_amdgcn_readlane(b, leader);
+        if (k == kl && valid) rk = b + (uint32_t)__popcll(peers & lt);
+        todo &= ~peers;
+    }
+    if (valid) {
+        key[s] = k;
+        rank[s] = rk;
+    }
 }
 
-// One workgroup: exclusive scan of per-tile counts, compaction of non-empty
-// tiles into the active list, counts cleared for the next binning.
-constexpr int SCAN_T = 1024;
-__global__ void __launch_bounds__(SCAN_T) k_scan(uint32_t* __restrict__ count, uint32_t* __restrict__ start,
-                                                 uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
-                                                 int slots) {
-    __shared__ uint32_t s_sum[SCAN_T];
-    __shared__ uint32_t s_act[SCAN_T];
+// Scan of the tile counts in two kernels over blocks of SCAN_B slots:
+// block sums and active counts, then per-block exclusive scan with the
+// preceding blocks' totals; writes starts, the active tile list, clears counts.
+constexpr int SCAN_T = 256;
+constexpr int SCAN_B = 1024;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
     const int tid = threadIdx.x;
-    const int per = (slots + SCAN_T - 1) / SCAN_T;
-    const int lo = tid * per;
-    const int hi = min(lo + per, slots);
-    uint32_t sum = 0, act = 0;
-    for (int c = lo; c < hi; ++c) {
-        uint32_t v = count[c];
-        sum += v;
-        act += v ? 1u : 0u;
-    }
-    s_sum[tid] = sum;
-    s_act[tid] = act;
+    sh[tid] = v;
     __syncthreads();
     for (int off = 1; off < SCAN_T; off <<= 1) {
-        uint32_t a = tid >= off ? s_sum[tid - off] : 0u;
-        uint32_t b = tid >= off ? s_act[tid - off] : 0u;
+        uint32_t t = tid >= off ? sh[tid - off] : 0u;
         __syncthreads();
-        s_sum[tid] += a;
-        s_act[tid] += b;
+        sh[tid] += t;
         __syncthreads();
     }
-    uint32_t run = s_sum[tid] - sum;
-    uint32_t pos = s_act[tid] - act;
-    for (int c = lo; c < hi; ++c) {
-        uint32_t v = count[c];
-        start[c] = run;
-        if (v) alist[pos++] = make_uint4((uint32_t)c, run, v, 0u);
-        run += v;
-        count[c] = 0u;
+    const uint32_t incl = sh[tid];
+    *total = sh[SCAN_T - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ count, int slots,
+                                                   uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t sh[SCAN_T];
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t s = 0, act = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = base + k < slots ? count[base + k] : 0u;
+        s += v;
+        act += v ? 1u : 0u;
     }
-    if (tid == SCAN_T - 1) *nactive = s_act[tid];
+    uint32_t ts, ta;
+    block_excl_scan(s, sh, &ts);
+    block_excl_scan(act, sh, &ta);
+    if (threadIdx.x == 0) {
+        bsum[2 * blockIdx.x] = ts;
+        bsum[2 * blockIdx.x + 1] = ta;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
+                                                   const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
+                                                   uint4* __restrict__ alist, uint32_t* __restrict__ nactive) {
+    __shared__ uint32_t sh[SCAN_T];
+    // totals of the preceding blocks
+    uint32_t ps = 0, pa = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += SCAN_T) {
+        ps += bsum[2 * b];
+        pa += bsum[2 * b + 1];
+    }
+    uint32_t pre_s, pre_a;
+    block_excl_scan(ps, sh, &pre_s);
+    block_excl_scan(pa, sh, &pre_a);
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t v[4], s = 0, act = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < slots ? count[base + k] : 0u;
+        s += v[k];
+        act += v[k] ? 1u : 0u;
+    }
+    uint32_t ts, ta;
+    uint32_t run = block_excl_scan(s, sh, &ts) + pre_s;
+    uint32_t pos = block_excl_scan(act, sh, &ta) + pre_a;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < slots) {
+            start[base + k] = run;
+            if (v[k]) alist[pos++] = make_uint4((uint32_t)(base + k), run, v[k], 0u);
+            run += v[k];
+            count[base + k] = 0u;
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *nactive = pre_a + ta;
 }
 
 __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
                           const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     perm[start[key[s]] + rank[s]] = (uint32_t)s;
 }
 
-hipError_t bin(TiledWork& w, int n, hipStream_t st) {
+hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32_t* nactive, hipStream_t st) {
     ProfScope ps(w.prof, PK_BIN, st);
-    k_scan<<<1, SCAN_T, 0, st>>>(w.count, w.start, reinterpret_cast<uint4*>(w.active), w.nactive, w.tile_slots);
-    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, w.perm);
+    const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
+    k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
+    k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
+                                    nactive);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
+}
+
+hipError_t bin(TiledWork& w, int n, hipStream_t st) { return bin_into(w, n, w.perm, w.active, w.nactive, st); }
+
+void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
+    a.key_out = w.key;
+    a.rank_out = w.rank;
+    a.count = w.count;
+    a.src_out = nullptr;
+    a.cap = cap;
+    a.ntx = w.ntx; a.nty = w.nty; a.ntz = w.ntz;
+    a.gbx = w.gbx; a.gby = w.gby; a.gbz = w.gbz;
 }
 
 }  // namespace
 
 hipError_t tiled_init_keys(const MlsGeom& g, TiledWork& w, int n, size_t cap, const float4* S, hipStream_t st) {
-    if (n > 0)
-        k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, cap, S, g, w.ntx, w.nty, w.ntz, w.key, w.rank, w.count);
+    (void)cap;
+    if (n > 0) k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, S, g, w.ntx, w.nty, w.ntz, w.key, w.rank, w.count);
     return bin(w, n, st);
 }
 
@@ -370,18 +539,13 @@ hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4
     if (nsteps <= 0) return hipSuccess;
     const int nblk = w.gbx * w.gby * w.gbz;
     TileArgs a;
+    fill_args(a, w, cap);
     a.perm = w.perm;
     a.alist = reinterpret_cast<const uint4*>(w.active);
     a.nactive = w.nactive;
-    a.key_out = w.key;
-    a.rank_out = w.rank;
-    a.count = w.count;
     a.gv = gv;
     a.mp = mp;
     a.flags = w.flags;
-    a.cap = cap;
-    a.ntx = w.ntx; a.nty = w.nty; a.ntz = w.ntz;
-    a.gbx = w.gbx; a.gby = w.gby; a.gbz = w.gbz;
     auto set_io = [&]() {
         a.Sin = S; a.Sout = S2;
         a.pid_in = pid; a.pid_out = pid2;
@@ -397,7 +561,7 @@ hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4
     }
     {
         ProfScope ps(w.prof, PK_GRID, st);
-        k_grid_tiled<<<nblk, 512, 0, st>>>(g, mp, gv, w.flags, w.gbx, w.gby);
+        k_grid_tiled<false><<<nblk, 512, 0, st>>>(g, mp, gv, w.flags, w.gbx, w.gby);
     }
     for (int s = 0; s + 1 < nsteps; ++s) {
         set_io();
@@ -407,7 +571,7 @@ hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4
         }
         {
             ProfScope ps(w.prof, PK_GRID, st);
-            k_grid_tiled<<<nblk, 512, 0, st>>>(g, mp, gv, w.flags, w.gbx, w.gby);
+            k_grid_tiled<false><<<nblk, 512, 0, st>>>(g, mp, gv, w.flags, w.gbx, w.gby);
         }
         e = bin(w, n, st);
         if (e != hipSuccess) return e;
@@ -419,6 +583,59 @@ hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4
         k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(g, a);
     }
     std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
+    return hipGetLastError();
+}
+
+hipError_t tiled_tape_forward(const MlsGeom& g, TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
+                              hipStream_t st) {
+    if (L <= 0) return hipSuccess;
+    const int nblk = w.gbx * w.gby * w.gbz;
+    TileArgs a;
+    fill_args(a, w, cap);
+    if (n > 0) k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, tape[0].S, g, w.ntx, w.nty, w.ntz, w.key, w.rank, w.count);
+    hipError_t e = bin_into(w, n, tape[0].perm, tape[0].alist, tape[0].nactive, st);
+    if (e != hipSuccess) return e;
+    for (int s = 0; s < L; ++s) {
+        TapeStep& cur = tape[s];
+        TapeStep& nxt = tape[s + 1];
+        a.Sin = cur.S;
+        a.pid_in = cur.pid;
+        a.E_in = cur.E;
+        a.m_in = cur.mass;
+        a.perm = cur.perm;
+        a.alist = reinterpret_cast<const uint4*>(cur.alist);
+        a.nactive = cur.nactive;
+        a.gv = cur.gv;
+        if (s == 0) {
+            a.mp = cur.mp;
+            a.flags = cur.flags;
+            ProfScope ps(w.prof, PK_TILE_P2G, st);
+            k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+        }
+        {
+            ProfScope ps(w.prof, PK_GRID, st);
+            k_grid_tiled<true><<<nblk, 512, 0, st>>>(g, cur.mp, cur.gv, cur.flags, w.gbx, w.gby);
+        }
+        a.Sout = nxt.S;
+        a.pid_out = nxt.pid;
+        a.E_out = nxt.E;
+        a.m_out = nxt.mass;
+        a.src_out = cur.src;
+        if (s + 1 < L) {
+            a.mp = nxt.mp;
+            a.flags = nxt.flags;
+            {
+                ProfScope ps(w.prof, PK_TILE_G2P2G, st);
+                k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+            }
+            e = bin_into(w, n, nxt.perm, nxt.alist, nxt.nactive, st);
+            if (e != hipSuccess) return e;
+        } else {
+            // last step of the segment: G2P only, to fix the end state's storage order
+            ProfScope ps(w.prof, PK_TILE_G2P, st);
+            k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+        }
+    }
     return hipGetLastError();
 }
 

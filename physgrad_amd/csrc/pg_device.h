@@ -98,6 +98,80 @@ PG_DEV void kirchhoff(int material, float mu, float lam, const float* F, float* 
     }
 }
 
+// Vector-Jacobian product of tau(F; mu, lam): aF += (d tau/d F)^T : at, and the
+// contributions to dL/dmu, dL/dlam.
+//   neo-Hookean:  d tau = mu (dF F^T + F dF^T) + lam tr(F^-1 dF) I
+//   fixed-corot.: d tau = 2mu (dF F^T + F dF^T - dR F^T - R dF^T) + lam (2J-1) dJ I,
+//                 dR = R [w]x, w = (tr(S) I - S)^-1 axial(R^T dF - dF^T R), S = R^T F
+//                 (differentiating F = R S with R^T dR skew and dS symmetric).
+PG_DEV void kirchhoff_vjp(int material, float mu, float lam, const float* F, const float* at, float* aF,
+                          float& amu, float& alam) {
+    float FiT[9];
+    const float J = inv_t3(F, FiT);
+    const float tr = at[0] + at[4] + at[8];
+    float aFm[9], aTF[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            aFm[3 * i + j] = at[3 * i] * F[j] + at[3 * i + 1] * F[3 + j] + at[3 * i + 2] * F[6 + j];  // at F
+            aTF[3 * i + j] = at[i] * F[j] + at[3 + i] * F[3 + j] + at[6 + i] * F[6 + j];              // at^T F
+        }
+    if (material == MAT_NEO_HOOKEAN) {
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float b = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+                s += at[3 * i + j] * (b - (i == j ? 1.f : 0.f));
+            }
+        amu += s;
+        alam += tr * logf(J);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) aF[i] += mu * (aFm[i] + aTF[i]) + lam * tr * FiT[i];
+    } else {
+        float R[9];
+        polar(F, R);
+        float S[9], M[9], aTR[9];
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                S[3 * i + j] = R[i] * F[j] + R[3 + i] * F[3 + j] + R[6 + i] * F[6 + j];            // R^T F
+                M[3 * i + j] = R[i] * aFm[j] + R[3 + i] * aFm[3 + j] + R[6 + i] * aFm[6 + j];      // R^T (at F)
+                aTR[3 * i + j] = at[i] * R[j] + at[3 + i] * R[3 + j] + at[6 + i] * R[6 + j];        // at^T R
+                float b = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+                float rf = R[3 * i] * F[3 * j] + R[3 * i + 1] * F[3 * j + 1] + R[3 * i + 2] * F[3 * j + 2];
+                s += at[3 * i + j] * 2.f * (b - rf);
+            }
+        amu += s;
+        alam += tr * (J - 1.f) * J;
+        // symmetric part of S, then H = tr(S) I - S
+        const float s01 = 0.5f * (S[1] + S[3]), s02 = 0.5f * (S[2] + S[6]), s12 = 0.5f * (S[5] + S[7]);
+        const float trS = S[0] + S[4] + S[8];
+        float H[9] = {trS - S[0], -s01, -s02, -s01, trS - S[4], -s12, -s02, -s12, trS - S[8]};
+        float Hi[9];
+        inv_t3(H, Hi);  // H symmetric: inverse transpose is the inverse
+        const float b0 = M[7] - M[5], b1 = M[2] - M[6], b2 = M[3] - M[1];
+        const float u0 = Hi[0] * b0 + Hi[1] * b1 + Hi[2] * b2;
+        const float u1 = Hi[3] * b0 + Hi[4] * b1 + Hi[5] * b2;
+        const float u2 = Hi[6] * b0 + Hi[7] * b1 + Hi[8] * b2;
+        // R [u]x
+        float Ru[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            Ru[3 * i + 0] = R[3 * i + 1] * u2 - R[3 * i + 2] * u1;
+            Ru[3 * i + 1] = -R[3 * i] * u2 + R[3 * i + 2] * u0;
+            Ru[3 * i + 2] = R[3 * i] * u1 - R[3 * i + 1] * u0;
+        }
+        const float vol = lam * (2.f * J - 1.f) * J * tr;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) aF[i] += 2.f * mu * (aFm[i] + aTF[i] - aTR[i] - Ru[i]) + vol * FiT[i];
+    }
+}
+
 // Quadratic B-spline stencil of MLS-MPM: base = floor(xs - 1/2), f = xs - base,
 // w0 = (1.5-f)^2/2, w1 = 3/4 - (f-1)^2, w2 = (f-1/2)^2/2.
 struct Stencil {

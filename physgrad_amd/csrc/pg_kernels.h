@@ -60,6 +60,9 @@ enum ProfKernel {
     PK_PLAIN_G2P = 6,
     PK_CLEAR = 7,
     PK_REF_STEP = 8,
+    PK_ADJ_G2P = 9,    // reverse G2P (tiled)
+    PK_ADJ_GRID = 10,  // reverse grid update
+    PK_ADJ_P2G = 11,   // reverse P2G (tiled)
     PK_COUNT = 16
 };
 
@@ -107,8 +110,39 @@ struct TiledWork {
     uint32_t* rank;        // [cap] rank within tile
     uint32_t* perm;        // [cap] sorted position -> storage slot
     uint32_t* flags;       // [gbx*gby*gbz] touched grid blocks
+    uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
     int nblocks_launch;    // persistent grid size of the tile kernels
 };
+
+// One step of a segment tape: the step's input state in its storage order, its
+// binning, and its grid (valid on the flagged 8^3 blocks).
+struct TapeStep {
+    float4* S;          // 6*cap state planes
+    uint32_t* pid;      // storage slot -> user index
+    float* E;           // per-particle E in storage order (or NULL)
+    float* mass;        // per-particle mass in storage order (or NULL)
+    uint32_t* perm;     // sorted position -> storage slot
+    uint32_t* src;      // output position of this step -> storage slot of this step
+    uint32_t* alist;    // uint4 (tile, start, count, 0) per active tile
+    uint32_t* nactive;
+    float4* mp;         // P2G accumulation (m, p); cleared by the reverse pass
+    float4* gv;         // (v, m) after the grid update
+    uint32_t* flags;    // touched blocks; cleared by the reverse pass
+};
+
+// Recompute L steps from tape[0].S (tape[0].pid/E/mass set), filling tape[0..L-1]
+// and tape[L].S/pid/E/mass (the end state in its storage order).
+hipError_t tiled_tape_forward(const MlsGeom& g, TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
+                              hipStream_t st);
+
+// Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
+hipError_t adjoint_segment(const MlsGeom& g, const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
+                           float4* abar, float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st);
+// user-order adjoint arrays (x,v [3n], C,F [9n]; NULL = 0) -> adjoint planes in storage order (pid)
+hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float* ax, const float* av,
+                             const float* aC, const float* aF, float4* abar, hipStream_t st);
+hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,
+                           float* aC, float* aF, hipStream_t st);
 
 hipError_t tiled_init_keys(const MlsGeom& g, TiledWork& w, int n, size_t cap, const float4* S, hipStream_t st);
 // Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on return

@@ -153,8 +153,28 @@ class MPMContext:
     def step(self, nsteps: int = 1):
         check(self._L.pg_mpm_step(self._h, int(nsteps), _stream_ptr()), self._h)
 
+    def forward_checkpointed(self, nsteps: int, checkpoint_every: int = 32):
+        """Advance nsteps steps keeping a checkpoint every `checkpoint_every` steps."""
+        check(self._L.pg_mpm_forward_checkpointed(self._h, int(nsteps), int(checkpoint_every), _stream_ptr()),
+              self._h)
+
+    def backward(self, ax=None, av=None, aC=None, aF=None):
+        """Adjoints of the initial state (and per-particle E) from adjoints of the
+        final state of the last forward_checkpointed.  Returns dict x, v, C, F, E."""
+        n = self.num_particles
+        ins = [None if t is None else _dev_f32(t, c, name).contiguous()
+               for t, c, name in ((ax, 3, "ax"), (av, 3, "av"), (aC, 9, "aC"), (aF, 9, "aF"))]
+        out = {k: torch.empty((n, c), dtype=torch.float32, device="cuda")
+               for k, c in (("x", 3), ("v", 3), ("C", 9), ("F", 9), ("E", 1))}
+        check(self._L.pg_mpm_backward(self._h, *[_dptr(t) for t in ins], _dptr(out["x"]), _dptr(out["v"]),
+                                      _dptr(out["C"]), _dptr(out["F"]), _dptr(out["E"]), _stream_ptr()), self._h)
+        out["C"] = out["C"].reshape(n, 3, 3)
+        out["F"] = out["F"].reshape(n, 3, 3)
+        out["E"] = out["E"].reshape(n)
+        return out
+
     KERNELS = {"tile_p2g": 0, "tile_g2p2g": 1, "tile_g2p": 2, "grid": 3, "bin": 4, "plain_p2g": 5,
-               "plain_g2p": 6, "clear": 7, "ref_step": 8}
+               "plain_g2p": 6, "clear": 7, "ref_step": 8, "adj_g2p": 9, "adj_grid": 10, "adj_p2g": 11}
 
     def set_profiling(self, on: bool):
         check(self._L.pg_mpm_set_profiling(self._h, 1 if on else 0), self._h)

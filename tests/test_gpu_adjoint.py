@@ -1,0 +1,139 @@
+"""GPU adjoint (checkpointed reverse pass) against the FP64 oracle's reverse
+steps and against FP64 central finite differences of the oracle forward.
+
+Tolerances: reverse steps vs the FP64 oracle reverse, inf-norm relative 1e-3
+per field (FP32 recompute and reverse); gradients of a rollout loss vs FP64
+central differences, relative 1e-3 (the bar for gradient accuracy).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def scene(rng, n, material, grid=16, span=0.35):
+    from physgrad_amd.scenes import Scene
+    x = (0.3 + span * rng.random((n, 3))).astype(np.float32)
+    v = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+    E = np.exp(rng.uniform(np.log(5e3), np.log(5e4), n)).astype(np.float32)
+    return Scene("adj", (grid,) * 3, 1 / grid, 2e-4, 1, material, 2e4, 0.3, 1000.0, span ** 3 / n, x, v,
+                 E_particles=E)
+
+
+def oracle_reverse(O, sc, x, v, C, F, E, nsteps, seeds):
+    prm = sc.oracle_params()
+    states = [(x, v, C, F)]
+    for _ in range(nsteps):
+        o = O.mls_run(prm, *states[-1], sc.p_mass, E, 1)
+        states.append((o["x"], o["v"], o["C"].reshape(-1, 9), o["F"].reshape(-1, 9)))
+    adj = seeds
+    aE = np.zeros(len(E))
+    for k in reversed(range(nsteps)):
+        ax, av, aC, aF, e = O.mls_adjoint_step(prm, *states[k], sc.p_mass, E, *adj)
+        adj = (ax, av, aC.reshape(-1, 9), aF.reshape(-1, 9))
+        aE += e
+    return adj, aE
+
+
+@pytest.mark.parametrize("material", [0, 1], ids=["fixed_corotated", "neo_hookean"])
+@pytest.mark.parametrize("nsteps,K", [(1, 1), (7, 3)], ids=["one_step", "three_segments"])
+def test_backward_vs_oracle_reverse(gpu, oracle_lib, material, nsteps, K):
+    from physgrad_amd.solver import MPMContext
+    O = oracle_lib
+    rng = np.random.default_rng(7 + material + 10 * nsteps)
+    n = 512
+    sc = scene(rng, n, material)
+    C = rng.normal(0, 0.3, (n, 9)).astype(np.float32)
+    F = (np.eye(3).reshape(1, 9) + rng.normal(0, 0.03, (n, 9))).astype(np.float32)
+    seeds = [rng.normal(size=(n, c)) for c in (3, 3, 9, 9)]
+    print("material", material, "steps", nsteps, "K", K, flush=True)
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
+    ctx.forward_checkpointed(nsteps, K)
+    got = ctx.backward(*[torch.as_tensor(s, dtype=torch.float32) for s in seeds])
+    torch.cuda.synchronize()
+    want, wE = oracle_reverse(O, sc, sc.x.astype(np.float64), sc.v.astype(np.float64), C.astype(np.float64),
+                              F.astype(np.float64), sc.E_particles.astype(np.float64), nsteps, seeds)
+    for key, w in zip(("x", "v", "C", "F"), want):
+        err = rel(got[key].cpu().numpy().reshape(-1), w.reshape(-1))
+        print(key, err)
+        assert err < 1e-3, (key, err)
+    errE = rel(got["E"].cpu().numpy(), wE)
+    print("E", errE)
+    assert errE < 1e-3
+    ctx.close()
+
+
+def test_backward_requires_forward(gpu):
+    from physgrad_amd import PgError
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(1)
+    sc = scene(rng, 64, 0)
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v)
+    with pytest.raises(PgError):
+        ctx.backward(torch.zeros(64, 3))
+    ctx.forward_checkpointed(2, 1)
+    ctx.step(1)  # state changed since the forward: backward must refuse
+    with pytest.raises(PgError):
+        ctx.backward(torch.zeros(64, 3))
+    ctx.close()
+
+
+def test_autograd_loss_gradient_vs_fd(gpu, oracle_lib):
+    """Centre-of-mass loss after 24 steps (fixed-corotated, per-particle E):
+    d loss / d v0 (shared, 3 values) and d loss / d E (8 sampled particles) via
+    torch.autograd through the GPU adjoint, against FP64 central differences of
+    the oracle (eps 1e-4 on v0, relative 1e-4 on E)."""
+    from physgrad_amd.autograd import physics_simulation
+    from physgrad_amd.solver import MPMContext
+    O = oracle_lib
+    rng = np.random.default_rng(3)
+    n, nsteps = 1024, 24
+    sc = scene(rng, n, 0, grid=32, span=0.2)
+    sc.dt = 1e-4
+    target = np.array([0.6, 0.3, 0.5])
+    v0 = np.array([0.3, -0.5, 0.2])
+    prm = sc.oracle_params()
+
+    def loss64(v0_, E_):
+        vv = np.tile(v0_, (n, 1))
+        o = O.mls_run(prm, sc.x.astype(np.float64), vv, None, None, sc.p_mass, E_, nsteps)
+        d = o["x"].mean(0) - target
+        return float(d @ d)
+
+    ctx = MPMContext(sc.params(impl=1))
+    x0 = torch.as_tensor(sc.x, device="cuda")
+    v0_t = torch.tensor(v0, dtype=torch.float32, device="cuda", requires_grad=True)
+    E_t = torch.as_tensor(sc.E_particles, device="cuda").clone().requires_grad_(True)
+    xN, _, _, _ = physics_simulation(ctx, x0, v0_t.expand(n, 3), E=E_t, num_steps=nsteps, checkpoint_every=8)
+    d = xN.double().mean(0) - torch.tensor(target, device="cuda")
+    loss = (d * d).sum()
+    loss.backward()
+    gv = v0_t.grad.cpu().numpy()
+    gE = E_t.grad.cpu().numpy()
+    E64 = sc.E_particles.astype(np.float64)
+    for a in range(3):
+        h = 1e-4
+        vp, vm = v0.copy(), v0.copy()
+        vp[a] += h
+        vm[a] -= h
+        fd = (loss64(vp, E64) - loss64(vm, E64)) / (2 * h)
+        print("dv0", a, fd, gv[a])
+        assert abs(fd - gv[a]) <= 1e-3 * abs(fd) + 1e-9, (a, fd, gv[a])
+    for p in rng.choice(n, 8, replace=False):
+        h = 1e-4 * E64[p]
+        Ep, Em = E64.copy(), E64.copy()
+        Ep[p] += h
+        Em[p] -= h
+        fd = (loss64(v0, Ep) - loss64(v0, Em)) / (2 * h)
+        print("dE", p, fd, gE[p])
+        assert abs(fd - gE[p]) <= 1e-3 * abs(fd) + 1e-3 * np.abs(gE).max(), (p, fd, gE[p])
+    ctx.close()

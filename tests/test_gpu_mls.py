@@ -86,7 +86,9 @@ def test_tiled_matches_plain_fast_particles(gpu):
     tiled path's out-of-window fallback; both paths must agree."""
     from physgrad_amd.scenes import two_boxes
     sc = two_boxes(n=65536)
-    sc.v *= 20.0  # 40 m/s: 0.26 cells per step
+    # 40 m/s (0.26 cells per step) plus a random part so that C carries real
+    # velocity gradients rather than rounding noise of a uniform translation
+    sc.v = (sc.v * 20.0 + np.random.default_rng(9).normal(0, 4.0, sc.v.shape)).astype(np.float32)
     a = gpu_step(sc, 0, sc.x, sc.v, None, None, None, steps=10)
     b = gpu_step(sc, 1, sc.x, sc.v, None, None, None, steps=10)
     for key in ("x", "v", "C", "F"):
