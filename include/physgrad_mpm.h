@@ -127,6 +127,27 @@ int pg_mpm_backward(pg_mpm_ctx* ctx, const float* ax, const float* av, const flo
 int pg_mpm_set_profiling(pg_mpm_ctx* ctx, int on);
 int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long* launches);
 
+/* The reference's standalone transfer launchers, same arguments and meaning
+ * as launchOptimizedP2G / launchOptimizedG2P / launchFullyFusedG2P2G
+ * (src/mpm_g2p2g_kernels.cu:506-617; kernels :72-271, :277-500) with the
+ * float3/int3 by-value arguments passed as 3-element host arrays.
+ *   positions  device float4[n] (x, y, z, mass); velocities float4[n] (vx, vy, vz, 0)
+ *   grid_mass  device float[G]; grid_momentum / grid_velocity float3[G]
+ *   order      2 (quadratic, 27 nodes) or 3 (cubic, 64 nodes)
+ * P2G accumulates into the caller's grid (no clear).  G2P normalises by the
+ * weight of nodes with mass > 1e-10 and blends v = flip v_old + (1-flip) v_grid.
+ * The fused launch clears the grid, scatters with gravity added to the
+ * particle velocity, forms v = p/m and gathers; the reference's cooperative
+ * grid.sync() phases become stream-ordered kernels. */
+int pg_launch_optimized_p2g(const float* positions, const float* velocities, float* grid_mass, float* grid_momentum,
+                            const float* origin, const float* cell, const int* dims, int n, int order, void* stream);
+int pg_launch_optimized_g2p(float* positions, float* velocities, const float* grid_mass, const float* grid_velocity,
+                            const float* origin, const float* cell, const int* dims, int n, float dt, float flip_ratio,
+                            int order, void* stream);
+int pg_launch_fully_fused_g2p2g(float* positions, float* velocities, float* grid_mass, float* grid_momentum,
+                                float* grid_velocity, const float* origin, const float* cell, const int* dims, int n,
+                                float dt, const float* gravity, float flip_ratio, int order, void* stream);
+
 /* Spatial hash: cell ids (Morton code of the clamped truncated cell coordinate,
  * modulo the cell count), stable sort of particle indices by cell id, per-cell
  * start and count; bit-identical to SpatialHashGrid::buildHashTable.

@@ -81,6 +81,14 @@ void orc_mls_adjoint_step_f64(const orc_mls_params* p, int n,
                               const double* mass, const double* E,
                               double* ax, double* av, double* aC, double* aF, double* aE);
 
+/* ---------------- reference launchers (float4 particles) ---------------- */
+void orc_launch_p2g(const float* pos, const float* vel, float* gm, float* gp, const float* o, const float* h,
+                    const int* d, int n, int order);
+void orc_launch_g2p(float* pos, float* vel, const float* gm, const float* gv, const float* o, const float* h,
+                    const int* d, int n, float dt, float flip, int order);
+void orc_launch_fused(float* pos, float* vel, float* gm, float* gp, float* gv, const float* o, const float* h,
+                      const int* d, int n, float dt, const float* grav, float flip, int order);
+
 /* ---------------- cell keys + counting sort ---------------- */
 unsigned orc_morton3(unsigned x, unsigned y, unsigned z);
 /* particle cell id = morton(clamped truncated coords) % total_cells */
