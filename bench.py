@@ -60,7 +60,7 @@ def cpu_baseline(scene, steps):
     """FP32 port of the same step (oracle), one host core, `steps` steps of the full workload."""
     import numpy as np
     from oracle import oracle as O
-    prm = scene.oracle_params()
+    prm = O.scene_params(scene)
     x = scene.x.copy()
     t0 = time.perf_counter()
     O.mls_run(prm, x, scene.v, None, None, scene.p_mass, scene.E, steps, dtype=np.float32)

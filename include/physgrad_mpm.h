@@ -127,6 +127,56 @@ int pg_mpm_backward(pg_mpm_ctx* ctx, const float* ax, const float* av, const flo
 int pg_mpm_set_profiling(pg_mpm_ctx* ctx, int on);
 int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long* launches);
 
+/* Batched independent scenes in one context (tiled MLS-MPM).  The reference
+ * runs one scene per MPMSolver (src/mpm_solver.h:150-171, one grid and one
+ * particle AoSoA each); a batch concatenates the scenes' particles and grids
+ * and advances all of them with the same launches.  Each scene has its own grid,
+ * cell size, origin, gravity, time step, material and elastic constants;
+ * pg_mpm_get_time reports scene 0's clock.  A batch context accepts every call
+ * a tiled MLS-MPM context
+ * does (step, get_particles, forward_checkpointed, backward, graph, autotune);
+ * get_grid returns the concatenated grids. */
+typedef struct pg_scene_desc {
+    int nx, ny, nz;           /* grid nodes per axis */
+    float dx;                 /* cell size */
+    float ox, oy, oz;         /* grid origin */
+    float dt;                 /* time step */
+    float gx, gy, gz;         /* gravity */
+    int material;             /* pg_material */
+    float youngs_modulus;
+    float poisson_ratio;
+    float density;
+    float particle_volume;
+    int boundary_thickness;
+    int max_particles;        /* this scene's capacity */
+} pg_scene_desc;
+
+int pg_mpm_batch_create(const pg_scene_desc* scenes, int nscenes, pg_mpm_ctx** out);
+/* counts[s] particles of scene s, scenes back to back in the arrays (same
+ * layouts as pg_mpm_set_particles); mass NULL uses each scene's
+ * density * particle_volume, E NULL each scene's youngs_modulus. */
+int pg_mpm_batch_set_particles(pg_mpm_ctx* ctx, const int* counts, const float* x, const float* v, const float* C,
+                               const float* F, const float* mass, const float* E, int memory, void* stream);
+int pg_mpm_num_scenes(const pg_mpm_ctx* ctx);
+
+/* Replay pg_mpm_step's launches from a captured hipGraph (tiled MLS-MPM).  One
+ * graph per step count is captured on first use and replayed afterwards;
+ * set_particles, set_launch and autotune drop the captured graphs.  Replaces
+ * the reference's per-step kernel launches of MPMSolver::step
+ * (src/mpm_solver.h:236-284). */
+int pg_mpm_set_graph(pg_mpm_ctx* ctx, int enable);
+
+/* Workgroups of the persistent tile kernels, and the autotuner that picks them
+ * by timing trial_steps steps per candidate on the current particle set (the
+ * state is restored afterwards).  Results are cached in a text file
+ * ($PG_AUTOTUNE_CACHE, else ~/.cache/physgrad_amd_autotune.txt) keyed by
+ * tile slots, grid blocks, particle count bucket and scene count; use_cache
+ * reads that file first.  The reference fixes its launch shapes at compile
+ * time (KernelConfig, src/mpm_g2p2g_kernels.h:68-75). */
+int pg_mpm_get_launch(const pg_mpm_ctx* ctx, int* tile_workgroups);
+int pg_mpm_set_launch(pg_mpm_ctx* ctx, int tile_workgroups);
+int pg_mpm_autotune(pg_mpm_ctx* ctx, int trial_steps, int use_cache, void* stream);
+
 /* The reference's standalone transfer launchers, same arguments and meaning
  * as launchOptimizedP2G / launchOptimizedG2P / launchFullyFusedG2P2G
  * (src/mpm_g2p2g_kernels.cu:506-617; kernels :72-271, :277-500) with the

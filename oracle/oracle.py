@@ -153,6 +153,12 @@ def mls_params(dims, dx, origin, dt, gravity, bc, material, nu, p_vol) -> MlsPar
                      int(bc), int(material), float(nu), float(p_vol))
 
 
+def scene_params(sc) -> MlsParams:
+    """MlsParams of a scene object with the attributes dims, dx, origin, dt,
+    gravity, bc, material, nu, p_vol (physgrad_amd.scenes.Scene)."""
+    return mls_params(sc.dims, sc.dx, sc.origin, sc.dt, sc.gravity, sc.bc, sc.material, sc.nu, sc.p_vol)
+
+
 def _state(x, v, C, F, dtype):
     n = np.asarray(x).reshape(-1, 3).shape[0]
     x = np.array(x, dtype).reshape(-1).copy()

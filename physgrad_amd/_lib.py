@@ -42,6 +42,19 @@ class PgMpmParams(ctypes.Structure):
                 ("boundary_thickness", ctypes.c_int), ("max_particles", ctypes.c_int)]
 
 
+class PgSceneDesc(ctypes.Structure):
+    """Mirror of struct pg_scene_desc (one scene of a batch context)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_float),
+                ("ox", ctypes.c_float), ("oy", ctypes.c_float), ("oz", ctypes.c_float),
+                ("dt", ctypes.c_float),
+                ("gx", ctypes.c_float), ("gy", ctypes.c_float), ("gz", ctypes.c_float),
+                ("material", ctypes.c_int),
+                ("youngs_modulus", ctypes.c_float), ("poisson_ratio", ctypes.c_float),
+                ("density", ctypes.c_float), ("particle_volume", ctypes.c_float),
+                ("boundary_thickness", ctypes.c_int), ("max_particles", ctypes.c_int)]
+
+
 class PgError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[pg error {code}] {msg}")
@@ -65,6 +78,13 @@ _SIGNATURES = {
     "pg_mpm_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_mpm_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_longlong)]),
+    "pg_mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(PgSceneDesc), ctypes.c_int, ctypes.POINTER(_vp)]),
+    "pg_mpm_batch_set_particles": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "pg_mpm_num_scenes": (ctypes.c_int, [_vp]),
+    "pg_mpm_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pg_mpm_get_launch": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "pg_mpm_set_launch": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pg_mpm_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pg_launch_optimized_p2g": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pg_launch_optimized_g2p": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float,
                                                ctypes.c_float, ctypes.c_int, _vp]),

@@ -36,6 +36,11 @@ struct pg_mpm_ctx {
     uint32_t *pid = nullptr, *pid2 = nullptr;
     float *E = nullptr, *E2 = nullptr, *mass = nullptr, *mass2 = nullptr;
     float4 *mp = nullptr, *gv = nullptr;
+    // scenes (one for pg_mpm_create, many for pg_mpm_batch_create)
+    std::vector<pg::SceneInfo> scenes_h;
+    pg::SceneInfo* scenes_d = nullptr;
+    int nscenes = 1;
+    size_t G = 0;   // grid nodes over all scenes
     pg::TiledWork tw{};
     pg::KernelProf prof;
     std::vector<void*> allocs;
@@ -45,6 +50,12 @@ struct pg_mpm_ctx {
     std::vector<pg::TapeStep> ckpt;   // S, pid, E, mass used
     std::vector<pg::TapeStep> tape;
     std::vector<void*> ck_allocs, tape_allocs;
+    // hipGraph replay of pg_mpm_step sequences: key (nsteps, state buffer) -> executable graph
+    bool use_graph = false;
+    struct GraphEntry { int nsteps; const void* S; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs;
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev[2] = {nullptr, nullptr};
     float4 *abar = nullptr, *part = nullptr, *agv = nullptr, *apam = nullptr;
     float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
 };
@@ -78,22 +89,7 @@ int dalloc(pg_mpm_ctx* c, T** out, size_t count) {
 hipMemcpyKind kind_to_dev(int memory) { return memory == PG_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice; }
 hipMemcpyKind kind_from_dev(int memory) { return memory == PG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice; }
 
-pg::MlsGeom mls_geom(const pg_mpm_ctx* c) {
-    pg::MlsGeom g;
-    const pg_mpm_params& p = c->p;
-    g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
-    g.dx = p.dx; g.inv_dx = 1.0f / p.dx;
-    g.o[0] = p.ox; g.o[1] = p.oy; g.o[2] = p.oz;
-    g.dt = p.dt;
-    g.g[0] = p.gx; g.g[1] = p.gy; g.g[2] = p.gz;
-    g.bc = p.boundary_thickness;
-    g.material = p.material;
-    g.nu = p.poisson_ratio;
-    g.E = p.youngs_modulus;
-    g.p_vol = p.particle_volume;
-    g.p_mass = p.density * p.particle_volume;
-    return g;
-}
+pg::MlsGeom mls_geom(const pg_mpm_ctx* c) { return c->scenes_h[0].g; }
 
 int ceil_log2(int v) {
     int k = 0;
@@ -111,6 +107,196 @@ int put(pg_mpm_ctx* c, void* dst, const void* src, size_t bytes, int memory, hip
         if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
     }
     return PG_OK;
+}
+
+pg_scene_desc scene_from_params(const pg_mpm_params& p) {
+    pg_scene_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.nx = p.nx; d.ny = p.ny; d.nz = p.nz;
+    d.dx = p.dx;
+    d.ox = p.ox; d.oy = p.oy; d.oz = p.oz;
+    d.dt = p.dt;
+    d.gx = p.gx; d.gy = p.gy; d.gz = p.gz;
+    d.material = p.material;
+    d.youngs_modulus = p.youngs_modulus;
+    d.poisson_ratio = p.poisson_ratio;
+    d.density = p.density;
+    d.particle_volume = p.particle_volume;
+    d.boundary_thickness = p.boundary_thickness;
+    d.max_particles = std::max(p.max_particles, 1);
+    return d;
+}
+
+const char* check_scene(const pg_scene_desc& d) {
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1) return "grid dims must be >= 1";
+    if ((size_t)d.nx * d.ny * d.nz > (size_t)1 << 31) return "grid too large";
+    if (!(d.dx > 0.f)) return "cell size must be > 0";
+    if (!(d.dt > 0.f) || !std::isfinite(d.dt)) return "dt must be finite and > 0";
+    if (d.material != PG_MAT_FIXED_COROTATED && d.material != PG_MAT_NEO_HOOKEAN) return "bad material";
+    if (!(d.poisson_ratio > -1.f && d.poisson_ratio < 0.5f)) return "poisson_ratio must be in (-1, 0.5)";
+    if (!(d.particle_volume > 0.f)) return "particle_volume must be > 0";
+    if (d.boundary_thickness < 0) return "boundary_thickness must be >= 0";
+    if (d.max_particles < 0 || d.max_particles > (1 << 30)) return "max_particles out of range";
+    return nullptr;
+}
+
+pg::MlsGeom geom_of(const pg_scene_desc& d) {
+    pg::MlsGeom g;
+    g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
+    g.dx = d.dx; g.inv_dx = 1.0f / d.dx;
+    g.o[0] = d.ox; g.o[1] = d.oy; g.o[2] = d.oz;
+    g.dt = d.dt;
+    g.g[0] = d.gx; g.g[1] = d.gy; g.g[2] = d.gz;
+    g.bc = d.boundary_thickness;
+    g.material = d.material;
+    g.nu = d.poisson_ratio;
+    g.E = d.youngs_modulus;
+    g.p_vol = d.particle_volume;
+    g.p_mass = d.density * d.particle_volume;
+    return g;
+}
+
+// Builds the scene table (tile ids, grid nodes and 8^3 blocks concatenated in
+// scene order) and allocates the MLS-MPM buffers of a context.
+int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
+    const bool tiled = c->p.impl == PG_IMPL_TILED;
+    std::vector<pg::SceneInfo>& S = c->scenes_h;
+    S.assign(ns, pg::SceneInfo{});
+    size_t goff = 0, cap = 0;
+    uint64_t toff = 0, boff = 0;
+    for (int s = 0; s < ns; ++s) {
+        pg::SceneInfo& e = S[s];
+        e.g = geom_of(d[s]);
+        e.goff = goff;
+        e.ntx = (d[s].nx + 3) / 4;
+        e.nty = (d[s].ny + 3) / 4;
+        e.ntz = (d[s].nz + 3) / 4;
+        const int bits = ceil_log2(std::max(e.ntx, std::max(e.nty, e.ntz)));
+        e.toff = (uint32_t)toff;
+        e.tslots = 1u << (3 * bits);
+        e.gbx = (d[s].nx + 7) / 8;
+        e.gby = (d[s].ny + 7) / 8;
+        e.gbz = (d[s].nz + 7) / 8;
+        e.boff = (uint32_t)boff;
+        e.nblk = (uint32_t)e.gbx * e.gby * e.gbz;
+        e.poff = (uint32_t)cap;
+        e.np = 0;
+        goff += (size_t)d[s].nx * d[s].ny * d[s].nz;
+        toff += e.tslots;
+        boff += e.nblk;
+        cap += (size_t)d[s].max_particles;
+    }
+    if (toff > (1ull << 31) || boff > (1ull << 31) || cap > (1ull << 31))
+        return fail(c, PG_ERR_INVALID, "scene table too large");
+    c->cap = std::max<size_t>(cap, 1);
+    c->G = goff;
+    c->nscenes = ns;
+    pg::TiledWork& w = c->tw;
+    w.nscenes = ns;
+    w.tile_slots = (int)toff;
+    w.total_blocks = (int)boff;
+    w.nblocks_launch = 2048;
+    w.prof = &c->prof;
+    int rc;
+    if ((rc = dalloc(c, &c->scenes_d, (size_t)ns)) || (rc = dalloc(c, &c->S, 6 * c->cap)) ||
+        (rc = dalloc(c, &c->pid, c->cap)) || (rc = dalloc(c, &c->mp, goff)) || (rc = dalloc(c, &c->gv, goff)) ||
+        (tiled && ((rc = dalloc(c, &c->S2, 6 * c->cap)) || (rc = dalloc(c, &c->pid2, c->cap)) ||
+                   (rc = dalloc(c, &w.count, toff)) || (rc = dalloc(c, &w.start, toff)) ||
+                   (rc = dalloc(c, &w.active, 4 * toff)) || (rc = dalloc(c, &w.nactive, 1)) ||
+                   (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
+                   (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
+                   (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))))))
+        return rc;
+    w.scenes = c->scenes_d;
+    hipMemset(c->mp, 0, goff * sizeof(float4));
+    hipMemset(c->gv, 0, goff * sizeof(float4));
+    if (tiled) {
+        hipMemset(w.count, 0, toff * 4);
+        hipMemset(w.flags, 0, boff * 4);
+        hipMemset(w.nactive, 0, 4);
+    }
+    hipError_t e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);
+    return e == hipSuccess ? PG_OK : hip_fail(c, e, "scene table upload");
+}
+
+// per-scene particle counts: poff/np of the table, uploaded on the stream
+int set_scene_counts(pg_mpm_ctx* c, const int* counts, hipStream_t st) {
+    uint32_t off = 0;
+    for (int s = 0; s < c->nscenes; ++s) {
+        c->scenes_h[s].poff = off;
+        c->scenes_h[s].np = (uint32_t)counts[s];
+        off += (uint32_t)counts[s];
+    }
+    hipError_t e = hipMemcpyAsync(c->scenes_d, c->scenes_h.data(), c->nscenes * sizeof(pg::SceneInfo),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? PG_OK : hip_fail(c, e, "scene table upload");
+}
+
+void drop_graphs(pg_mpm_ctx* c) {
+    if (c->graphs.empty()) return;
+    hipDeviceSynchronize();
+    for (auto& ge : c->graphs) hipGraphExecDestroy(ge.exec);
+    c->graphs.clear();
+}
+
+// Runs nsteps tiled steps as one hipGraph launch.  The step sequence of a
+// given length starting from a given state buffer is captured once and then
+// replayed; the host-side ping-pong of the state buffers is applied after
+// each launch exactly as the captured sequence did it.  A NULL (legacy)
+// stream cannot be captured, so the work then runs on an internal stream
+// ordered against it with events.
+hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
+    hipError_t e = hipSuccess;
+    hipStream_t st = user_st;
+    if (!st) {
+        if (!c->gstream) {
+            if ((e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking)) != hipSuccess) return e;
+            if ((e = hipEventCreateWithFlags(&c->gev[0], hipEventDisableTiming)) != hipSuccess) return e;
+            if ((e = hipEventCreateWithFlags(&c->gev[1], hipEventDisableTiming)) != hipSuccess) return e;
+        }
+        if ((e = hipEventRecord(c->gev[0], nullptr)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(c->gstream, c->gev[0], 0)) != hipSuccess) return e;
+        st = c->gstream;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& ge : c->graphs)
+        if (ge.nsteps == nsteps && ge.S == c->S) exec = ge.exec;
+    if (!exec) {
+        const bool prof = c->prof.on;
+        c->prof.on = false;  // no event records inside the captured sequence
+        float4 *S = c->S, *S2 = c->S2;
+        uint32_t *pid = c->pid, *pid2 = c->pid2;
+        float* E = c->have_E ? c->E : nullptr;
+        float* E2 = c->have_E ? c->E2 : nullptr;
+        float* m = c->have_mass ? c->mass : nullptr;
+        float* m2 = c->have_mass ? c->mass2 : nullptr;
+        hipGraph_t graph = nullptr;
+        e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            hipError_t ek = pg::tiled_steps(c->tw, c->n, c->cap, S, S2, pid, pid2, E, E2, m, m2, c->mp, c->gv,
+                                            nsteps, st);
+            e = hipStreamEndCapture(st, &graph);
+            if (ek != hipSuccess) e = ek;
+        }
+        c->prof.on = prof;
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) hipGraphDestroy(graph);
+        if (e != hipSuccess) return e;
+        c->graphs.push_back({nsteps, c->S, exec});
+    }
+    if ((e = hipGraphLaunch(exec, st)) != hipSuccess) return e;
+    if (nsteps & 1) {
+        std::swap(c->S, c->S2);
+        std::swap(c->pid, c->pid2);
+        if (c->have_E) std::swap(c->E, c->E2);
+        if (c->have_mass) std::swap(c->mass, c->mass2);
+    }
+    if (!user_st) {
+        if ((e = hipEventRecord(c->gev[1], st)) != hipSuccess) return e;
+        e = hipStreamWaitEvent(nullptr, c->gev[1], 0);
+    }
+    return e;
 }
 
 }  // namespace
@@ -181,39 +367,11 @@ int pg_mpm_create(const pg_mpm_params* p, pg_mpm_ctx** out) {
         hipMemset(c->rgp, 0, 3 * G * 4);
         hipMemset(c->rgv, 0, 3 * G * 4);
     } else {
-        const bool tiled = p->impl == PG_IMPL_TILED;
-        pg::TiledWork& w = c->tw;
-        w.T = 4;
-        w.ntx = (p->nx + w.T - 1) / w.T;
-        w.nty = (p->ny + w.T - 1) / w.T;
-        w.ntz = (p->nz + w.T - 1) / w.T;
-        int bits = ceil_log2(std::max(w.ntx, std::max(w.nty, w.ntz)));
-        w.tile_slots = 1 << (3 * bits);
-        w.gbx = (p->nx + 7) / 8;
-        w.gby = (p->ny + 7) / 8;
-        w.gbz = (p->nz + 7) / 8;
-        w.nblocks_launch = 2048;
-        w.prof = &c->prof;
-        if ((rc = dalloc(c, &c->S, 6 * c->cap)) || (rc = dalloc(c, &c->pid, c->cap)) ||
-            (rc = dalloc(c, &c->mp, G)) || (rc = dalloc(c, &c->gv, G)) ||
-            (tiled && ((rc = dalloc(c, &c->S2, 6 * c->cap)) || (rc = dalloc(c, &c->pid2, c->cap)) ||
-                       (rc = dalloc(c, &w.count, (size_t)w.tile_slots)) ||
-                       (rc = dalloc(c, &w.start, (size_t)w.tile_slots)) ||
-                       (rc = dalloc(c, &w.active, 4 * (size_t)w.tile_slots)) || (rc = dalloc(c, &w.nactive, 1)) ||
-                       (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
-                       (rc = dalloc(c, &w.perm, c->cap)) ||
-                       (rc = dalloc(c, &w.flags, (size_t)w.gbx * w.gby * w.gbz)) ||
-                       (rc = dalloc(c, &w.bsum, 2 * (size_t)((w.tile_slots + 1023) / 1024)))))) {
+        pg_scene_desc d = scene_from_params(*p);
+        if ((rc = init_mls(c, &d, 1))) {
             std::string m = c->err;
             pg_mpm_destroy(c);
             return fail(nullptr, rc, m);
-        }
-        hipMemset(c->mp, 0, G * sizeof(float4));
-        hipMemset(c->gv, 0, G * sizeof(float4));
-        if (tiled) {
-            hipMemset(w.count, 0, (size_t)w.tile_slots * 4);
-            hipMemset(w.flags, 0, (size_t)w.gbx * w.gby * w.gbz * 4);
-            hipMemset(w.nactive, 0, 4);
         }
     }
     hipError_t e = hipDeviceSynchronize();
@@ -233,6 +391,10 @@ int pg_mpm_destroy(pg_mpm_ctx* c) {
     for (void* p : c->allocs) hipFree(p);
     for (void* p : c->ck_allocs) hipFree(p);
     for (void* p : c->tape_allocs) hipFree(p);
+    for (auto& ge : c->graphs) hipGraphExecDestroy(ge.exec);
+    if (c->gstream) hipStreamDestroy(c->gstream);
+    for (hipEvent_t ev : c->gev)
+        if (ev) hipEventDestroy(ev);
     delete c;
     return PG_OK;
 }
@@ -246,9 +408,13 @@ int pg_mpm_get_time(const pg_mpm_ctx* c, double* t, long long* s) {
     return PG_OK;
 }
 
-int pg_mpm_set_particles(pg_mpm_ctx* c, int n, const float* x, const float* v, const float* C, const float* F,
-                         const float* mass, const float* E, int memory, void* stream) {
-    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+}  // extern "C"
+
+namespace {
+
+// counts: per-scene particle counts of an MLS-MPM context (NULL: one scene of n)
+int set_particles_impl(pg_mpm_ctx* c, int n, const int* counts, const float* x, const float* v, const float* C,
+                       const float* F, const float* mass, const float* E, int memory, void* stream) {
     if (n < 0 || (size_t)n > c->cap)
         return fail(c, PG_ERR_INVALID, "particle count " + std::to_string(n) + " exceeds max_particles " +
                                            std::to_string(c->cap));
@@ -272,6 +438,7 @@ int pg_mpm_set_particles(pg_mpm_ctx* c, int n, const float* x, const float* v, c
         }
         return PG_OK;
     }
+    if ((rc = set_scene_counts(c, counts ? counts : &n, st))) return rc;
     // MLS-MPM: stage user arrays on the device, then pack into the float4 planes.
     float *tx = nullptr, *tv = nullptr, *tC = nullptr, *tF = nullptr;
     const float *dx_ = x, *dv_ = v, *dC_ = C, *dF_ = F;
@@ -297,6 +464,7 @@ int pg_mpm_set_particles(pg_mpm_ctx* c, int n, const float* x, const float* v, c
         cleanup();
         return hip_fail(c, e, "pack");
     }
+    drop_graphs(c);
     c->have_mass = mass != nullptr;
     c->have_E = E != nullptr;
     const bool tiled = c->p.impl == PG_IMPL_TILED;
@@ -316,6 +484,17 @@ int pg_mpm_set_particles(pg_mpm_ctx* c, int n, const float* x, const float* v, c
     }
     cleanup();
     return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_mpm_set_particles(pg_mpm_ctx* c, int n, const float* x, const float* v, const float* C, const float* F,
+                         const float* mass, const float* E, int memory, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->nscenes > 1) return fail(c, PG_ERR_INVALID, "a batch context takes pg_mpm_batch_set_particles");
+    return set_particles_impl(c, n, nullptr, x, v, C, F, mass, E, memory, stream);
 }
 
 int pg_mpm_get_particles(pg_mpm_ctx* c, float* x, float* v, float* C, float* F, int memory, void* stream) {
@@ -363,7 +542,7 @@ int pg_mpm_get_grid(pg_mpm_ctx* c, float* mass, float* momentum, float* velocity
     if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
     if (memory != PG_HOST && memory != PG_DEVICE) return fail(c, PG_ERR_INVALID, "bad memory kind");
     hipStream_t st = (hipStream_t)stream;
-    const size_t G = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    const size_t G = c->p.mode == PG_MODE_REF_COMPAT ? (size_t)c->p.nx * c->p.ny * c->p.nz : c->G;
     hipError_t e = hipSuccess;
     if (c->p.mode == PG_MODE_REF_COMPAT) {
         if (mass) e = hipMemcpyAsync(mass, c->rgm, 4 * G, kind_from_dev(memory), st);
@@ -421,10 +600,14 @@ int pg_mpm_step(pg_mpm_ctx* c, int nsteps, void* stream) {
             float* E2 = c->have_E ? c->E2 : nullptr;
             float* m = c->have_mass ? c->mass : nullptr;
             float* m2 = c->have_mass ? c->mass2 : nullptr;
-            e = pg::tiled_steps(g, c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
-                                nsteps, st);
-            if (c->have_E) { c->E = E; c->E2 = E2; }
-            if (c->have_mass) { c->mass = m; c->mass2 = m2; }
+            if (c->use_graph) {
+                e = graph_steps(c, nsteps, st);
+            } else {
+                e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
+                                    nsteps, st);
+                if (c->have_E) { c->E = E; c->E2 = E2; }
+                if (c->have_mass) { c->mass = m; c->mass2 = m2; }
+            }
         }
     }
     if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_step");
@@ -518,9 +701,9 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     free_all(c->tape_allocs);
     c->tape.assign(K + 1, pg::TapeStep{});
     c->tape_len = 0;
-    const size_t G = (size_t)c->p.nx * c->p.ny * c->p.nz;
+    const size_t G = c->G;
     const pg::TiledWork& w = c->tw;
-    const size_t nblk = (size_t)w.gbx * w.gby * w.gbz;
+    const size_t nblk = (size_t)w.total_blocks;
     auto& A = c->tape_allocs;
     int rc;
     for (auto& t : c->tape) {
@@ -563,7 +746,6 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
     int rc;
     if ((rc = ensure_checkpoints(c, ncp)) || (rc = ensure_tape(c, K))) return rc;
     hipStream_t st = (hipStream_t)stream;
-    pg::MlsGeom g = mls_geom(c);
     hipError_t e = hipSuccess;
     for (int k = 0; k < ncp && e == hipSuccess; ++k) {
         pg::TapeStep& ck = c->ckpt[k];
@@ -578,7 +760,7 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
         float* m = c->have_mass ? c->mass : nullptr;
         float* m2 = c->have_mass ? c->mass2 : nullptr;
         if (e == hipSuccess)
-            e = pg::tiled_steps(g, c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv, L, st);
+            e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv, L, st);
         if (c->have_E) { c->E = E; c->E2 = E2; }
         if (c->have_mass) { c->mass = m; c->mass2 = m2; }
     }
@@ -601,7 +783,6 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     hipStream_t st = (hipStream_t)stream;
     const int n = c->n, K = c->ck_every;
     const int ncp = (c->ck_steps + K - 1) / K;
-    pg::MlsGeom g = mls_geom(c);
     hipError_t e = hipSuccess;
     auto seed = [&](float* dst, const float* src, size_t count) {
         if (e != hipSuccess) return;
@@ -622,10 +803,10 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(T[0].E, ck.E, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && c->have_mass)
             e = hipMemcpyAsync(T[0].mass, ck.mass, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = pg::tiled_tape_forward(g, c->tw, n, c->cap, T, L, st);
+        if (e == hipSuccess) e = pg::tiled_tape_forward(c->tw, n, c->cap, T, L, st);
         if (e == hipSuccess) e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, st);
         if (e == hipSuccess)
-            e = pg::adjoint_segment(g, c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, st);
+            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, st);
         if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, st);
     }
     auto out = [&](float* dst, const float* src, size_t count) {
@@ -637,6 +818,211 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     out(gF, c->uF, 9 * (size_t)n);
     out(gE, c->aE, (size_t)n);
     if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_backward");
+    return PG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches, graphs, autotuning
+namespace {
+
+std::string tune_cache_path() {
+    if (const char* p = std::getenv("PG_AUTOTUNE_CACHE")) return p;
+    const char* home = std::getenv("HOME");
+    return std::string(home ? home : "/tmp") + "/.cache/physgrad_amd_autotune.txt";
+}
+
+std::string tune_key(const pg_mpm_ctx* c) {
+    size_t nb = 1;
+    while (nb < (size_t)std::max(c->n, 1)) nb <<= 1;
+    return "slots=" + std::to_string(c->tw.tile_slots) + ";blocks=" + std::to_string(c->tw.total_blocks) +
+           ";n=" + std::to_string(nb) + ";scenes=" + std::to_string(c->nscenes);
+}
+
+// cached launch size for this workload shape, or 0
+int tune_lookup(const pg_mpm_ctx* c) {
+    FILE* f = std::fopen(tune_cache_path().c_str(), "r");
+    if (!f) return 0;
+    const std::string key = tune_key(c);
+    char line[512];
+    int val = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        char k[400];
+        int v = 0;
+        if (std::sscanf(line, "%399s %d", k, &v) == 2 && key == k && v > 0) val = v;
+    }
+    std::fclose(f);
+    return val;
+}
+
+void tune_store(const pg_mpm_ctx* c, int val) {
+    FILE* f = std::fopen(tune_cache_path().c_str(), "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d\n", tune_key(c).c_str(), val);
+    std::fclose(f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_mpm_batch_create(const pg_scene_desc* scenes, int nscenes, pg_mpm_ctx** out) {
+    if (!scenes || !out || nscenes < 1 || nscenes > 65536)
+        return fail(nullptr, PG_ERR_INVALID, "pg_mpm_batch_create: bad arguments");
+    *out = nullptr;
+    for (int s = 0; s < nscenes; ++s)
+        if (const char* why = check_scene(scenes[s]))
+            return fail(nullptr, PG_ERR_INVALID, "scene " + std::to_string(s) + ": " + why);
+    pg_mpm_ctx* c = new (std::nothrow) pg_mpm_ctx();
+    if (!c) return fail(nullptr, PG_ERR_NOMEM, "host allocation failed");
+    pg_mpm_default_params(&c->p);
+    c->p.mode = PG_MODE_MLS_MPM;
+    c->p.impl = PG_IMPL_TILED;
+    c->p.dt = scenes[0].dt;
+    c->p.density = scenes[0].density;
+    c->p.particle_volume = scenes[0].particle_volume;
+    hipGetDevice(&c->device);
+    int rc = init_mls(c, scenes, nscenes);
+    if (rc == PG_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(c, e, "pg_mpm_batch_create");
+    }
+    if (rc != PG_OK) {
+        std::string m = c->err;
+        pg_mpm_destroy(c);
+        return fail(nullptr, rc, m);
+    }
+    *out = c;
+    return PG_OK;
+}
+
+int pg_mpm_batch_set_particles(pg_mpm_ctx* c, const int* counts, const float* x, const float* v, const float* C,
+                               const float* F, const float* mass, const float* E, int memory, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
+        return fail(c, PG_ERR_INVALID, "batched particles need a tiled MLS-MPM context");
+    if (!counts) return fail(c, PG_ERR_INVALID, "counts is required");
+    long long total = 0;
+    for (int s = 0; s < c->nscenes; ++s) {
+        if (counts[s] < 0) return fail(c, PG_ERR_INVALID, "negative particle count");
+        total += counts[s];
+    }
+    if ((size_t)total > c->cap) return fail(c, PG_ERR_INVALID, "particle total exceeds the batch capacity");
+    return set_particles_impl(c, (int)total, counts, x, v, C, F, mass, E, memory, stream);
+}
+
+int pg_mpm_num_scenes(const pg_mpm_ctx* c) { return c ? c->nscenes : -1; }
+
+int pg_mpm_set_graph(pg_mpm_ctx* c, int enable) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (enable && (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED))
+        return fail(c, PG_ERR_INVALID, "graph replay is implemented for the tiled MLS-MPM path");
+    if (!enable) drop_graphs(c);
+    c->use_graph = enable != 0;
+    return PG_OK;
+}
+
+int pg_mpm_get_launch(const pg_mpm_ctx* c, int* tile_workgroups) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (tile_workgroups) *tile_workgroups = c->tw.nblocks_launch;
+    return PG_OK;
+}
+
+int pg_mpm_set_launch(pg_mpm_ctx* c, int tile_workgroups) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (tile_workgroups < 8 || tile_workgroups > (1 << 20))
+        return fail(c, PG_ERR_INVALID, "tile_workgroups out of range");
+    drop_graphs(c);
+    c->tw.nblocks_launch = tile_workgroups;
+    return PG_OK;
+}
+
+int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
+        return fail(c, PG_ERR_INVALID, "the autotuner tunes the tiled MLS-MPM path");
+    if (trial_steps < 1) return fail(c, PG_ERR_INVALID, "trial_steps must be >= 1");
+    if (c->n == 0) return PG_OK;
+    if (use_cache) {
+        const int v = tune_lookup(c);
+        if (v > 0) return pg_mpm_set_launch(c, v);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    drop_graphs(c);
+    // save the state (and the grid) so that the trials leave no trace
+    const size_t n = (size_t)c->n;
+    float4 *S0 = c->S, *S20 = c->S2;
+    uint32_t *pid0 = c->pid, *pid20 = c->pid2;
+    float *E0 = c->E, *E20 = c->E2, *m0 = c->mass, *m20 = c->mass2;
+    float4 *sS = nullptr, *sG = nullptr;
+    uint32_t* sP = nullptr;
+    float *sE = nullptr, *sM = nullptr;
+    auto release = [&]() {
+        hipStreamSynchronize(st);
+        hipFree(sS); hipFree(sG); hipFree(sP); hipFree(sE); hipFree(sM);
+    };
+    if (hipMalloc(&sS, 6 * n * sizeof(float4)) != hipSuccess || hipMalloc(&sG, c->G * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&sP, n * 4) != hipSuccess || (c->have_E && hipMalloc(&sE, n * 4) != hipSuccess) ||
+        (c->have_mass && hipMalloc(&sM, n * 4) != hipSuccess)) {
+        release();
+        return fail(c, PG_ERR_NOMEM, "autotune scratch allocation failed");
+    }
+    auto save = [&](bool back) {
+        for (int p = 0; p < 6; ++p)
+            hipMemcpyAsync(back ? S0 + p * c->cap : sS + p * n, back ? sS + p * n : S0 + p * c->cap,
+                           n * sizeof(float4), hipMemcpyDeviceToDevice, st);
+        hipMemcpyAsync(back ? (void*)c->gv : (void*)sG, back ? (void*)sG : (void*)c->gv, c->G * sizeof(float4),
+                       hipMemcpyDeviceToDevice, st);
+        hipMemcpyAsync(back ? (void*)pid0 : (void*)sP, back ? (void*)sP : (void*)pid0, n * 4, hipMemcpyDeviceToDevice,
+                       st);
+        if (c->have_E)
+            hipMemcpyAsync(back ? (void*)E0 : (void*)sE, back ? (void*)sE : (void*)E0, n * 4, hipMemcpyDeviceToDevice,
+                           st);
+        if (c->have_mass)
+            hipMemcpyAsync(back ? (void*)m0 : (void*)sM, back ? (void*)sM : (void*)m0, n * 4, hipMemcpyDeviceToDevice,
+                           st);
+        c->S = S0; c->S2 = S20; c->pid = pid0; c->pid2 = pid20;
+        c->E = E0; c->E2 = E20; c->mass = m0; c->mass2 = m20;
+    };
+    save(false);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    const int cand[] = {256, 512, 1024, 2048, 4096, 8192};
+    int best = c->tw.nblocks_launch;
+    float best_ms = 1e30f;
+    hipError_t e = hipSuccess;
+    for (int v : cand) {
+        c->tw.nblocks_launch = v;
+        float ms_sum = 0.f;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+            save(true);
+            float* E = c->have_E ? c->E : nullptr;
+            float* E2 = c->have_E ? c->E2 : nullptr;
+            float* m = c->have_mass ? c->mass : nullptr;
+            float* m2 = c->have_mass ? c->mass2 : nullptr;
+            hipEventRecord(a, st);
+            e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
+                                trial_steps, st);
+            hipEventRecord(b, st);
+            hipEventSynchronize(b);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, a, b);
+            if (rep == 1) ms_sum += ms;  // the first repetition warms up
+        }
+        if (e != hipSuccess) break;
+        if (ms_sum < best_ms) {
+            best_ms = ms_sum;
+            best = v;
+        }
+    }
+    c->tw.nblocks_launch = best;
+    save(true);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    release();
+    if (e != hipSuccess) return hip_fail(c, e, "autotune");
+    tune_store(c, best);
     return PG_OK;
 }
 

@@ -45,6 +45,7 @@ struct AdjArgs {
     float4* agv;            // grid accumulator of the velocity adjoint (x,y,z used)
     const float4* apam;     // (am, ap) per node
     float* aE_user;         // dL/dE in user order (or NULL)
+    const SceneInfo* scenes;
     size_t cap;
 };
 
@@ -62,24 +63,28 @@ PG_DEV void weight_grads(const Stencil& st, float dw[3][3]) {
     }
 }
 
-__global__ void __launch_bounds__(TB) k_adj_g2p(MlsGeom g, AdjArgs a) {
+__global__ void __launch_bounds__(TB) k_adj_g2p(AdjArgs a) {
     constexpr int NP = 27;               // 9 nodes x 3 channels per slice
     __shared__ float4 sv[WV];
     __shared__ float sa[3 * WV];         // velocity-adjoint window, channel-major
     __shared__ float scr[TB / 64][64 * NP];
     const XcdChunk ch(*a.nactive);
-    const float s4 = 4.f * g.inv_dx * g.inv_dx;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float* myscr = scr[wave];
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
+        const SceneInfo& sc = a.scenes[te.w];
+        const MlsGeom g = sc.g;
+        const float4* __restrict__ gvs = a.gv + sc.goff;
+        float4* __restrict__ agvs = a.agv + sc.goff;
+        const float s4 = 4.f * g.inv_dx * g.inv_dx;
         const uint32_t beg = te.y, cnt = te.z;
         int t0[3];
-        tile_origin(te.x, t0);
+        tile_origin(te.x - sc.toff, t0);
         for (int l = tid; l < WV; l += TB) {
             int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
             float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gi < g.nx && gj < g.ny && gk < g.nz) u = a.gv[node_id(g, gi, gj, gk)];
+            if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
             sv[l] = u;
             sa[l] = 0.f; sa[WV + l] = 0.f; sa[2 * WV + l] = 0.f;
         }
@@ -119,7 +124,7 @@ __global__ void __launch_bounds__(TB) k_adj_g2p(MlsGeom g, AdjArgs a) {
                             u = sv[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
                         } else if (act) {
                             const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                            if (in_grid(g, gi, gj, gk)) u = a.gv[node_id(g, gi, gj, gk)];
+                            if (in_grid(g, gi, gj, gk)) u = gvs[node_id(g, gi, gj, gk)];
                         }
                         const float uv[3] = {u.x, u.y, u.z};
 #pragma unroll
@@ -163,7 +168,7 @@ __global__ void __launch_bounds__(TB) k_adj_g2p(MlsGeom g, AdjArgs a) {
                         } else if (act) {
                             gi = st.b[0] + ii; gj = st.b[1] + jj; gk = st.b[2] + kk;
                             ok = in_grid(g, gi, gj, gk);
-                            if (ok) u = a.gv[node_id(g, gi, gj, gk)];
+                            if (ok) u = gvs[node_id(g, gi, gj, gk)];
                         }
                         const float uv[3] = {u.x, u.y, u.z};
                         float g3[3] = {0.f, 0.f, 0.f};
@@ -185,7 +190,7 @@ __global__ void __launch_bounds__(TB) k_adj_g2p(MlsGeom g, AdjArgs a) {
                             af[1] += aw * wx * dw[1][jj] * wz;
                             af[2] += aw * wx * wy * dw[2][kk];
                             if (!inwin) {
-                                float* cp = reinterpret_cast<float*>(a.agv + node_id(g, gi, gj, gk));
+                                float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
                                 atomic_add_f32(cp + 0, g3[0]);
                                 atomic_add_f32(cp + 1, g3[1]);
                                 atomic_add_f32(cp + 2, g3[2]);
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(TB) k_adj_g2p(MlsGeom g, AdjArgs a) {
             int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
             float s0 = sa[l], s1 = sa[WV + l], s2 = sa[2 * WV + l];
             if ((s0 != 0.f || s1 != 0.f || s2 != 0.f) && gi < g.nx && gj < g.ny && gk < g.nz) {
-                float* cp = reinterpret_cast<float*>(a.agv + node_id(g, gi, gj, gk));
+                float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
                 atomic_add_f32(cp + 0, s0);
                 atomic_add_f32(cp + 1, s1);
                 atomic_add_f32(cp + 2, s2);
@@ -223,12 +228,18 @@ __global__ void __launch_bounds__(TB) k_adj_g2p(MlsGeom g, AdjArgs a) {
 }
 
 // reverse of v_i = (p_i/m_i + dt g) * mask_i
-__global__ void __launch_bounds__(512) k_adj_grid(MlsGeom g, float4* __restrict__ agv, float4* __restrict__ apam,
-                                                  float4* __restrict__ mp, uint32_t* __restrict__ flags, int gbx,
-                                                  int gby) {
+__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes, int nscenes,
+                                                  float4* __restrict__ agv_all, float4* __restrict__ apam_all,
+                                                  float4* __restrict__ mp_all, uint32_t* __restrict__ flags) {
     const int b = blockIdx.x;
     if (flags[b] == 0u) return;
-    const int bx = b % gbx, by = (b / gbx) % gby, bz = b / (gbx * gby);
+    const SceneInfo& sc = scenes[scene_of(scenes, nscenes, (uint32_t)b, [](const SceneInfo& s) { return s.boff; })];
+    const MlsGeom g = sc.g;
+    float4* __restrict__ agv = agv_all + sc.goff;
+    float4* __restrict__ apam = apam_all + sc.goff;
+    float4* __restrict__ mp = mp_all + sc.goff;
+    const int lb = b - (int)sc.boff;
+    const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
     const int i = bx * GB + (threadIdx.x & 7), j = by * GB + ((threadIdx.x >> 3) & 7), k = bz * GB + (threadIdx.x >> 6);
     if (i < g.nx && j < g.ny && k < g.nz) {
         const size_t id = node_id(g, i, j, k);
@@ -250,23 +261,26 @@ __global__ void __launch_bounds__(512) k_adj_grid(MlsGeom g, float4* __restrict_
     if (threadIdx.x == 0) flags[b] = 0u;
 }
 
-__global__ void __launch_bounds__(TB) k_adj_p2g(MlsGeom g, AdjArgs a) {
+__global__ void __launch_bounds__(TB) k_adj_p2g(AdjArgs a) {
     __shared__ float sp[4][WV];
     const XcdChunk ch(*a.nactive);
-    const float s4 = 4.f * g.inv_dx * g.inv_dx;
-    const float kscale = -g.dt * g.p_vol * s4;
-    const float emu = 1.f / (2.f * (1.f + g.nu));
-    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
     const int tid = threadIdx.x;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
+        const SceneInfo& sc = a.scenes[te.w];
+        const MlsGeom g = sc.g;
+        const float4* __restrict__ apams = a.apam + sc.goff;
+        const float s4 = 4.f * g.inv_dx * g.inv_dx;
+        const float kscale = -g.dt * g.p_vol * s4;
+        const float emu = 1.f / (2.f * (1.f + g.nu));
+        const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
         const uint32_t beg = te.y, cnt = te.z;
         int t0[3];
-        tile_origin(te.x, t0);
+        tile_origin(te.x - sc.toff, t0);
         for (int l = tid; l < WV; l += TB) {
             int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
             float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gi < g.nx && gj < g.ny && gk < g.nz) u = a.apam[node_id(g, gi, gj, gk)];
+            if (gi < g.nx && gj < g.ny && gk < g.nz) u = apams[node_id(g, gi, gj, gk)];
             sp[0][l] = u.x; sp[1][l] = u.y; sp[2][l] = u.z; sp[3][l] = u.w;
         }
         __syncthreads();
@@ -305,7 +319,7 @@ __global__ void __launch_bounds__(TB) k_adj_p2g(MlsGeom g, AdjArgs a) {
                         } else {
                             int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
                             if (!in_grid(g, gi, gj, gk)) continue;
-                            float4 q = a.apam[node_id(g, gi, gj, gk)];
+                            float4 q = apams[node_id(g, gi, gj, gk)];
                             am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
                         }
                         float aw = am * mq;
@@ -385,9 +399,10 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
 
 }  // namespace
 
-hipError_t adjoint_segment(const MlsGeom& g, const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
-                           float4* abar, float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st) {
-    const int nblk = w.gbx * w.gby * w.gbz;
+hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L, float4* abar,
+                           float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st) {
+    (void)n;
+    const int nblk = w.total_blocks;
     for (int s = L - 1; s >= 0; --s) {
         TapeStep& t = tape[s];
         AdjArgs a;
@@ -399,18 +414,19 @@ hipError_t adjoint_segment(const MlsGeom& g, const TiledWork& w, int n, size_t c
         a.gv = t.gv;
         a.abar = abar; a.part = part; a.agv = agv; a.apam = apam;
         a.aE_user = aE_user;
+        a.scenes = w.scenes;
         a.cap = cap;
         {
             ProfScope ps(w.prof, PK_ADJ_G2P, st);
-            k_adj_g2p<<<w.nblocks_launch, TB, 0, st>>>(g, a);
+            k_adj_g2p<<<w.nblocks_launch, TB, 0, st>>>(a);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(g, agv, apam, t.mp, t.flags, w.gbx, w.gby);
+            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, agv, apam, t.mp, t.flags);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);
-            k_adj_p2g<<<w.nblocks_launch, TB, 0, st>>>(g, a);
+            k_adj_p2g<<<w.nblocks_launch, TB, 0, st>>>(a);
         }
     }
     return hipGetLastError();

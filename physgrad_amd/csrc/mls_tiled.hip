@@ -51,16 +51,15 @@ struct TileArgs {
     uint32_t* rank_out;
     uint32_t* count;
     uint32_t* src_out;       // output position -> input slot (segment tapes only)
-    const float4* gv;
+    const float4* gv;        // concatenated grids of all scenes
     float4* mp;
-    uint32_t* flags;
+    uint32_t* flags;         // concatenated 8^3 block flags of all scenes
+    const SceneInfo* scenes;
     size_t cap;
-    int ntx, nty, ntz;
-    int gbx, gby, gbz;
 };
 
-__device__ __forceinline__ void flag_node(uint32_t* flags, const TileArgs& a, int gi, int gj, int gk) {
-    flags[((gk / GB) * a.gby + gj / GB) * a.gbx + gi / GB] = 1u;
+__device__ __forceinline__ void flag_node(uint32_t* flags, const SceneInfo& sc, int gi, int gj, int gk) {
+    flags[((gk / GB) * sc.gby + gj / GB) * sc.gbx + gi / GB] = 1u;
 }
 
 __device__ __forceinline__ uint32_t morton_tile(int tx, int ty, int tz) {
@@ -81,7 +80,7 @@ __device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, in
 // MODE bit 1: G2P (reads x, F; writes the new state at the cell-sorted position)
 // MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state)
 template <int MODE>
-__global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
+__global__ void __launch_bounds__(TB) k_tile(TileArgs a) {
     constexpr bool G2P = (MODE & 1) != 0;
     constexpr bool P2G = (MODE & 2) != 0;
     constexpr bool SORT = P2G;           // local cell order matters for the P2G reduction
@@ -95,16 +94,21 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
     __shared__ uint32_t dcnt[27], dbase[27];
 
     const XcdChunk ch(*a.nactive);
-    const float s4 = 4.f * g.inv_dx * g.inv_dx;
-    const float kscale = -g.dt * g.p_vol * s4;
-    const float emu = 1.f / (2.f * (1.f + g.nu));
-    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float* myscr = scr[wave];
 
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
-        const uint32_t tile = te.x, beg = te.y, cnt = te.z;
+        const SceneInfo& sc = a.scenes[te.w];
+        const MlsGeom g = sc.g;
+        const float4* __restrict__ gvs = a.gv + sc.goff;
+        float4* __restrict__ mps = a.mp + sc.goff;
+        uint32_t* __restrict__ flags = a.flags + sc.boff;
+        const float s4 = 4.f * g.inv_dx * g.inv_dx;
+        const float kscale = -g.dt * g.p_vol * s4;
+        const float emu = 1.f / (2.f * (1.f + g.nu));
+        const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+        const uint32_t tile = te.x - sc.toff, beg = te.y, cnt = te.z;
         int t0[3];
         tile_origin(tile, t0);
         const int tc[3] = {t0[0] / TT, t0[1] / TT, t0[2] / TT};
@@ -112,7 +116,7 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
             if (G2P) {
                 int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
                 float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gi < g.nx && gj < g.ny && gk < g.nz) u = a.gv[node_id(g, gi, gj, gk)];
+                if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
                 sv[l] = u;
             }
             if (P2G) { sm[l] = 0.f; sm[WV + l] = 0.f; sm[2 * WV + l] = 0.f; sm[3 * WV + l] = 0.f; }
@@ -198,7 +202,7 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
                                         u = sv[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
                                     } else {
                                         const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                                        if (in_grid(g, gi, gj, gk)) u = a.gv[node_id(g, gi, gj, gk)];
+                                        if (in_grid(g, gi, gj, gk)) u = gvs[node_id(g, gi, gj, gk)];
                                     }
                                     const float uv[3] = {u.x, u.y, u.z};
 #pragma unroll
@@ -229,9 +233,9 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
                         if (a.src_out) a.src_out[o] = s;
                         if (P2G) {
                             int nt[3];
-                            tile_coords(x, g, a.ntx, a.nty, a.ntz, nt);
+                            tile_coords(x, g, sc.ntx, sc.nty, sc.ntz, nt);
                             const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
-                            const uint32_t k = morton_tile(nt[0], nt[1], nt[2]);
+                            const uint32_t k = sc.toff + morton_tile(nt[0], nt[1], nt[2]);
                             a.key_out[o] = k;
                             if (dx_ >= -1 && dx_ <= 1 && dy_ >= -1 && dy_ <= 1 && dz_ >= -1 && dz_ <= 1) {
                                 const int dir = (dx_ + 1) + 3 * (dy_ + 1) + 9 * (dz_ + 1);
@@ -279,12 +283,12 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
                                     const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
                                     if (!in_grid(g, gi, gj, gk)) continue;
                                     const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                                    float* cp = reinterpret_cast<float*>(a.mp + node_id(g, gi, gj, gk));
+                                    float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
                                     atomic_add_f32(cp + 0, w * mq);
 #pragma unroll
                                     for (int c = 0; c < 3; ++c)
                                         atomic_add_f32(cp + 1 + c, w * (q3[c] + a0[c] * ii + a1[c] * jj + a2[c] * kk));
-                                    flag_node(a.flags, a, gi, gj, gk);
+                                    flag_node(flags, sc, gi, gj, gk);
                                 }
                     }
                     const int base = inwin ? (l2 * WN + l1) * WN + l0 : -1;
@@ -317,7 +321,7 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
                     const uint32_t nd = dcnt[tid];
                     if (nd) {
                         const int dx_ = tid % 3 - 1, dy_ = (tid / 3) % 3 - 1, dz_ = tid / 9 - 1;
-                        const uint32_t k = morton_tile(tc[0] + dx_, tc[1] + dy_, tc[2] + dz_);
+                        const uint32_t k = sc.toff + morton_tile(tc[0] + dx_, tc[1] + dy_, tc[2] + dz_);
                         dbase[tid] = atomicAdd(a.count + k, nd);
                     }
                 }
@@ -336,7 +340,7 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
                 if (m != 0.f) {
                     const int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
                     if (gi < g.nx && gj < g.ny && gk < g.nz) {
-                        float* cp = reinterpret_cast<float*>(a.mp + node_id(g, gi, gj, gk));
+                        float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
                         atomic_add_f32(cp + 0, m);
                         atomic_add_f32(cp + 1, sm[WV + l]);
                         atomic_add_f32(cp + 2, sm[2 * WV + l]);
@@ -348,7 +352,7 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
                 const int bx = t0[0] / GB + (tid & 1), by = t0[1] / GB + ((tid >> 1) & 1), bz = t0[2] / GB + (tid >> 2);
                 // the window spans nodes [t0, t0 + WN); flag each 8^3 block it touches
                 const bool touch = bx * GB < t0[0] + WN && by * GB < t0[1] + WN && bz * GB < t0[2] + WN;
-                if (touch && bx < a.gbx && by < a.gby && bz < a.gbz) a.flags[(bz * a.gby + by) * a.gbx + bx] = 1u;
+                if (touch && bx < sc.gbx && by < sc.gby && bz < sc.gbz) flags[(bz * sc.gby + by) * sc.gbx + bx] = 1u;
             }
         }
         __syncthreads();
@@ -360,11 +364,17 @@ __global__ void __launch_bounds__(TB) k_tile(MlsGeom g, TileArgs a) {
 // (segment recompute for the adjoint): both are kept for the reverse pass,
 // which clears them after use.
 template <bool TAPE>
-__global__ void __launch_bounds__(512) k_grid_tiled(MlsGeom g, float4* __restrict__ mp, float4* __restrict__ gv,
-                                                    uint32_t* __restrict__ flags, int gbx, int gby) {
+__global__ void __launch_bounds__(512) k_grid_tiled(const SceneInfo* __restrict__ scenes, int nscenes,
+                                                    float4* __restrict__ mp_all, float4* __restrict__ gv_all,
+                                                    uint32_t* __restrict__ flags) {
     const int b = blockIdx.x;
     if (flags[b] == 0u) return;
-    const int bx = b % gbx, by = (b / gbx) % gby, bz = b / (gbx * gby);
+    const SceneInfo& sc = scenes[scene_of(scenes, nscenes, (uint32_t)b, [](const SceneInfo& s) { return s.boff; })];
+    const MlsGeom g = sc.g;
+    float4* __restrict__ mp = mp_all + sc.goff;
+    float4* __restrict__ gv = gv_all + sc.goff;
+    const int lb = b - (int)sc.boff;
+    const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
     const int i = bx * GB + (threadIdx.x & 7), j = by * GB + ((threadIdx.x >> 3) & 7), k = bz * GB + (threadIdx.x >> 6);
     if (i < g.nx && j < g.ny && k < g.nz) {
         const size_t id = node_id(g, i, j, k);
@@ -388,15 +398,17 @@ __global__ void __launch_bounds__(512) k_grid_tiled(MlsGeom g, float4* __restric
 // First binning of a call: tile key per storage slot, ranks from one global
 // atomic per distinct key in each wave (storage is normally tile-sorted
 // already, so a wave holds one or two keys).
-__global__ void k_init_keys(int n, const float4* __restrict__ S, MlsGeom g, int ntx, int nty, int ntz,
-                            uint32_t* __restrict__ key, uint32_t* __restrict__ rank, uint32_t* __restrict__ count) {
+__global__ void k_init_keys(int n, const float4* __restrict__ S, const uint32_t* __restrict__ pid,
+                            const SceneInfo* __restrict__ scenes, int nscenes, uint32_t* __restrict__ key,
+                            uint32_t* __restrict__ rank, uint32_t* __restrict__ count) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = s < n;
     uint32_t k = 0xFFFFFFFFu;
     if (valid) {
         const float4 q = S[s];
         const float x[3] = {q.x, q.y, q.z};
-        k = tile_key(x, g, ntx, nty, ntz);
+        const SceneInfo& sc = scenes[scene_of(scenes, nscenes, pid[s], [](const SceneInfo& t) { return t.poff; })];
+        k = sc.toff + tile_key(x, sc.g, sc.ntx, sc.nty, sc.ntz);
     }
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -462,7 +474,8 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ 
 
 __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
-                                                   uint4* __restrict__ alist, uint32_t* __restrict__ nactive) {
+                                                   uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
+                                                   const SceneInfo* __restrict__ scenes, int nscenes) {
     __shared__ uint32_t sh[SCAN_T];
     // totals of the preceding blocks
     uint32_t ps = 0, pa = 0;
@@ -488,7 +501,10 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
     for (int k = 0; k < 4; ++k) {
         if (base + k < slots) {
             start[base + k] = run;
-            if (v[k]) alist[pos++] = make_uint4((uint32_t)(base + k), run, v[k], 0u);
+            if (v[k]) {
+                const int sc = scene_of(scenes, nscenes, (uint32_t)(base + k), [](const SceneInfo& t) { return t.toff; });
+                alist[pos++] = make_uint4((uint32_t)(base + k), run, v[k], (uint32_t)sc);
+            }
             run += v[k];
             count[base + k] = 0u;
         }
@@ -508,7 +524,7 @@ hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive);
+                                    nactive, w.scenes, w.nscenes);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
 }
@@ -520,24 +536,23 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.rank_out = w.rank;
     a.count = w.count;
     a.src_out = nullptr;
+    a.scenes = w.scenes;
     a.cap = cap;
-    a.ntx = w.ntx; a.nty = w.nty; a.ntz = w.ntz;
-    a.gbx = w.gbx; a.gby = w.gby; a.gbz = w.gbz;
 }
 
 }  // namespace
 
-hipError_t tiled_init_keys(const MlsGeom& g, TiledWork& w, int n, size_t cap, const float4* S, hipStream_t st) {
-    (void)cap;
-    if (n > 0) k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, S, g, w.ntx, w.nty, w.ntz, w.key, w.rank, w.count);
+hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t* pid, hipStream_t st) {
+    if (n > 0)
+        k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, S, pid, w.scenes, w.nscenes, w.key, w.rank, w.count);
     return bin(w, n, st);
 }
 
-hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
+hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
                        uint32_t*& pid2, float*& E, float*& E2, float*& mass, float*& mass2, float4* mp, float4* gv,
                        int nsteps, hipStream_t st) {
     if (nsteps <= 0) return hipSuccess;
-    const int nblk = w.gbx * w.gby * w.gbz;
+    const int nblk = w.total_blocks;
     TileArgs a;
     fill_args(a, w, cap);
     a.perm = w.perm;
@@ -552,26 +567,26 @@ hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4
         a.E_in = E; a.E_out = E2;
         a.m_in = mass; a.m_out = mass2;
     };
-    hipError_t e = tiled_init_keys(g, w, n, cap, S, st);
+    hipError_t e = tiled_init_keys(w, n, S, pid, st);
     if (e != hipSuccess) return e;
     set_io();
     {
         ProfScope ps(w.prof, PK_TILE_P2G, st);
-        k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+        k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
     }
     {
         ProfScope ps(w.prof, PK_GRID, st);
-        k_grid_tiled<false><<<nblk, 512, 0, st>>>(g, mp, gv, w.flags, w.gbx, w.gby);
+        k_grid_tiled<false><<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, mp, gv, w.flags);
     }
     for (int s = 0; s + 1 < nsteps; ++s) {
         set_io();
         {
             ProfScope ps(w.prof, PK_TILE_G2P2G, st);
-            k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+            k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
         }
         {
             ProfScope ps(w.prof, PK_GRID, st);
-            k_grid_tiled<false><<<nblk, 512, 0, st>>>(g, mp, gv, w.flags, w.gbx, w.gby);
+            k_grid_tiled<false><<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, mp, gv, w.flags);
         }
         e = bin(w, n, st);
         if (e != hipSuccess) return e;
@@ -580,19 +595,20 @@ hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4
     set_io();
     {
         ProfScope ps(w.prof, PK_TILE_G2P, st);
-        k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+        k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(a);
     }
     std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
     return hipGetLastError();
 }
 
-hipError_t tiled_tape_forward(const MlsGeom& g, TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
-                              hipStream_t st) {
+hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, int L, hipStream_t st) {
     if (L <= 0) return hipSuccess;
-    const int nblk = w.gbx * w.gby * w.gbz;
+    const int nblk = w.total_blocks;
     TileArgs a;
     fill_args(a, w, cap);
-    if (n > 0) k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, tape[0].S, g, w.ntx, w.nty, w.ntz, w.key, w.rank, w.count);
+    if (n > 0)
+        k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, tape[0].S, tape[0].pid, w.scenes, w.nscenes, w.key, w.rank,
+                                                     w.count);
     hipError_t e = bin_into(w, n, tape[0].perm, tape[0].alist, tape[0].nactive, st);
     if (e != hipSuccess) return e;
     for (int s = 0; s < L; ++s) {
@@ -610,11 +626,11 @@ hipError_t tiled_tape_forward(const MlsGeom& g, TiledWork& w, int n, size_t cap,
             a.mp = cur.mp;
             a.flags = cur.flags;
             ProfScope ps(w.prof, PK_TILE_P2G, st);
-            k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+            k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
         }
         {
             ProfScope ps(w.prof, PK_GRID, st);
-            k_grid_tiled<true><<<nblk, 512, 0, st>>>(g, cur.mp, cur.gv, cur.flags, w.gbx, w.gby);
+            k_grid_tiled<true><<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, cur.mp, cur.gv, cur.flags);
         }
         a.Sout = nxt.S;
         a.pid_out = nxt.pid;
@@ -626,14 +642,14 @@ hipError_t tiled_tape_forward(const MlsGeom& g, TiledWork& w, int n, size_t cap,
             a.flags = nxt.flags;
             {
                 ProfScope ps(w.prof, PK_TILE_G2P2G, st);
-                k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+                k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
             }
             e = bin_into(w, n, nxt.perm, nxt.alist, nxt.nactive, st);
             if (e != hipSuccess) return e;
         } else {
             // last step of the segment: G2P only, to fix the end state's storage order
             ProfScope ps(w.prof, PK_TILE_G2P, st);
-            k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(g, a);
+            k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(a);
         }
     }
     return hipGetLastError();

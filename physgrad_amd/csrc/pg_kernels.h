@@ -96,12 +96,29 @@ struct ProfScope {
 };
 
 // ---- MLS-MPM tiled path (cell-binned particles, LDS-staged transfers)
+//
+// One context advances one or many independent scenes.  Scene s owns the user
+// particle range [poff, poff+np), the grid nodes [goff, goff + nx*ny*nz) of
+// the concatenated grid, the tile ids [toff, toff + tslots) and the 8^3
+// blocks [boff, boff + gbx*gby*gbz); every kernel looks its scene up per tile
+// or per block.
+struct SceneInfo {
+    MlsGeom g;
+    unsigned long long goff;
+    uint32_t toff, tslots;
+    uint32_t boff, nblk;
+    uint32_t poff, np;
+    int ntx, nty, ntz;
+    int gbx, gby, gbz;
+    int pad;
+};
+
 struct TiledWork {
     KernelProf* prof;      // optional per-kernel timing
-    int T;                 // tile edge in cells
-    int ntx, nty, ntz;     // tiles per axis
-    int tile_slots;        // number of tile ids (Morton range)
-    int gbx, gby, gbz;     // 8^3 grid blocks per axis
+    const SceneInfo* scenes;  // device scene table
+    int nscenes;
+    int tile_slots;        // total tile ids over all scenes
+    int total_blocks;      // total 8^3 grid blocks over all scenes
     uint32_t* count;       // [tile_slots]
     uint32_t* start;       // [tile_slots]
     uint32_t* active;      // [tile_slots] active tile list
@@ -109,7 +126,7 @@ struct TiledWork {
     uint32_t* key;         // [cap] tile id per storage slot
     uint32_t* rank;        // [cap] rank within tile
     uint32_t* perm;        // [cap] sorted position -> storage slot
-    uint32_t* flags;       // [gbx*gby*gbz] touched grid blocks
+    uint32_t* flags;       // [total_blocks] touched grid blocks
     uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
     int nblocks_launch;    // persistent grid size of the tile kernels
 };
@@ -123,7 +140,7 @@ struct TapeStep {
     float* mass;        // per-particle mass in storage order (or NULL)
     uint32_t* perm;     // sorted position -> storage slot
     uint32_t* src;      // output position of this step -> storage slot of this step
-    uint32_t* alist;    // uint4 (tile, start, count, 0) per active tile
+    uint32_t* alist;    // uint4 (tile, start, count, scene) per active tile
     uint32_t* nactive;
     float4* mp;         // P2G accumulation (m, p); cleared by the reverse pass
     float4* gv;         // (v, m) after the grid update
@@ -132,11 +149,11 @@ struct TapeStep {
 
 // Recompute L steps from tape[0].S (tape[0].pid/E/mass set), filling tape[0..L-1]
 // and tape[L].S/pid/E/mass (the end state in its storage order).
-hipError_t tiled_tape_forward(const MlsGeom& g, TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
+hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
                               hipStream_t st);
 
 // Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
-hipError_t adjoint_segment(const MlsGeom& g, const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
+hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
                            float4* abar, float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st);
 // user-order adjoint arrays (x,v [3n], C,F [9n]; NULL = 0) -> adjoint planes in storage order (pid)
 hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float* ax, const float* av,
@@ -144,10 +161,10 @@ hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float
 hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,
                            float* aC, float* aF, hipStream_t st);
 
-hipError_t tiled_init_keys(const MlsGeom& g, TiledWork& w, int n, size_t cap, const float4* S, hipStream_t st);
-// Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on return
-// the current buffers are in S (swapped by the caller as reported by *swapped).
-hipError_t tiled_steps(const MlsGeom& g, TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
+hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t* pid, hipStream_t st);
+// Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on
+// return the current buffers are S, pid, E, mass (the references are swapped).
+hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
                        uint32_t*& pid2, float*& E, float*& E2, float*& mass, float*& mass2, float4* mp, float4* gv,
                        int nsteps, hipStream_t st);
 

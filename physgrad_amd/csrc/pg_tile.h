@@ -107,6 +107,17 @@ __device__ __forceinline__ void wave_slice_reduce(const float* scr, int base, in
     if (cur >= 0 && pl) atomicAdd(&win[c * WV + cur + off], acc);
 }
 
+// scene owning an index, by binary search of a monotone offset field
+template <typename F>
+__device__ __forceinline__ int scene_of(const SceneInfo* sc, int ns, uint32_t idx, F off) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off(sc[mid]) <= idx) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ bool in_grid(const MlsGeom& g, int i, int j, int k) {
     return i >= 0 && i < g.nx && j >= 0 && j < g.ny && k >= 0 && k < g.nz;
 }

@@ -1,18 +1,24 @@
 """Synthetic workloads of the MLS-MPM transfer loop (seeded, reproducible).
 
-    single_box        config 1: 64^3, dx 1/64, 32,768 particles uniform in
+    single_box        64^3, dx 1/64, 32,768 particles uniform in
                       [0.3,0.5]^3 (seed 1), fixed-corotated E 1e4, nu 0.2, rho 1,
                       v (0,-1,0), g -9.8, dt 1e-4, sticky walls 3 nodes, 200 steps
-    two_boxes         config 2: 128^3, dx 1/128, 1,048,576 particles, half in
+    two_boxes         128^3, dx 1/128, 1,048,576 particles, half in
                       [0.2,0.45]^3 and half in [0.55,0.8]x[0.2,0.45]^2 (seed 2),
                       v = +-2 along x (towards each other), neo-Hookean E 5e4,
                       nu 0.3, dt 5e-5, 1000 steps.  The density is not given by
                       the workload; rho = 1000 keeps the elastic CFL number
                       c dt/dx = 0.05 (rho = 1 would give 1.66, an unstable run).
-    rollout           config 3: 64^3, 262,144 particles in a seeded random box
+    rollout           64^3, 262,144 particles in a seeded random box
                       (seed 3), fixed-corotated, nu 0.2, per-particle E
                       log-uniform in [5e3,5e4], rho 1000 (unstated; CFL 0.05),
                       dt 1e-4, 512 steps; loss = |mean x - (0.6,0.3,0.5)|^2.
+    batch_scenes      64 independent scenes (seed 4): grid side in {32,48,64}
+                      (dx = 1/side), log-uniform 4,096..65,536 particles in one
+                      seeded random box per scene, fixed-corotated or
+                      neo-Hookean per scene, E log-uniform in [5e3,5e4],
+                      nu 0.2..0.35, rho 1000, dt 1e-4, 500 steps, all advanced
+                      by the same launches.
 Particle volume V0 = box volume / N; mass = rho V0.
 """
 from __future__ import annotations
@@ -51,6 +57,12 @@ class Scene:
     def p_mass(self) -> float:
         return self.rho * self.p_vol
 
+    def desc(self, max_particles: Optional[int] = None):
+        """pg_scene_desc of this scene (one entry of a batch context)."""
+        from .solver import scene_desc
+        return scene_desc(self.dims, self.dx, self.origin, self.dt, self.gravity, self.material, self.E, self.nu,
+                          self.rho, self.p_vol, self.bc, int(max_particles or max(self.n, 1)))
+
     def params(self, impl: int = 1, max_particles: Optional[int] = None):
         from .solver import make_params
         from . import _lib
@@ -58,11 +70,6 @@ class Scene:
                            dt=self.dt, gravity=self.gravity, material=self.material, youngs_modulus=self.E,
                            poisson_ratio=self.nu, density=self.rho, particle_volume=self.p_vol,
                            boundary_thickness=self.bc, max_particles=int(max_particles or self.n))
-
-    def oracle_params(self):
-        from oracle import oracle as O  # test infrastructure
-        return O.mls_params(self.dims, self.dx, self.origin, self.dt, self.gravity, self.bc, self.material,
-                            self.nu, self.p_vol)
 
 
 def _uniform(rng, n, lo, hi):
@@ -101,3 +108,23 @@ def rollout(n: int = 262144, seed: int = 3) -> Scene:
     s = Scene("rollout_64", (64, 64, 64), 1 / 64, 1e-4, 512, 0, 1e4, 0.2, 1000.0, float(np.prod(size)) / n, x, v,
               E_particles=E, target=(0.6, 0.3, 0.5))
     return s
+
+
+def batch_scenes(nscenes: int = 64, seed: int = 4, min_n: int = 4096, max_n: int = 65536,
+                 sides=(32, 48, 64)) -> list:
+    """Independent scenes of a batch (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nscenes):
+        side = int(sides[rng.integers(len(sides))])
+        n = int(np.exp(rng.uniform(np.log(min_n), np.log(max_n))))
+        size = 0.15 + 0.15 * rng.random(3)
+        lo = 0.15 + (0.7 - size) * rng.random(3)
+        x = _uniform(rng, n, lo, lo + size)
+        v = rng.normal(0.0, 0.5, (n, 3)).astype(np.float32)
+        material = int(rng.integers(2))
+        E = float(np.exp(rng.uniform(np.log(5e3), np.log(5e4))))
+        nu = float(rng.uniform(0.2, 0.35))
+        out.append(Scene(f"batch_{s}_{side}", (side,) * 3, 1.0 / side, 1e-4, 500, material, E, nu, 1000.0,
+                         float(np.prod(size)) / n, x, v))
+    return out

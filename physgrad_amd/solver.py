@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PgMpmParams, check, load
+from ._lib import PgMpmParams, PgSceneDesc, check, load
 
 
 def _stream_ptr():
@@ -84,6 +84,7 @@ class MPMContext:
         h = ctypes.c_void_p()
         check(self._L.pg_mpm_create(ctypes.byref(params), ctypes.byref(h)))
         self._h = h
+        self._nodes = params.nx * params.ny * params.nz
 
     def close(self):
         if getattr(self, "_h", None):
@@ -112,7 +113,27 @@ class MPMContext:
 
     @property
     def num_nodes(self) -> int:
-        return self.params.nx * self.params.ny * self.params.nz
+        return self._nodes
+
+    def set_graph(self, enable: bool = True):
+        """Replay step() from captured hipGraphs (tiled MLS-MPM)."""
+        check(self._L.pg_mpm_set_graph(self._h, 1 if enable else 0), self._h)
+
+    @property
+    def launch(self) -> int:
+        """Workgroups of the persistent tile kernels."""
+        v = ctypes.c_int()
+        check(self._L.pg_mpm_get_launch(self._h, ctypes.byref(v)), self._h)
+        return v.value
+
+    @launch.setter
+    def launch(self, workgroups: int):
+        check(self._L.pg_mpm_set_launch(self._h, int(workgroups)), self._h)
+
+    def autotune(self, trial_steps: int = 10, use_cache: bool = True) -> int:
+        """Pick the tile-kernel launch size by timing; returns the choice."""
+        check(self._L.pg_mpm_autotune(self._h, int(trial_steps), 1 if use_cache else 0, _stream_ptr()), self._h)
+        return self.launch
 
     def set_particles(self, x, v, C=None, F=None, mass=None, E=None):
         x = _dev_f32(x, 3, "x")
@@ -195,6 +216,69 @@ class MPMContext:
         s = ctypes.c_longlong()
         check(self._L.pg_mpm_get_time(self._h, ctypes.byref(t), ctypes.byref(s)), self._h)
         return t.value, s.value
+
+
+def scene_desc(dims, dx, origin=(0.0, 0.0, 0.0), dt=1e-4, gravity=(0.0, -9.8, 0.0), material=0,
+               youngs_modulus=1e4, poisson_ratio=0.2, density=1.0, particle_volume=1e-6, boundary_thickness=3,
+               max_particles=1 << 16) -> PgSceneDesc:
+    d = PgSceneDesc()
+    d.nx, d.ny, d.nz = (int(v) for v in dims)
+    d.dx = float(dx)
+    d.ox, d.oy, d.oz = (float(v) for v in origin)
+    d.dt = float(dt)
+    d.gx, d.gy, d.gz = (float(v) for v in gravity)
+    d.material = int(material)
+    d.youngs_modulus = float(youngs_modulus)
+    d.poisson_ratio = float(poisson_ratio)
+    d.density = float(density)
+    d.particle_volume = float(particle_volume)
+    d.boundary_thickness = int(boundary_thickness)
+    d.max_particles = int(max_particles)
+    return d
+
+
+class MPMBatch(MPMContext):
+    """Many independent MLS-MPM scenes advanced by the same launches (one
+    pg_mpm_ctx built by pg_mpm_batch_create).  Particles of all scenes are
+    concatenated in scene order; get_particles returns them in that order and
+    get_grid returns the concatenated grids."""
+
+    def __init__(self, scenes: Sequence[PgSceneDesc]):
+        self._L = load()
+        self.scenes = list(scenes)
+        arr = (PgSceneDesc * len(self.scenes))(*self.scenes)
+        h = ctypes.c_void_p()
+        check(self._L.pg_mpm_batch_create(arr, len(self.scenes), ctypes.byref(h)))
+        self._h = h
+        self.params = None
+        self._nodes = sum(d.nx * d.ny * d.nz for d in self.scenes)
+        self.counts = [0] * len(self.scenes)
+
+    @property
+    def num_scenes(self) -> int:
+        return int(self._L.pg_mpm_num_scenes(self._h))
+
+    def set_particles(self, counts, x, v, C=None, F=None, mass=None, E=None):
+        counts = [int(c) for c in counts]
+        if len(counts) != len(self.scenes):
+            raise ValueError(f"{len(counts)} counts for {len(self.scenes)} scenes")
+        x = _dev_f32(x, 3, "x")
+        v = _dev_f32(v, 3, "v")
+        n = x.shape[0]
+        if n != sum(counts):
+            raise ValueError(f"{n} particles but counts sum to {sum(counts)}")
+        C = _dev_f32(C, 9, "C")
+        F = _dev_f32(F, 9, "F")
+        mass = _dev_f32(mass, 1, "mass")
+        E = _dev_f32(E, 1, "E")
+        for name, t, cols in (("v", v, 3), ("C", C, 9), ("F", F, 9), ("mass", mass, 1), ("E", E, 1)):
+            if t is not None and t.numel() != n * cols:
+                raise ValueError(f"{name} has {t.numel()} values, expected {n * cols}")
+        carr = (ctypes.c_int * len(counts))(*counts)
+        check(self._L.pg_mpm_batch_set_particles(self._h, carr, _dptr(x), _dptr(v), _dptr(C), _dptr(F),
+                                                 _dptr(mass), _dptr(E), _lib.DEVICE,
+                                                 _stream_ptr()), self._h)
+        self.counts = counts
 
 
 @dataclass

@@ -58,7 +58,7 @@ def write_step_fixtures():
             np.savez_compressed(os.path.join(HERE, f"refcompat_{name}_o{order}.npz"), **out)
 
 
-def config1_inputs():
+def single_box_inputs():
     """64^3 grid, dx = 1/64, 32,768 particles uniform in [0.3,0.5]^3 (seed 1), v = (0,-1,0)."""
     rng = np.random.default_rng(1)
     n = 32768
@@ -68,8 +68,8 @@ def config1_inputs():
     return x, v, m
 
 
-def write_config1_fixture(nsteps=200):
-    x, v, m = config1_inputs()
+def write_single_box_fixture(nsteps=200):
+    x, v, m = single_box_inputs()
     prm = O.ref_params((64, 64, 64), (1 / 64,) * 3, (0, 0, 0), 1e-4, (0, -9.8, 0), 2)
     sample = np.linspace(0, x.shape[0] - 1, 64).astype(np.int64)
     sums = np.zeros((nsteps, 10), np.float64)
@@ -82,7 +82,7 @@ def write_config1_fixture(nsteps=200):
         if s in (1, 10, 50, 100, 200):
             samples[f"x_{s}"] = cx[sample]
             samples[f"v_{s}"] = cv[sample]
-    np.savez_compressed(os.path.join(HERE, "refcompat_config1.npz"), x0=x, v0=v, m=m, sample=sample,
+    np.savez_compressed(os.path.join(HERE, "refcompat_single_box.npz"), x0=x, v0=v, m=m, sample=sample,
                         sums=sums, **samples)
 
 
@@ -104,5 +104,5 @@ if __name__ == "__main__":
         O.build(ref=True)
     write_step_fixtures()
     write_hash_fixtures()
-    write_config1_fixture()
+    write_single_box_fixture()
     print("fixtures written to", HERE)

@@ -28,7 +28,7 @@ def scene(rng, n, material, grid=16, span=0.35):
 
 
 def oracle_reverse(O, sc, x, v, C, F, E, nsteps, seeds):
-    prm = sc.oracle_params()
+    prm = O.scene_params(sc)
     states = [(x, v, C, F)]
     for _ in range(nsteps):
         o = O.mls_run(prm, *states[-1], sc.p_mass, E, 1)
@@ -87,9 +87,21 @@ def test_backward_requires_forward(gpu):
     ctx.close()
 
 
+def _autograd_case(rng, n):
+    sc = scene(rng, n, 0, grid=32, span=0.2)
+    sc.dt = 1e-4
+    x64 = sc.x.astype(np.float64)
+    c = x64.mean(0)
+    # fixed deformation field (compression + spin) so that the stress, and
+    # with it E, changes the trajectory
+    w = -3.0 * (x64 - c) + np.cross([0.0, 0.0, 4.0], x64 - c)
+    return sc, w
+
+
 def test_autograd_loss_gradient_vs_fd(gpu, oracle_lib):
-    """Centre-of-mass loss after 24 steps (fixed-corotated, per-particle E):
-    d loss / d v0 (shared, 3 values) and d loss / d E (8 sampled particles) via
+    """Loss = mean_i |x_i(T) - target|^2 after 24 steps (fixed-corotated,
+    per-particle E, initial v = shared v0 + a fixed deformation field):
+    d loss / d v0 (3 values) and d loss / d E (8 sampled particles) via
     torch.autograd through the GPU adjoint, against FP64 central differences of
     the oracle (eps 1e-4 on v0, relative 1e-4 on E)."""
     from physgrad_amd.autograd import physics_simulation
@@ -97,25 +109,25 @@ def test_autograd_loss_gradient_vs_fd(gpu, oracle_lib):
     O = oracle_lib
     rng = np.random.default_rng(3)
     n, nsteps = 1024, 24
-    sc = scene(rng, n, 0, grid=32, span=0.2)
-    sc.dt = 1e-4
+    sc, w = _autograd_case(rng, n)
     target = np.array([0.6, 0.3, 0.5])
     v0 = np.array([0.3, -0.5, 0.2])
-    prm = sc.oracle_params()
+    prm = O.scene_params(sc)
 
     def loss64(v0_, E_):
-        vv = np.tile(v0_, (n, 1))
-        o = O.mls_run(prm, sc.x.astype(np.float64), vv, None, None, sc.p_mass, E_, nsteps)
-        d = o["x"].mean(0) - target
-        return float(d @ d)
+        o = O.mls_run(prm, sc.x.astype(np.float64), np.tile(v0_, (n, 1)) + w, None, None, sc.p_mass, E_, nsteps)
+        d = o["x"] - target
+        return float((d * d).sum(1).mean())
 
     ctx = MPMContext(sc.params(impl=1))
     x0 = torch.as_tensor(sc.x, device="cuda")
+    w_t = torch.as_tensor(w, dtype=torch.float32, device="cuda")
     v0_t = torch.tensor(v0, dtype=torch.float32, device="cuda", requires_grad=True)
     E_t = torch.as_tensor(sc.E_particles, device="cuda").clone().requires_grad_(True)
-    xN, _, _, _ = physics_simulation(ctx, x0, v0_t.expand(n, 3), E=E_t, num_steps=nsteps, checkpoint_every=8)
-    d = xN.double().mean(0) - torch.tensor(target, device="cuda")
-    loss = (d * d).sum()
+    xN, _, _, _ = physics_simulation(ctx, x0, v0_t.expand(n, 3) + w_t, E=E_t, num_steps=nsteps,
+                                     checkpoint_every=8)
+    d = xN.double() - torch.tensor(target, device="cuda")
+    loss = (d * d).sum(1).mean()
     loss.backward()
     gv = v0_t.grad.cpu().numpy()
     gE = E_t.grad.cpu().numpy()
@@ -135,5 +147,33 @@ def test_autograd_loss_gradient_vs_fd(gpu, oracle_lib):
         Em[p] -= h
         fd = (loss64(v0, Ep) - loss64(v0, Em)) / (2 * h)
         print("dE", p, fd, gE[p])
+        assert abs(fd) > 1e-15
         assert abs(fd - gE[p]) <= 1e-3 * abs(fd) + 1e-3 * np.abs(gE).max(), (p, fd, gE[p])
+    ctx.close()
+
+
+def test_centroid_gradient_wrt_E_vanishes(gpu):
+    """Internal forces conserve momentum, so away from the walls the centre of
+    mass does not depend on the elastic constants: d centroid-loss / d E must
+    be zero up to FP32 round-off, while the same rollout's d loss / d v0 is
+    not."""
+    from physgrad_amd.autograd import physics_simulation
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(3)
+    n, nsteps = 1024, 24
+    sc, w = _autograd_case(rng, n)
+    ctx = MPMContext(sc.params(impl=1))
+    x0 = torch.as_tensor(sc.x, device="cuda")
+    w_t = torch.as_tensor(w, dtype=torch.float32, device="cuda")
+    v0_t = torch.tensor([0.3, -0.5, 0.2], dtype=torch.float32, device="cuda", requires_grad=True)
+    E_t = torch.as_tensor(sc.E_particles, device="cuda").clone().requires_grad_(True)
+    xN, _, _, _ = physics_simulation(ctx, x0, v0_t.expand(n, 3) + w_t, E=E_t, num_steps=nsteps,
+                                     checkpoint_every=8)
+    d = xN.double().mean(0) - torch.tensor([0.6, 0.3, 0.5], device="cuda")
+    (d * d).sum().backward()
+    gE = E_t.grad.abs().max().item()
+    gv = v0_t.grad.abs().max().item()
+    # dL/dlog E (= E dL/dE) must be negligible next to the v0 sensitivity
+    assert gv > 1e-4
+    assert gE * float(sc.E_particles.mean()) < 1e-6 * gv, (gE, gv)
     ctx.close()

@@ -48,7 +48,7 @@ def test_step_parity_vs_fp64_oracle(gpu, oracle_lib, impl, material):
     O = oracle_lib
     rng = np.random.default_rng(100 + material)
     sc, C, F = small_scene(rng, material=material)
-    prm = sc.oracle_params()
+    prm = O.scene_params(sc)
     x, v = sc.x.astype(np.float64), sc.v.astype(np.float64)
     C64, F64 = C.astype(np.float64), F.astype(np.float64)
     E = sc.E_particles.astype(np.float64)
@@ -66,13 +66,13 @@ def test_step_parity_vs_fp64_oracle(gpu, oracle_lib, impl, material):
 
 
 def test_error_vs_fp64_after_n_steps(gpu, oracle_lib):
-    """Config-1 workload (64^3, 32,768 particles, fixed-corotated): the FP32
+    """Single-box workload (64^3, 32,768 particles, fixed-corotated): the FP32
     tiled GPU trajectory against the FP64 oracle after 50 steps."""
     from physgrad_amd.scenes import single_box
     O = oracle_lib
     sc = single_box()
     nsteps = 50
-    want = O.mls_run(sc.oracle_params(), sc.x, sc.v, None, None, sc.p_mass, sc.E, nsteps)
+    want = O.mls_run(O.scene_params(sc), sc.x, sc.v, None, None, sc.p_mass, sc.E, nsteps)
     got = gpu_step(sc, 1, sc.x, sc.v, None, None, None, steps=nsteps)
     ex = rel(got["x"], want["x"])
     ev = rel(got["v"], want["v"])

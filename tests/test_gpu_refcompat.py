@@ -55,12 +55,12 @@ def test_refcompat_vs_reference_fixtures(gpu, path):
     assert abs(s.getCurrentTime() - 10 * float(f["dt"])) < 1e-9
 
 
-def test_refcompat_config1_per_step(gpu, oracle_lib):
-    """Config-1 geometry: one-step parity from identical inputs at steps
+def test_refcompat_single_box_per_step(gpu, oracle_lib):
+    """Single-box geometry (64^3, 32,768 particles): one-step parity from identical inputs at steps
     0, 50, 150 of the reference trajectory, and the 200-step trajectory against
     the reference's per-step checksums."""
     O = oracle_lib
-    f = np.load(os.path.join(GOLDEN, "refcompat_config1.npz"))
+    f = np.load(os.path.join(GOLDEN, "refcompat_single_box.npz"))
     dims, cell, dt, grav = (64, 64, 64), (1 / 64,) * 3, 1e-4, (0, -9.8, 0)
     prm = O.ref_params(dims, cell, (0, 0, 0), dt, grav, 2)
     x, v, m = f["x0"], f["v0"], f["m"]
@@ -89,7 +89,7 @@ def test_refcompat_config1_per_step(gpu, oracle_lib):
                              *p["x"].double().sum(0).tolist(), *p["v"].double().sum(0).tolist()])
             want = f["sums"][k - 1]
             worst = max(worst, float(np.max(np.abs(sums - want) / np.maximum(np.abs(want), 1e-3))))
-    print(f"config-1 200-step checksum drift (max rel): {worst:.3e}")
+    print(f"single-box 200-step checksum drift (max rel): {worst:.3e}")
     assert worst < 1e-4
 
 

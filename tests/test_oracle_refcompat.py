@@ -30,11 +30,11 @@ def test_restatement_bitwise_vs_reference_fixtures(oracle_lib, path):
             assert np.array_equal(got.reshape(want.shape).view(np.uint32), want.view(np.uint32)), (key, steps)
 
 
-def test_config1_checksums_and_samples(oracle_lib):
-    """Config-1 geometry (64^3, 32,768 particles): per-step sums and sampled
+def test_single_box_checksums_and_samples(oracle_lib):
+    """Single-box geometry (64^3, 32,768 particles): per-step sums and sampled
     particles of the reference for 200 steps, bit for bit on the samples."""
     O = oracle_lib
-    f = np.load(os.path.join(GOLDEN, "refcompat_config1.npz"))
+    f = np.load(os.path.join(GOLDEN, "refcompat_single_box.npz"))
     prm = O.ref_params((64, 64, 64), (1 / 64,) * 3, (0, 0, 0), 1e-4, (0, -9.8, 0), 2)
     x, v, m = f["x0"], f["v0"], f["m"]
     checkpoints = (1, 10, 50)
