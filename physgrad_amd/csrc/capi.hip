@@ -11,7 +11,7 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/physgrad_mpm.h"
+#include "physgrad_mpm.h"
 #include "pg_kernels.h"
 
 namespace {

@@ -13,7 +13,7 @@
 // also defines start for empty cells as the reference's prefix sum does.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "../../include/physgrad_mpm.h"
+#include "physgrad_mpm.h"
 
 namespace pg {
 namespace {

@@ -13,7 +13,7 @@
 // Launch shape: 256 threads, one particle per thread.
 #include "pg_device.h"
 #include "pg_kernels.h"
-#include "../../include/physgrad_mpm.h"
+#include "physgrad_mpm.h"
 
 namespace pg {
 namespace {

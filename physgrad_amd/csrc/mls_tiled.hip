@@ -33,6 +33,7 @@ namespace pg {
 namespace {
 
 constexpr int PASS = 2048;       // particles sorted per LDS pass of a tile
+constexpr int NW = TB / 64;      // waves per workgroup
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 
 struct TileArgs {
@@ -45,7 +46,7 @@ struct TileArgs {
     const float* m_in;
     float* m_out;
     const uint32_t* perm;
-    const uint4* alist;      // (tile, start, count, 0)
+    const uint4* alist;      // (tile, start, count, scene)
     const uint32_t* nactive;
     uint32_t* key_out;
     uint32_t* rank_out;
@@ -77,25 +78,145 @@ __device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, in
     }
 }
 
-// MODE bit 1: G2P (reads x, F; writes the new state at the cell-sorted position)
-// MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state)
+// Keeps the compiler from moving LDS accesses across this point.  The window
+// updates of one wave are read-modify-writes whose lanes touch overlapping
+// nodes in successive instructions; the LDS executes a wave's instructions in
+// issue order, so program order is all they need.
+__device__ __forceinline__ void lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+// G2P sums v = sum_n w u_n and C = 4/dx sum_n w u_n (n - f)^T over the 27
+// stencil nodes, factored along the axes (x innermost).  load(x, y, z) gives
+// the (v, m) of stencil node (x, y, z).
+template <class Load>
+__device__ __forceinline__ void g2p_sum(Load load, const Stencil& st, float inv_dx, float* v, float* C) {
+    float wd[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) wd[a][n] = st.w[a][n] * ((float)n - st.f[a]);
+    float vv[3] = {0.f, 0.f, 0.f}, cx[3] = {0.f, 0.f, 0.f}, cy[3] = {0.f, 0.f, 0.f}, cz[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int z = 0; z < 3; ++z) {
+        float R[3] = {0.f, 0.f, 0.f}, Ry[3] = {0.f, 0.f, 0.f}, Rx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const float4 u = load(x, y, z);
+                const float uu[3] = {u.x, u.y, u.z};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    P[c] += st.w[0][x] * uu[c];
+                    Q[c] += wd[0][x] * uu[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                R[c] += st.w[1][y] * P[c];
+                Ry[c] += wd[1][y] * P[c];
+                Rx[c] += st.w[1][y] * Q[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vv[c] += st.w[2][z] * R[c];
+            cz[c] += wd[2][z] * R[c];
+            cy[c] += st.w[2][z] * Ry[c];
+            cx[c] += st.w[2][z] * Rx[c];
+        }
+        // at most one 9-node slice of loads in flight: bounds the registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const float s = 4.f * inv_dx;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = vv[c];
+        C[3 * c + 0] = s * cx[c];
+        C[3 * c + 1] = s * cy[c];
+        C[3 * c + 2] = s * cz[c];
+    }
+}
+
+// P2G of one particle: node n = base + (x, y, z) receives w_n (m, q + a0 x + a1 y + a2 z)
+// with A = -dt V0 4/dx^2 tau + m C, q = m v - dx A f, a_j = dx A[:, j].
+struct P2GVal {
+    float m;
+    float q[3], a0[3], a1[3], a2[3];
+};
+
+__device__ __forceinline__ void p2g_prep(const MlsGeom& g, float Eq, float mq, const float* F, const float* C,
+                                         const float* v, const Stencil& st, P2GVal& p) {
+    const float kscale = -g.dt * g.p_vol * 4.f * g.inv_dx * g.inv_dx;
+    const float mu = Eq / (2.f * (1.f + g.nu));
+    const float lam = Eq * g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+    float tau[9];
+    kirchhoff(g.material, mu, lam, F, tau);
+    float A[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
+    p.m = mq;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        p.q[c] = mq * v[c] - g.dx * (A[3 * c] * st.f[0] + A[3 * c + 1] * st.f[1] + A[3 * c + 2] * st.f[2]);
+        p.a0[c] = g.dx * A[3 * c];
+        p.a1[c] = g.dx * A[3 * c + 1];
+        p.a2[c] = g.dx * A[3 * c + 2];
+    }
+}
+
+// visits the 27 contributions: emit(x, y, z, float4(w m, w p))
+template <class Emit>
+__device__ __forceinline__ void p2g_visit(const Stencil& st, const P2GVal& p, Emit emit) {
+#pragma unroll
+    for (int z = 0; z < 3; ++z)
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            const float wyz = st.w[1][y] * st.w[2][z];
+            float b[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) b[c] = p.q[c] + p.a1[c] * (float)y + p.a2[c] * (float)z;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const float w = wyz * st.w[0][x];
+                emit(x, y, z, make_float4(w * p.m, w * (b[0] + p.a0[0] * (float)x), w * (b[1] + p.a0[1] * (float)x),
+                                          w * (b[2] + p.a0[2] * (float)x)));
+            }
+        }
+}
+
+// Tile kernel.  MODE bit 1: G2P (reads x, F; writes the new state);
+// MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state).
+//
+// Per pass of up to PASS particles of the tile: an LDS counting sort by
+// stencil-base cell (64 cells, plus "outside" for clamped keys), then rounds:
+// in round r lane c of a wave takes the r-th particle of cell c, waves take
+// rounds in turn.  All lanes of a round sit in distinct cells, so the 27 node
+// updates of a lane's P2G never meet another lane's in the same instruction,
+// and each wave accumulates into its own LDS window with plain
+// read-modify-writes in program order.  A particle that left its cell during
+// G2P adds with LDS atomics (inside the window) or global atomics (outside).
+// Outputs are written round by round, so stores and the next step's loads
+// coalesce.  At the end of the tile the four windows are summed and added to
+// the grid.
 template <int MODE>
-__global__ void __launch_bounds__(TB) k_tile(TileArgs a) {
+__global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     constexpr bool G2P = (MODE & 1) != 0;
     constexpr bool P2G = (MODE & 2) != 0;
-    constexpr bool SORT = P2G;           // local cell order matters for the P2G reduction
-    constexpr int NP = 36;               // 9 nodes x 4 channels per slice
-    __shared__ float4 sv[WV];            // grid (v, m) window
-    __shared__ float sm[4 * WV];         // P2G accumulation window, channel-major
-    __shared__ float scr[TB / 64][64 * NP];
-    __shared__ uint16_t lperm[PASS];     // local cell-sorted order within the pass
-    __shared__ uint16_t ldir[PASS];      // (direction << 11 | rank) of the new tile, or FAR
+    __shared__ float4 sv[G2P ? WV : 1];          // grid (v, m) window
+    __shared__ float4 pw[P2G ? NW : 1][WV];      // per-wave P2G windows (m, p)
+    __shared__ uint16_t lperm[PASS];             // local cell-sorted order within the pass
+    __shared__ uint16_t rs[PASS];                // first output offset of each round
+    __shared__ uint16_t ldir[(G2P && P2G) ? PASS : 1];  // (direction << 11 | rank) of the new tile, or FAR
     __shared__ uint32_t lcnt[65], lstart[65];
     __shared__ uint32_t dcnt[27], dbase[27];
+    __shared__ uint32_t s_max, s_tot;
 
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float* myscr = scr[wave];
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int ccx = lane & 3, ccy = (lane >> 2) & 3, ccz = lane >> 4;     // this lane's cell in a round
+    const int cnb = (ccz * WN + ccy) * WN + ccx;                          // its base node in the window
 
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
@@ -104,216 +225,212 @@ __global__ void __launch_bounds__(TB) k_tile(TileArgs a) {
         const float4* __restrict__ gvs = a.gv + sc.goff;
         float4* __restrict__ mps = a.mp + sc.goff;
         uint32_t* __restrict__ flags = a.flags + sc.boff;
-        const float s4 = 4.f * g.inv_dx * g.inv_dx;
-        const float kscale = -g.dt * g.p_vol * s4;
-        const float emu = 1.f / (2.f * (1.f + g.nu));
-        const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
         const uint32_t tile = te.x - sc.toff, beg = te.y, cnt = te.z;
         int t0[3];
         tile_origin(tile, t0);
         const int tc[3] = {t0[0] / TT, t0[1] / TT, t0[2] / TT};
         for (int l = tid; l < WV; l += TB) {
             if (G2P) {
-                int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
+                const int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
                 float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
                 sv[l] = u;
             }
-            if (P2G) { sm[l] = 0.f; sm[WV + l] = 0.f; sm[2 * WV + l] = 0.f; sm[3 * WV + l] = 0.f; }
+            if (P2G) {
+#pragma unroll
+                for (int w = 0; w < NW; ++w) pw[w][l] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
 
         for (uint32_t pb = 0; pb < cnt; pb += PASS) {
             const uint32_t pc = min((uint32_t)PASS, cnt - pb);
-            // ---- local counting sort of the pass by stencil-base cell (64 cells + "outside")
-            if (SORT) {
-                if (tid < 65) lcnt[tid] = 0;
-                if (tid < 27) dcnt[tid] = 0;
-                __syncthreads();
-                uint32_t key[PASS / TB];
+            // ---- counting sort of the pass by stencil-base cell
+            if (tid < 65) lcnt[tid] = 0;
+            if (tid < 27) dcnt[tid] = 0;
+            __syncthreads();
+            uint32_t key[PASS / TB];
 #pragma unroll
-                for (int k = 0; k < PASS / TB; ++k) {
-                    const uint32_t r = tid + k * TB;
-                    key[k] = 0xFFFFFFFFu;
-                    if (r < pc) {
-                        const float4 q = a.Sin[a.perm[beg + pb + r]];
-                        const float xx[3] = {q.x, q.y, q.z};
-                        int cell = 64;
-                        int l3[3];
-                        bool in = true;
+            for (int k = 0; k < PASS / TB; ++k) {
+                const uint32_t r = tid + k * TB;
+                key[k] = 0u;
+                if (r < pc) {
+                    const float4 q = a.Sin[a.perm[beg + pb + r]];
+                    const float xx[3] = {q.x, q.y, q.z};
+                    int l3[3];
+                    bool in = true;
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            l3[d] = (int)floorf((xx[d] - g.o[d]) * g.inv_dx - 0.5f) - t0[d];
-                            in = in && l3[d] >= 0 && l3[d] < TT;
-                        }
-                        if (in) cell = l3[0] + TT * l3[1] + TT * TT * l3[2];
-                        const uint32_t rk = atomicAdd(&lcnt[cell], 1u);
-                        key[k] = ((uint32_t)cell << 16) | rk;
+                    for (int d = 0; d < 3; ++d) {
+                        l3[d] = (int)floorf((xx[d] - g.o[d]) * g.inv_dx - 0.5f) - t0[d];
+                        in = in && l3[d] >= 0 && l3[d] < TT;
                     }
-                }
-                __syncthreads();
-                if (tid == 0) {
-                    uint32_t run = 0;
-                    for (int c = 0; c < 65; ++c) {
-                        lstart[c] = run;
-                        run += lcnt[c];
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < PASS / TB; ++k) {
-                    const uint32_t r = tid + k * TB;
-                    if (r < pc) lperm[lstart[key[k] >> 16] + (key[k] & 0xFFFFu)] = (uint16_t)r;
+                    const int cell = in ? l3[0] + TT * l3[1] + TT * TT * l3[2] : 64;
+                    const uint32_t rk = atomicAdd(&lcnt[cell], 1u);
+                    key[k] = ((uint32_t)cell << 16) | rk;
                 }
             }
             __syncthreads();
-
-            // ---- transfers, 256 particles at a time in cell order
-            for (uint32_t c0 = 0; c0 < pc; c0 += TB) {
-                const uint32_t j = c0 + tid;
-                const bool act = j < pc;
-                const uint32_t r = act ? (SORT ? (uint32_t)lperm[j] : j) : 0u;
-                const uint32_t s = act ? a.perm[beg + pb + r] : 0u;
-                const uint32_t o = beg + pb + j;
-                float x[3] = {0.f, 0.f, 0.f}, F[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
-                float v[3] = {0.f, 0.f, 0.f}, C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                float Eq = g.E, mq = g.p_mass;
-                if (act) {
-                    if (a.E_in) Eq = a.E_in[s];
-                    if (a.m_in) mq = a.m_in[s];
+            if (wave == 0) {
+                // cell starts (wave scan), the largest cell count, round offsets
+                const uint32_t cl = lcnt[lane];
+                uint32_t inc = cl;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t t = __shfl_up(inc, off);
+                    if (lane >= off) inc += t;
                 }
+                lstart[lane] = inc - cl;
+                if (lane == 63) lstart[64] = inc;
+                uint32_t mx = cl;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+                uint32_t run = 0;
+                for (uint32_t r = 0; r < mx; ++r) {
+                    const uint64_t m = __ballot(cl > r);
+                    if (lane == 0) rs[r] = (uint16_t)run;
+                    run += (uint32_t)__popcll(m);
+                }
+                if (lane == 0) {
+                    s_max = mx;
+                    s_tot = run;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PASS / TB; ++k) {
+                const uint32_t r = tid + k * TB;
+                if (r < pc) lperm[lstart[key[k] >> 16] + (key[k] & 0xFFFFu)] = (uint16_t)r;
+            }
+            __syncthreads();
+
+            // one particle: li = index within the pass (storage order), oo = output offset,
+            // regular = it is this lane's cell in a round (G2P from the window, P2G by RMW)
+            auto particle = [&](uint32_t li, uint32_t oo, bool regular) {
+                const uint32_t s = a.perm[beg + pb + li];
+                const uint32_t o = beg + pb + oo;
+                float Eq = g.E, mq = g.p_mass;
+                if (a.E_in) Eq = a.E_in[s];
+                if (a.m_in) mq = a.m_in[s];
+                float x[3], F[9], v[3], C[9];
                 if (G2P) {
-                    if (act) {
-                        load_xF(a.Sin, a.cap, s, x, F);
-                        Stencil st;
-                        make_stencil(x, g.o, g.inv_dx, st);
-                        const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-                        const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
+                    load_xF(a.Sin, a.cap, s, x, F);
+                    Stencil st;
+                    make_stencil(x, g.o, g.inv_dx, st);
+                    // (the base equals the lane's cell by construction of the sort; checked so
+                    // that a different rounding of the two computations cannot matter)
+                    if (regular && st.b[0] - t0[0] == ccx && st.b[1] - t0[1] == ccy && st.b[2] - t0[2] == ccz) {
+                        g2p_sum([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; }, st,
+                                g.inv_dx, v, C);
+                    } else {
+                        g2p_sum(
+                            [&](int ix, int iy, int iz) {
+                                const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+                                return in_grid(g, gi, gj, gk) ? gvs[node_id(g, gi, gj, gk)]
+                                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+                            },
+                            st, g.inv_dx, v, C);
+                    }
+                    float Fn[9];
 #pragma unroll
-                        for (int ii = 0; ii < 3; ++ii)
+                    for (int i = 0; i < 3; ++i)
 #pragma unroll
-                            for (int jj = 0; jj < 3; ++jj)
+                        for (int j = 0; j < 3; ++j)
+                            Fn[3 * i + j] = F[3 * i + j] + g.dt * (C[3 * i] * F[j] + C[3 * i + 1] * F[3 + j] +
+                                                                   C[3 * i + 2] * F[6 + j]);
 #pragma unroll
-                                for (int kk = 0; kk < 3; ++kk) {
-                                    const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                                    const float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
-                                                        ((float)kk - st.f[2]) * g.dx};
-                                    float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-                                    if (inwin) {
-                                        u = sv[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
-                                    } else {
-                                        const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                                        if (in_grid(g, gi, gj, gk)) u = gvs[node_id(g, gi, gj, gk)];
-                                    }
-                                    const float uv[3] = {u.x, u.y, u.z};
+                    for (int c = 0; c < 9; ++c) F[c] = Fn[c];
 #pragma unroll
-                                    for (int c = 0; c < 3; ++c) {
-                                        const float wu = w * uv[c];
-                                        v[c] += wu;
-#pragma unroll
-                                        for (int e = 0; e < 3; ++e) C[3 * c + e] += wu * d[e];
-                                    }
-                                }
-#pragma unroll
-                        for (int c = 0; c < 9; ++c) C[c] *= s4;
-                        float Fn[9];
-#pragma unroll
-                        for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-                            for (int jj = 0; jj < 3; ++jj)
-                                Fn[3 * ii + jj] = F[3 * ii + jj] + g.dt * (C[3 * ii] * F[jj] + C[3 * ii + 1] * F[3 + jj] +
-                                                                           C[3 * ii + 2] * F[6 + jj]);
-#pragma unroll
-                        for (int c = 0; c < 9; ++c) F[c] = Fn[c];
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
-                        store_state(a.Sout, a.cap, o, x, F, v, C);
-                        a.pid_out[o] = a.pid_in[s];
-                        if (a.E_out && a.E_in) a.E_out[o] = Eq;
-                        if (a.m_out && a.m_in) a.m_out[o] = mq;
-                        if (a.src_out) a.src_out[o] = s;
-                        if (P2G) {
-                            int nt[3];
-                            tile_coords(x, g, sc.ntx, sc.nty, sc.ntz, nt);
-                            const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
-                            const uint32_t k = sc.toff + morton_tile(nt[0], nt[1], nt[2]);
-                            a.key_out[o] = k;
-                            if (dx_ >= -1 && dx_ <= 1 && dy_ >= -1 && dy_ <= 1 && dz_ >= -1 && dz_ <= 1) {
-                                const int dir = (dx_ + 1) + 3 * (dy_ + 1) + 9 * (dz_ + 1);
-                                const uint32_t rk = atomicAdd(&dcnt[dir], 1u);
-                                ldir[j] = (uint16_t)((dir << 11) | rk);
-                            } else {
-                                ldir[j] = FAR;
-                                a.rank_out[o] = atomicAdd(a.count + k, 1u);
-                            }
+                    for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
+                    store_state(a.Sout, a.cap, o, x, F, v, C);
+                    a.pid_out[o] = a.pid_in[s];
+                    if (a.E_out && a.E_in) a.E_out[o] = Eq;
+                    if (a.m_out && a.m_in) a.m_out[o] = mq;
+                    if (a.src_out) a.src_out[o] = s;
+                    if (P2G) {
+                        int nt[3];
+                        tile_coords(x, g, sc.ntx, sc.nty, sc.ntz, nt);
+                        const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
+                        const uint32_t k = sc.toff + morton_tile(nt[0], nt[1], nt[2]);
+                        a.key_out[o] = k;
+                        const bool near = dx_ >= -1 && dx_ <= 1 && dy_ >= -1 && dy_ <= 1 && dz_ >= -1 && dz_ <= 1;
+                        const int dir = near ? (dx_ + 1) + 3 * (dy_ + 1) + 9 * (dz_ + 1) : -1;
+                        // most particles stay in the tile: one LDS atomic per wave for those
+                        const uint64_t stay = __ballot(dir == 13);
+                        uint32_t rk = 0;
+                        if (stay) {
+                            const int leader = __ffsll((unsigned long long)stay) - 1;
+                            uint32_t b0 = 0;
+                            if (lane == leader) b0 = atomicAdd(&dcnt[13], (uint32_t)__popcll(stay));
+                            b0 = __builtin_amdgcn_readlane(b0, leader);
+                            rk = b0 + (uint32_t)__popcll(stay & lt);
+                        }
+                        if (dir >= 0 && dir != 13) rk = atomicAdd(&dcnt[dir], 1u);
+                        if (dir >= 0) {
+                            ldir[oo] = (uint16_t)((dir << 11) | rk);
+                        } else {
+                            ldir[oo] = FAR;
+                            a.rank_out[o] = atomicAdd(a.count + k, 1u);
                         }
                     }
-                } else if (act) {
+                } else {
                     load_state(a.Sin, a.cap, s, x, F, v, C);
                 }
                 if (P2G) {
-                    // scatter the (new) state: m_i += w m, p_i += w (m v + A (x_i - x_p))
-                    float A[9];
-                    {
-                        float tau[9];
-                        kirchhoff(g.material, Eq * emu, Eq * elam, F, tau);
-#pragma unroll
-                        for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
-                    }
                     Stencil st;
                     make_stencil(x, g.o, g.inv_dx, st);
+                    P2GVal p;
+                    p2g_prep(g, Eq, mq, F, C, v, st, p);
                     const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-                    const bool inwin = act && l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
-                    // contribution at offset d: w(d) (m, q + a0 d0 + a1 d1 + a2 d2),
-                    // q = m v - dx A f, a_j = dx A[:, j]
-                    float q3[3], a0[3], a1[3], a2[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        q3[c] = mq * v[c] - g.dx * (A[3 * c] * st.f[0] + A[3 * c + 1] * st.f[1] + A[3 * c + 2] * st.f[2]);
-                        a0[c] = g.dx * A[3 * c];
-                        a1[c] = g.dx * A[3 * c + 1];
-                        a2[c] = g.dx * A[3 * c + 2];
+                    const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
+                    const int nb = (l2 * WN + l1) * WN + l0;
+                    float4* __restrict__ win = pw[wave];
+                    if (regular && inwin && nb == cnb) {
+                        p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                            float4* t = win + nb + (iz * WN + iy) * WN + ix;
+                            float4 u = *t;
+                            u.x += c.x;
+                            u.y += c.y;
+                            u.z += c.z;
+                            u.w += c.w;
+                            *t = u;
+                            lds_order();
+                        });
+                    } else if (inwin) {
+                        float* wf = reinterpret_cast<float*>(win);
+                        p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                            float* t = wf + 4 * (nb + (iz * WN + iy) * WN + ix);
+                            atomicAdd(t + 0, c.x);
+                            atomicAdd(t + 1, c.y);
+                            atomicAdd(t + 2, c.z);
+                            atomicAdd(t + 3, c.w);
+                        });
+                    } else {
+                        p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+                            if (!in_grid(g, gi, gj, gk)) return;
+                            float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
+                            atomic_add_f32(cp + 0, c.x);
+                            atomic_add_f32(cp + 1, c.y);
+                            atomic_add_f32(cp + 2, c.z);
+                            atomic_add_f32(cp + 3, c.w);
+                            flag_node(flags, sc, gi, gj, gk);
+                        });
                     }
-                    if (act && !inwin) {
-#pragma unroll
-                        for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-                            for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-                                for (int kk = 0; kk < 3; ++kk) {
-                                    const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                                    if (!in_grid(g, gi, gj, gk)) continue;
-                                    const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                                    float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
-                                    atomic_add_f32(cp + 0, w * mq);
-#pragma unroll
-                                    for (int c = 0; c < 3; ++c)
-                                        atomic_add_f32(cp + 1 + c, w * (q3[c] + a0[c] * ii + a1[c] * jj + a2[c] * kk));
-                                    flag_node(flags, sc, gi, gj, gk);
-                                }
-                    }
-                    const int base = inwin ? (l2 * WN + l1) * WN + l0 : -1;
-#pragma unroll
-                    for (int ii = 0; ii < 3; ++ii) {
-#pragma unroll
-                        for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-                            for (int kk = 0; kk < 3; ++kk) {
-                                float4 cv = make_float4(0.f, 0.f, 0.f, 0.f);
-                                if (inwin) {
-                                    const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                                    cv.x = w * mq;
-                                    cv.y = w * (q3[0] + a0[0] * ii + a1[0] * jj + a2[0] * kk);
-                                    cv.z = w * (q3[1] + a0[1] * ii + a1[1] * jj + a2[1] * kk);
-                                    cv.w = w * (q3[2] + a0[2] * ii + a1[2] * jj + a2[2] * kk);
-                                }
-                                *reinterpret_cast<float4*>(&myscr[lane * NP + 4 * (3 * jj + kk)]) = cv;
-                            }
-                        __builtin_amdgcn_wave_barrier();
-                        wave_slice_reduce<4>(myscr, base, ii, sm);
-                        __builtin_amdgcn_wave_barrier();
-                    }
+                    lds_order();
                 }
+            };
+
+            // ---- rounds: lane = cell, round r = the r-th particle of every cell
+            const uint32_t maxcnt = s_max, tot = s_tot;
+            const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
+            for (uint32_t r = wave; r < maxcnt; r += NW) {
+                const bool act = r < mycnt;
+                const uint64_t am = __ballot(act);
+                const uint32_t oo = rs[r] + (uint32_t)__popcll(am & lt);
+                if (act) particle(lperm[mystart + r], oo, true);
             }
+            // ---- particles whose stencil base is outside the tile (clamped keys)
+            const uint32_t nout = lcnt[64], sout = lstart[64];
+            for (uint32_t t = tid; t < nout; t += TB) particle(lperm[sout + t], tot + t, false);
             __syncthreads();
             // ---- ranks in the new tiles: one global atomic per (tile, direction)
             if (G2P && P2G) {
@@ -335,16 +452,25 @@ __global__ void __launch_bounds__(TB) k_tile(TileArgs a) {
         }
 
         if (P2G) {
+            __syncthreads();
             for (int l = tid; l < WV; l += TB) {
-                const float m = sm[l];
-                if (m != 0.f) {
+                float4 m = pw[0][l];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) {
+                    const float4 u = pw[w][l];
+                    m.x += u.x;
+                    m.y += u.y;
+                    m.z += u.z;
+                    m.w += u.w;
+                }
+                if (m.x != 0.f) {
                     const int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
                     if (gi < g.nx && gj < g.ny && gk < g.nz) {
                         float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
-                        atomic_add_f32(cp + 0, m);
-                        atomic_add_f32(cp + 1, sm[WV + l]);
-                        atomic_add_f32(cp + 2, sm[2 * WV + l]);
-                        atomic_add_f32(cp + 3, sm[3 * WV + l]);
+                        atomic_add_f32(cp + 0, m.x);
+                        atomic_add_f32(cp + 1, m.y);
+                        atomic_add_f32(cp + 2, m.z);
+                        atomic_add_f32(cp + 3, m.w);
                     }
                 }
             }
