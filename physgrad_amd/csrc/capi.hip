@@ -205,8 +205,15 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.active, 4 * toff)) || (rc = dalloc(c, &w.nactive, 1)) ||
                    (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
                    (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
-                   (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))))))
+                   (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, toff)) ||
+                   (rc = dalloc(c, &w.nflag, toff)))))
         return rc;
+    if (tiled) {
+        // P2G window buffers (8^3 nodes, 8 KiB each) for up to min(tile slots,
+        // capacity, 2^16) active tiles; tiles beyond that add with atomics
+        w.wcap = (int)std::min<uint64_t>(std::min<uint64_t>(toff, c->cap), 1u << 16);
+        if ((rc = dalloc(c, &w.wbuf, (size_t)w.wcap * 512))) return rc;
+    }
     w.scenes = c->scenes_d;
     hipMemset(c->mp, 0, goff * sizeof(float4));
     hipMemset(c->gv, 0, goff * sizeof(float4));
@@ -214,6 +221,8 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
         hipMemset(w.count, 0, toff * 4);
         hipMemset(w.flags, 0, boff * 4);
         hipMemset(w.nactive, 0, 4);
+        hipMemset(w.nflag, 0, toff * 4);
+        hipMemset(w.tslot, 0xFF, toff * 4);
     }
     hipError_t e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);
     return e == hipSuccess ? PG_OK : hip_fail(c, e, "scene table upload");

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(TB) k_adj_g2p(AdjArgs a) {
     float* myscr = scr[wave];
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
-        const SceneInfo& sc = a.scenes[te.w];
+        const SceneInfo sc = a.scenes[te.w];   // by value: kept in registers across the stores
         const MlsGeom g = sc.g;
         const float4* __restrict__ gvs = a.gv + sc.goff;
         float4* __restrict__ agvs = a.agv + sc.goff;
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(TB) k_adj_p2g(AdjArgs a) {
     const int tid = threadIdx.x;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
-        const SceneInfo& sc = a.scenes[te.w];
+        const SceneInfo sc = a.scenes[te.w];   // by value: kept in registers across the stores
         const MlsGeom g = sc.g;
         const float4* __restrict__ apams = a.apam + sc.goff;
         const float s4 = 4.f * g.inv_dx * g.inv_dx;

@@ -9,30 +9,39 @@
 // starts and the list of non-empty tiles, and a scatter builds
 // perm = sorted position -> storage slot.
 //
-// Transfers.  One workgroup processes one tile at a time.  Inside the tile
-// the particles are ordered by stencil-base cell with an LDS counting sort
-// (64 cells), then processed 256 at a time:
+// Transfers.  One workgroup processes one tile at a time.  Per pass of up to
+// PASS particles, x and the storage slots are loaded in bulk and ordered by
+// stencil-base cell with an LDS counting sort (64 cells), then processed in
+// rounds: lane c of a wave takes the next particle of cell c.
 //   G2P  gathers the 27 node velocities from the tile's (TT+2)^3 window of
 //        grid velocity held in LDS;
-//   P2G  writes the particle's contributions to a per-wave scratch and
-//        reduces them across the wave (wave_slice_reduce): runs of particles
-//        sharing a cell become one LDS atomic per node and channel, and the
-//        window is flushed with one global atomic per node and channel.
+//   P2G  all particles of lane c's cell scatter to the same 27 nodes, so the
+//        lane accumulates their contributions in registers over the rounds
+//        and adds them to its wave's LDS window once per pass (plain
+//        read-modify-writes: the lanes of one instruction touch distinct
+//        nodes, and a wave's LDS instructions execute in order).  A particle
+//        that moved to another cell adds with LDS atomics.
 // G2P of step n and P2G of step n+1 are fused (the new state never leaves
-// registers) and the new state is written at the particle's cell-sorted
-// position, so storage stays sorted and the next step's gathers coalesce;
-// pid carries the user's particle index.  A particle whose stencil leaves the
-// window (it crossed a tile face this step) scatters with global atomics, so
-// correctness never depends on the binning.
+// registers); the new state is written round by round at consecutive
+// positions of the tile's output range, so stores and the next step's loads
+// coalesce; pid carries the user's particle index.  A particle whose stencil
+// leaves the P2G window (more than one cell per step) scatters with global
+// atomics, so correctness never depends on the binning.
 //
-// Grid update.  Runs over 8^3 node blocks flagged by the P2G flush; writes
-// (v, m) and clears the accumulator, so there is no dense grid clear.
+// Grid.  Each tile accumulates P2G in an (TT+4)^3-node window (one node
+// wider than the stencils of its particles on every side, so a particle that
+// moved one cell out of the tile still lands in it) and stores the window in
+// a buffer indexed by its place in the active list.  The grid update runs
+// over 4^3 node blocks: each node gathers the windows of the 8 tiles that
+// cover it (plus the global accumulator where the rare out-of-window P2G
+// landed), computes v = p/m + dt g with sticky walls, and clears what it
+// read.  There are no atomics on the regular path and no dense grid clear.
 #include "pg_tile.h"
 
 namespace pg {
 namespace {
 
-constexpr int PASS = 2048;       // particles sorted per LDS pass of a tile
+constexpr int PASS = 512;        // particles staged per LDS pass of a tile
 constexpr int NW = TB / 64;      // waves per workgroup
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 
@@ -54,17 +63,20 @@ struct TileArgs {
     uint32_t* src_out;       // output position -> input slot (segment tapes only)
     const float4* gv;        // concatenated grids of all scenes
     float4* mp;
-    uint32_t* flags;         // concatenated 8^3 block flags of all scenes
+    uint32_t* nflag;         // 4^3 node blocks hit by global-atomic P2G (by tile slot)
+    float4* wbuf;            // window sums by active-list index
+    int wcap;
     const SceneInfo* scenes;
     size_t cap;
 };
 
-__device__ __forceinline__ void flag_node(uint32_t* flags, const SceneInfo& sc, int gi, int gj, int gk) {
-    flags[((gk / GB) * sc.gby + gj / GB) * sc.gbx + gi / GB] = 1u;
-}
-
 __device__ __forceinline__ uint32_t morton_tile(int tx, int ty, int tz) {
     return expand3((uint32_t)tx) | (expand3((uint32_t)ty) << 1) | (expand3((uint32_t)tz) << 2);
+}
+
+// marks the 4^3 node block of node (gi, gj, gk) as holding global-atomic P2G
+__device__ __forceinline__ void flag_node(uint32_t* nflag, const SceneInfo& sc, int gi, int gj, int gk) {
+    nflag[sc.toff + morton_tile(gi / TT, gj / TT, gk / TT)] = 1u;
 }
 
 // tile coordinates of a position, clamped like tile_key
@@ -76,6 +88,11 @@ __device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, in
         int tc = b >= 0 ? b / TT : 0;
         t[a] = tc < nt[a] ? tc : nt[a] - 1;
     }
+}
+
+// number of set bits of a wave mask below this lane
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Keeps the compiler from moving LDS accesses across this point.  The window
@@ -185,58 +202,136 @@ __device__ __forceinline__ void p2g_visit(const Stencil& st, const P2GVal& p, Em
         }
 }
 
+// Rare paths, written as rolled loops so that they stay out of the register
+// budget of the rounds.
+// G2P from the global grid (particle whose stencil is not its lane's cell).
+__device__ __forceinline__ void g2p_global(const float4* __restrict__ gvs, const MlsGeom& g, const Stencil& st,
+                                           float* v, float* C) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) C[c] = 0.f;
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
+        const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+        if (!in_grid(g, gi, gj, gk)) continue;
+        const float4 u = gvs[node_id(g, gi, gj, gk)];
+        const float w = st.w[0][ix] * st.w[1][iy] * st.w[2][iz];
+        const float d[3] = {(float)ix - st.f[0], (float)iy - st.f[1], (float)iz - st.f[2]};
+        const float uu[3] = {u.x, u.y, u.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] += w * uu[c];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) C[3 * c + e] += w * uu[c] * d[e];
+        }
+    }
+    const float s = 4.f * g.inv_dx;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) C[c] *= s;
+}
+
+__device__ __forceinline__ float4 p2g_contrib(const Stencil& st, const P2GVal& p, int ix, int iy, int iz) {
+    const float w = st.w[0][ix] * st.w[1][iy] * st.w[2][iz];
+    return make_float4(w * p.m, w * (p.q[0] + p.a0[0] * (float)ix + p.a1[0] * (float)iy + p.a2[0] * (float)iz),
+                       w * (p.q[1] + p.a0[1] * (float)ix + p.a1[1] * (float)iy + p.a2[1] * (float)iz),
+                       w * (p.q[2] + p.a0[2] * (float)ix + p.a1[2] * (float)iy + p.a2[2] * (float)iz));
+}
+
+// P2G into the LDS P2G window with atomics (nb = window index of the stencil base).
+__device__ __forceinline__ void p2g_window_atomic(float* wf, int nb, const Stencil& st, const P2GVal& p) {
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
+        const float4 c = p2g_contrib(st, p, ix, iy, iz);
+        float* t = wf + 4 * (nb + (iz * PN + iy) * PN + ix);
+        atomicAdd(t + 0, c.x);
+        atomicAdd(t + 1, c.y);
+        atomicAdd(t + 2, c.z);
+        atomicAdd(t + 3, c.w);
+    }
+}
+
+// P2G into the global grid with atomics; the node blocks hit are flagged for
+// the grid update.
+__device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* __restrict__ nflag,
+                                           const SceneInfo& sc, const MlsGeom& g, const Stencil& st,
+                                           const P2GVal& p) {
+#pragma unroll 1
+    for (int n = 0; n < 27; ++n) {
+        const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
+        const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+        if (!in_grid(g, gi, gj, gk)) continue;
+        const float4 c = p2g_contrib(st, p, ix, iy, iz);
+        float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
+        atomic_add_f32(cp + 0, c.x);
+        atomic_add_f32(cp + 1, c.y);
+        atomic_add_f32(cp + 2, c.z);
+        atomic_add_f32(cp + 3, c.w);
+        flag_node(nflag, sc, gi, gj, gk);
+    }
+}
+
 // Tile kernel.  MODE bit 1: G2P (reads x, F; writes the new state);
 // MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state).
 //
-// Per pass of up to PASS particles of the tile: an LDS counting sort by
-// stencil-base cell (64 cells, plus "outside" for clamped keys), then rounds:
-// in round r lane c of a wave takes the r-th particle of cell c, waves take
-// rounds in turn.  All lanes of a round sit in distinct cells, so the 27 node
-// updates of a lane's P2G never meet another lane's in the same instruction,
-// and each wave accumulates into its own LDS window with plain
-// read-modify-writes in program order.  A particle that left its cell during
-// G2P adds with LDS atomics (inside the window) or global atomics (outside).
-// Outputs are written round by round, so stores and the next step's loads
-// coalesce.  At the end of the tile the four windows are summed and added to
-// the grid.
+// Per pass of up to PASS particles of the tile: the particles' x, F (and
+// slot, pid) are loaded in bulk and counted by stencil-base cell (64 cells,
+// plus "outside" for clamped keys), then staged in LDS in cell order.  Then
+// rounds: in round r lane c of a wave takes the r-th particle of cell c, waves
+// take rounds in turn; a round touches global memory only to store.  All
+// lanes of a round sit in distinct cells, so the 27 node updates of a lane's
+// P2G never meet another lane's in the same instruction, and each wave
+// accumulates into its own LDS window with plain read-modify-writes in
+// program order.  A particle that left its cell during G2P adds with LDS
+// atomics (inside the window) or global atomics (outside).  Outputs are
+// written round by round, so stores and the next step's loads coalesce.  At
+// the end of the tile the four windows are summed into the tile's buffer.
 template <int MODE>
 __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     constexpr bool G2P = (MODE & 1) != 0;
     constexpr bool P2G = (MODE & 2) != 0;
+    constexpr int PT = PASS / TB;                 // particles per thread in the bulk phases
     __shared__ float4 sv[G2P ? WV : 1];          // grid (v, m) window
-    __shared__ float4 pw[P2G ? NW : 1][WV];      // per-wave P2G windows (m, p)
-    __shared__ uint16_t lperm[PASS];             // local cell-sorted order within the pass
+    __shared__ float4 pw[P2G ? NW : 1][PV];      // per-wave P2G windows (m, p)
+    __shared__ float4 lq0[PASS];                 // (x, F00) of the pass in cell order
+    __shared__ uint32_t lslot[PASS];             // storage slot, cell order
+    __shared__ uint32_t lgen[PASS];              // (j | oo << 16) of round particles sent to the generic path
+    __shared__ uint16_t lqueue[(G2P && P2G) ? PASS : 1];  // output offsets of P2G that left the window
     __shared__ uint16_t rs[PASS];                // first output offset of each round
     __shared__ uint16_t ldir[(G2P && P2G) ? PASS : 1];  // (direction << 11 | rank) of the new tile, or FAR
     __shared__ uint32_t lcnt[65], lstart[65];
     __shared__ uint32_t dcnt[27], dbase[27];
-    __shared__ uint32_t s_max, s_tot;
+    __shared__ uint32_t s_max, s_tot, s_ngen, s_nq;
 
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const int ccx = lane & 3, ccy = (lane >> 2) & 3, ccz = lane >> 4;     // this lane's cell in a round
-    const int cnb = (ccz * WN + ccy) * WN + ccx;                          // its base node in the window
+    // this lane's cell in a round: cell index x + 4 z + 16 y
+    const int ccx = lane & 3, ccz = (lane >> 2) & 3, ccy = lane >> 4;
+    const int cnb = (ccz * WN + ccy) * WN + ccx;                          // its base node in the G2P window
+    const int cnp = ((ccz + 1) * PN + ccy + 1) * PN + ccx + 1;            // ... and in the P2G window
 
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
-        const SceneInfo& sc = a.scenes[te.w];
+        const SceneInfo sc = a.scenes[te.w];   // by value: kept in registers across the stores
         const MlsGeom g = sc.g;
         const float4* __restrict__ gvs = a.gv + sc.goff;
         float4* __restrict__ mps = a.mp + sc.goff;
-        uint32_t* __restrict__ flags = a.flags + sc.boff;
         const uint32_t tile = te.x - sc.toff, beg = te.y, cnt = te.z;
         int t0[3];
         tile_origin(tile, t0);
         const int tc[3] = {t0[0] / TT, t0[1] / TT, t0[2] / TT};
-        for (int l = tid; l < WV; l += TB) {
-            if (G2P) {
-                const int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
-                float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
-                sv[l] = u;
-            }
-            if (P2G) {
+        // the G2P window is loaded here and written to LDS after the first pass's
+        // bulk loads have been issued, so that the two latencies overlap
+        static_assert(WV <= TB, "one window node per thread");
+        float4 wvel = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (G2P && tid < WV) {
+            const int gi = t0[0] + tid % WN, gj = t0[1] + (tid / WN) % WN, gk = t0[2] + tid / (WN * WN);
+            if (gi < g.nx && gj < g.ny && gk < g.nz) wvel = gvs[node_id(g, gi, gj, gk)];
+        }
+        if (P2G) {
+            for (int l = tid; l < PV; l += TB) {
 #pragma unroll
                 for (int w = 0; w < NW; ++w) pw[w][l] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
@@ -244,18 +339,32 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 
         for (uint32_t pb = 0; pb < cnt; pb += PASS) {
             const uint32_t pc = min((uint32_t)PASS, cnt - pb);
-            // ---- counting sort of the pass by stencil-base cell
+            // ---- bulk load and count by stencil-base cell
             if (tid < 65) lcnt[tid] = 0;
             if (tid < 27) dcnt[tid] = 0;
+            if (tid == 0) {
+                s_ngen = 0;
+                s_nq = 0;
+            }
             __syncthreads();
-            uint32_t key[PASS / TB];
+            uint32_t key[PT], ks[PT];
+            float4 k0[PT];
 #pragma unroll
-            for (int k = 0; k < PASS / TB; ++k) {
+            for (int k = 0; k < PT; ++k) {
                 const uint32_t r = tid + k * TB;
                 key[k] = 0u;
                 if (r < pc) {
-                    const float4 q = a.Sin[a.perm[beg + pb + r]];
-                    const float xx[3] = {q.x, q.y, q.z};
+                    const uint32_t s = a.perm[beg + pb + r];
+                    ks[k] = s;
+                    k0[k] = a.Sin[s];
+                }
+            }
+            if (G2P && pb == 0 && tid < WV) sv[tid] = wvel;
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                const uint32_t r = tid + k * TB;
+                if (r < pc) {
+                    const float xx[3] = {k0[k].x, k0[k].y, k0[k].z};
                     int l3[3];
                     bool in = true;
 #pragma unroll
@@ -263,14 +372,14 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                         l3[d] = (int)floorf((xx[d] - g.o[d]) * g.inv_dx - 0.5f) - t0[d];
                         in = in && l3[d] >= 0 && l3[d] < TT;
                     }
-                    const int cell = in ? l3[0] + TT * l3[1] + TT * TT * l3[2] : 64;
+                    const int cell = in ? l3[0] + TT * l3[2] + TT * TT * l3[1] : 64;
                     const uint32_t rk = atomicAdd(&lcnt[cell], 1u);
                     key[k] = ((uint32_t)cell << 16) | rk;
                 }
             }
             __syncthreads();
             if (wave == 0) {
-                // cell starts (wave scan), the largest cell count, round offsets
+                // cell starts (wave scan) and the largest cell count
                 const uint32_t cl = lcnt[lane];
                 uint32_t inc = cl;
 #pragma unroll
@@ -279,63 +388,97 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     if (lane >= off) inc += t;
                 }
                 lstart[lane] = inc - cl;
-                if (lane == 63) lstart[64] = inc;
                 uint32_t mx = cl;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-                uint32_t run = 0;
-                for (uint32_t r = 0; r < mx; ++r) {
-                    const uint64_t m = __ballot(cl > r);
-                    if (lane == 0) rs[r] = (uint16_t)run;
-                    run += (uint32_t)__popcll(m);
-                }
-                if (lane == 0) {
+                if (lane == 63) {
+                    lstart[64] = inc;
+                    s_tot = inc;
                     s_max = mx;
-                    s_tot = run;
                 }
             }
             __syncthreads();
+            // ---- round offsets: round r starts after sum_c min(count_c, r) particles
+            for (uint32_t r = tid; r < s_max; r += TB) {
+                uint32_t o = 0;
+#pragma unroll 16
+                for (int c = 0; c < 64; ++c) o += min(lcnt[c], r);
+                rs[r] = (uint16_t)o;
+            }
+            // ---- stage in cell order
 #pragma unroll
-            for (int k = 0; k < PASS / TB; ++k) {
+            for (int k = 0; k < PT; ++k) {
                 const uint32_t r = tid + k * TB;
-                if (r < pc) lperm[lstart[key[k] >> 16] + (key[k] & 0xFFFFu)] = (uint16_t)r;
+                if (r < pc) {
+                    const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
+                    lq0[j] = k0[k];
+                    lslot[j] = ks[k];
+                }
             }
             __syncthreads();
 
-            // one particle: li = index within the pass (storage order), oo = output offset,
-            // regular = it is this lane's cell in a round (G2P from the window, P2G by RMW)
-            auto particle = [&](uint32_t li, uint32_t oo, bool regular) {
-                const uint32_t s = a.perm[beg + pb + li];
+            // binning of a moved particle: key, and its rank in the new tile through the
+            // per-direction LDS counters (FAR: a global atomic on the tile count)
+            auto bin_out = [&](const float* xn, uint32_t o, uint32_t oo, bool aggregate) {
+                int nt[3];
+                tile_coords(xn, g, sc.ntx, sc.nty, sc.ntz, nt);
+                const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
+                const uint32_t k = sc.toff + morton_tile(nt[0], nt[1], nt[2]);
+                a.key_out[o] = k;
+                const bool near = dx_ >= -1 && dx_ <= 1 && dy_ >= -1 && dy_ <= 1 && dz_ >= -1 && dz_ <= 1;
+                const int dir = near ? (dx_ + 1) + 3 * (dy_ + 1) + 9 * (dz_ + 1) : -1;
+                uint32_t rk = 0;
+                bool done = false;
+                if (aggregate) {
+                    // most particles stay in the tile: one LDS atomic per wave for those
+                    const uint64_t stay = __ballot(dir == 13);
+                    if (stay) {
+                        const int leader = __ffsll((unsigned long long)stay) - 1;
+                        uint32_t b0 = 0;
+                        if (lane == leader) b0 = atomicAdd(&dcnt[13], (uint32_t)__popcll(stay));
+                        b0 = __builtin_amdgcn_readlane(b0, leader);
+                        rk = b0 + lanes_below(stay);
+                        done = dir == 13;
+                    }
+                }
+                if (dir >= 0 && !done) rk = atomicAdd(&dcnt[dir], 1u);
+                if (dir >= 0) {
+                    ldir[oo] = (uint16_t)((dir << 11) | rk);
+                } else {
+                    ldir[oo] = FAR;
+                    a.rank_out[o] = atomicAdd(a.count + k, 1u);
+                }
+            };
+
+            // One particle of a round: j = its cell-order index, oo = its output offset.
+            // Returns false if its stencil base is not this lane's cell (the sort and the
+            // stencil could round differently); it then takes the generic path.  A
+            // particle whose P2G leaves the window is queued for the global scatter.
+            auto round_particle = [&](uint32_t j, uint32_t oo) -> bool {
+                float x[3], F[9], v[3], C[9];
+                const float4 q0 = lq0[j];
+                x[0] = q0.x; x[1] = q0.y; x[2] = q0.z; F[0] = q0.w;
+                Stencil st;
+                make_stencil(x, g.o, g.inv_dx, st);
+                if (st.b[0] - t0[0] != ccx || st.b[1] - t0[1] != ccy || st.b[2] - t0[2] != ccz) return false;
+                const uint32_t s = lslot[j];
                 const uint32_t o = beg + pb + oo;
+                const float4 q1 = a.Sin[a.cap + s], q2 = a.Sin[2 * a.cap + s];
                 float Eq = g.E, mq = g.p_mass;
                 if (a.E_in) Eq = a.E_in[s];
                 if (a.m_in) mq = a.m_in[s];
-                float x[3], F[9], v[3], C[9];
                 if (G2P) {
-                    load_xF(a.Sin, a.cap, s, x, F);
-                    Stencil st;
-                    make_stencil(x, g.o, g.inv_dx, st);
-                    // (the base equals the lane's cell by construction of the sort; checked so
-                    // that a different rounding of the two computations cannot matter)
-                    if (regular && st.b[0] - t0[0] == ccx && st.b[1] - t0[1] == ccy && st.b[2] - t0[2] == ccz) {
-                        g2p_sum([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; }, st,
-                                g.inv_dx, v, C);
-                    } else {
-                        g2p_sum(
-                            [&](int ix, int iy, int iz) {
-                                const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
-                                return in_grid(g, gi, gj, gk) ? gvs[node_id(g, gi, gj, gk)]
-                                                              : make_float4(0.f, 0.f, 0.f, 0.f);
-                            },
-                            st, g.inv_dx, v, C);
-                    }
+                    g2p_sum([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; }, st,
+                            g.inv_dx, v, C);
+                    F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
+                    F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
                     float Fn[9];
 #pragma unroll
                     for (int i = 0; i < 3; ++i)
 #pragma unroll
-                        for (int j = 0; j < 3; ++j)
-                            Fn[3 * i + j] = F[3 * i + j] + g.dt * (C[3 * i] * F[j] + C[3 * i + 1] * F[3 + j] +
-                                                                   C[3 * i + 2] * F[6 + j]);
+                        for (int jj = 0; jj < 3; ++jj)
+                            Fn[3 * i + jj] = F[3 * i + jj] + g.dt * (C[3 * i] * F[jj] + C[3 * i + 1] * F[3 + jj] +
+                                                                     C[3 * i + 2] * F[6 + jj]);
 #pragma unroll
                     for (int c = 0; c < 9; ++c) F[c] = Fn[c];
 #pragma unroll
@@ -345,78 +488,54 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     if (a.E_out && a.E_in) a.E_out[o] = Eq;
                     if (a.m_out && a.m_in) a.m_out[o] = mq;
                     if (a.src_out) a.src_out[o] = s;
-                    if (P2G) {
-                        int nt[3];
-                        tile_coords(x, g, sc.ntx, sc.nty, sc.ntz, nt);
-                        const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
-                        const uint32_t k = sc.toff + morton_tile(nt[0], nt[1], nt[2]);
-                        a.key_out[o] = k;
-                        const bool near = dx_ >= -1 && dx_ <= 1 && dy_ >= -1 && dy_ <= 1 && dz_ >= -1 && dz_ <= 1;
-                        const int dir = near ? (dx_ + 1) + 3 * (dy_ + 1) + 9 * (dz_ + 1) : -1;
-                        // most particles stay in the tile: one LDS atomic per wave for those
-                        const uint64_t stay = __ballot(dir == 13);
-                        uint32_t rk = 0;
-                        if (stay) {
-                            const int leader = __ffsll((unsigned long long)stay) - 1;
-                            uint32_t b0 = 0;
-                            if (lane == leader) b0 = atomicAdd(&dcnt[13], (uint32_t)__popcll(stay));
-                            b0 = __builtin_amdgcn_readlane(b0, leader);
-                            rk = b0 + (uint32_t)__popcll(stay & lt);
-                        }
-                        if (dir >= 0 && dir != 13) rk = atomicAdd(&dcnt[dir], 1u);
-                        if (dir >= 0) {
-                            ldir[oo] = (uint16_t)((dir << 11) | rk);
-                        } else {
-                            ldir[oo] = FAR;
-                            a.rank_out[o] = atomicAdd(a.count + k, 1u);
-                        }
-                    }
+                    if (P2G) bin_out(x, o, oo, true);
                 } else {
-                    load_state(a.Sin, a.cap, s, x, F, v, C);
+                    F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
+                    F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
+                    const float4 q3 = a.Sin[3 * a.cap + s], q4 = a.Sin[4 * a.cap + s], q5 = a.Sin[5 * a.cap + s];
+                    v[0] = q3.x; v[1] = q3.y; v[2] = q3.z; C[0] = q3.w;
+                    C[1] = q4.x; C[2] = q4.y; C[3] = q4.z; C[4] = q4.w;
+                    C[5] = q5.x; C[6] = q5.y; C[7] = q5.z; C[8] = q5.w;
                 }
                 if (P2G) {
-                    Stencil st;
-                    make_stencil(x, g.o, g.inv_dx, st);
-                    P2GVal p;
-                    p2g_prep(g, Eq, mq, F, C, v, st, p);
-                    const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-                    const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
-                    const int nb = (l2 * WN + l1) * WN + l0;
-                    float4* __restrict__ win = pw[wave];
-                    if (regular && inwin && nb == cnb) {
-                        p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                            float4* t = win + nb + (iz * WN + iy) * WN + ix;
-                            float4 u = *t;
-                            u.x += c.x;
-                            u.y += c.y;
-                            u.z += c.z;
-                            u.w += c.w;
-                            *t = u;
-                            lds_order();
-                        });
-                    } else if (inwin) {
-                        float* wf = reinterpret_cast<float*>(win);
-                        p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                            float* t = wf + 4 * (nb + (iz * WN + iy) * WN + ix);
-                            atomicAdd(t + 0, c.x);
-                            atomicAdd(t + 1, c.y);
-                            atomicAdd(t + 2, c.z);
-                            atomicAdd(t + 3, c.w);
-                        });
+                    if (G2P) make_stencil(x, g.o, g.inv_dx, st);
+                    // base relative to the P2G window origin t0 - 1
+                    const int l0 = st.b[0] - t0[0] + 1, l1 = st.b[1] - t0[1] + 1, l2 = st.b[2] - t0[2] + 1;
+                    const bool inwin = l0 >= 0 && l0 < PN - 2 && l1 >= 0 && l1 < PN - 2 && l2 >= 0 && l2 < PN - 2;
+                    const int nb = (l2 * PN + l1) * PN + l0;
+                    if (!inwin) {
+                        lqueue[atomicAdd(&s_nq, 1u)] = (uint16_t)oo;   // scattered after the rounds
                     } else {
-                        p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
-                            if (!in_grid(g, gi, gj, gk)) return;
-                            float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
-                            atomic_add_f32(cp + 0, c.x);
-                            atomic_add_f32(cp + 1, c.y);
-                            atomic_add_f32(cp + 2, c.z);
-                            atomic_add_f32(cp + 3, c.w);
-                            flag_node(flags, sc, gi, gj, gk);
-                        });
+                        P2GVal p;
+                        p2g_prep(g, Eq, mq, F, C, v, st, p);
+                        if (nb == cnp) {
+                            // the lanes of a round sit in distinct cells: plain read-modify-writes
+                            // of the wave's window, kept in program order
+                            float4* __restrict__ win = pw[wave];
+                            p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                                float4* t = win + nb + (iz * PN + iy) * PN + ix;
+                                float4 u = *t;
+                                u.x += c.x;
+                                u.y += c.y;
+                                u.z += c.z;
+                                u.w += c.w;
+                                *t = u;
+                                lds_order();
+                            });
+                        } else {
+                            // moved to another cell of the tile: may share its base with another lane
+                            float* wf = reinterpret_cast<float*>(pw[wave]);
+                            p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                                float* t = wf + 4 * (nb + (iz * PN + iy) * PN + ix);
+                                atomicAdd(t + 0, c.x);
+                                atomicAdd(t + 1, c.y);
+                                atomicAdd(t + 2, c.z);
+                                atomicAdd(t + 3, c.w);
+                            });
+                        }
                     }
-                    lds_order();
                 }
+                return true;
             };
 
             // ---- rounds: lane = cell, round r = the r-th particle of every cell
@@ -425,12 +544,87 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             for (uint32_t r = wave; r < maxcnt; r += NW) {
                 const bool act = r < mycnt;
                 const uint64_t am = __ballot(act);
-                const uint32_t oo = rs[r] + (uint32_t)__popcll(am & lt);
-                if (act) particle(lperm[mystart + r], oo, true);
+                const uint32_t oo = rs[r] + lanes_below(am);
+                if (act && !round_particle(mystart + r, oo))
+                    lgen[atomicAdd(&s_ngen, 1u)] = (mystart + r) | (oo << 16);
             }
-            // ---- particles whose stencil base is outside the tile (clamped keys)
+
+            // ---- generic path: particles whose stencil base is outside the tile (clamped
+            // keys) and the listed ones; global G2P, LDS atomics or global P2G
+            __syncthreads();
             const uint32_t nout = lcnt[64], sout = lstart[64];
-            for (uint32_t t = tid; t < nout; t += TB) particle(lperm[sout + t], tot + t, false);
+            const uint32_t ngen = nout + s_ngen;
+            for (uint32_t t = tid; t < ngen; t += TB) {
+                uint32_t j, oo;
+                if (t < nout) {
+                    j = sout + t;
+                    oo = tot + t;
+                } else {
+                    const uint32_t e = lgen[t - nout];
+                    j = e & 0xFFFFu;
+                    oo = e >> 16;
+                }
+                const uint32_t s = lslot[j];
+                const uint32_t o = beg + pb + oo;
+                float Eq = g.E, mq = g.p_mass;
+                if (a.E_in) Eq = a.E_in[s];
+                if (a.m_in) mq = a.m_in[s];
+                float x[3], F[9], v[3], C[9];
+                load_state(a.Sin, a.cap, s, x, F, v, C);
+                Stencil st;
+                make_stencil(x, g.o, g.inv_dx, st);
+                if (G2P) {
+                    g2p_global(gvs, g, st, v, C);
+                    float Fn[9];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int jj = 0; jj < 3; ++jj)
+                            Fn[3 * i + jj] = F[3 * i + jj] + g.dt * (C[3 * i] * F[jj] + C[3 * i + 1] * F[3 + jj] +
+                                                                     C[3 * i + 2] * F[6 + jj]);
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) F[c] = Fn[c];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
+                    store_state(a.Sout, a.cap, o, x, F, v, C);
+                    a.pid_out[o] = a.pid_in[s];
+                    if (a.E_out && a.E_in) a.E_out[o] = Eq;
+                    if (a.m_out && a.m_in) a.m_out[o] = mq;
+                    if (a.src_out) a.src_out[o] = s;
+                    if (P2G) bin_out(x, o, oo, false);
+                    make_stencil(x, g.o, g.inv_dx, st);
+                }
+                if (P2G) {
+                    P2GVal p;
+                    p2g_prep(g, Eq, mq, F, C, v, st, p);
+                    const int l0 = st.b[0] - t0[0] + 1, l1 = st.b[1] - t0[1] + 1, l2 = st.b[2] - t0[2] + 1;
+                    const bool inwin = l0 >= 0 && l0 < PN - 2 && l1 >= 0 && l1 < PN - 2 && l2 >= 0 && l2 < PN - 2;
+                    if (inwin) {
+                        float* wf = reinterpret_cast<float*>(pw[wave]);
+                        p2g_window_atomic(wf, (l2 * PN + l1) * PN + l0, st, p);
+                    } else {
+                        p2g_global(mps, a.nflag, sc, g, st, p);
+                    }
+                }
+            }
+            // ---- queued P2G of particles that left the window: global atomics, from the
+            // state just written
+            if (P2G && G2P) {
+                const uint32_t nq = s_nq;
+                for (uint32_t t = tid; t < nq; t += TB) {
+                    const uint32_t o = beg + pb + lqueue[t];
+                    float x[3], F[9], v[3], C[9];
+                    load_state(a.Sout, a.cap, o, x, F, v, C);
+                    float Eq = g.E, mq = g.p_mass;
+                    if (a.E_out && a.E_in) Eq = a.E_out[o];
+                    if (a.m_out && a.m_in) mq = a.m_out[o];
+                    Stencil st;
+                    make_stencil(x, g.o, g.inv_dx, st);
+                    P2GVal p;
+                    p2g_prep(g, Eq, mq, F, C, v, st, p);
+                    p2g_global(mps, a.nflag, sc, g, st, p);
+                }
+            }
             __syncthreads();
             // ---- ranks in the new tiles: one global atomic per (tile, direction)
             if (G2P && P2G) {
@@ -453,7 +647,10 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 
         if (P2G) {
             __syncthreads();
-            for (int l = tid; l < WV; l += TB) {
+            // the tile's window sum goes to its buffer (atomics once the buffers are used up)
+            const bool buffered = (int)ti < a.wcap;
+            float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
+            for (int l = tid; l < PV; l += TB) {
                 float4 m = pw[0][l];
 #pragma unroll
                 for (int w = 1; w < NW; ++w) {
@@ -463,48 +660,91 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     m.z += u.z;
                     m.w += u.w;
                 }
-                if (m.x != 0.f) {
-                    const int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
-                    if (gi < g.nx && gj < g.ny && gk < g.nz) {
+                if (buffered) {
+                    wb[l] = m;
+                } else if (m.x != 0.f) {
+                    const int gi = t0[0] - 1 + l % PN, gj = t0[1] - 1 + (l / PN) % PN, gk = t0[2] - 1 + l / (PN * PN);
+                    if (in_grid(g, gi, gj, gk)) {
                         float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
                         atomic_add_f32(cp + 0, m.x);
                         atomic_add_f32(cp + 1, m.y);
                         atomic_add_f32(cp + 2, m.z);
                         atomic_add_f32(cp + 3, m.w);
+                        flag_node(a.nflag, sc, gi, gj, gk);
                     }
                 }
-            }
-            if (tid < 8) {
-                const int bx = t0[0] / GB + (tid & 1), by = t0[1] / GB + ((tid >> 1) & 1), bz = t0[2] / GB + (tid >> 2);
-                // the window spans nodes [t0, t0 + WN); flag each 8^3 block it touches
-                const bool touch = bx * GB < t0[0] + WN && by * GB < t0[1] + WN && bz * GB < t0[2] + WN;
-                if (touch && bx < sc.gbx && by < sc.gby && bz < sc.gbz) flags[(bz * sc.gby + by) * sc.gbx + bx] = 1u;
             }
         }
         __syncthreads();
     }
 }
 
-// Grid update over flagged 8^3 blocks: gv = (p/m + dt g, m) with sticky walls.
-// Forward: the accumulator and the flag are cleared for the next P2G.  TAPE
-// (segment recompute for the adjoint): both are kept for the reverse pass,
-// which clears them after use.
+// Grid update over 4^3 node blocks (one wave per block, lane = node).  The
+// P2G window of tile T covers nodes [4T - 1, 4T + 7) per axis, so node block
+// B is covered by tiles B + {-1, 0, 1}^3, each node by 8 of them; the node's
+// (m, p) is the sum of those tiles' buffered windows plus the global
+// accumulator where the rare global-atomic P2G landed.  Writes
+// gv = (p/m + dt g, m) with sticky walls and clears the accumulator.  TAPE
+// (segment recompute for the adjoint): also stores the summed (m, p) in the
+// step's grid and flags its 8^3 block, both of which the reverse pass uses
+// and clears.
 template <bool TAPE>
-__global__ void __launch_bounds__(512) k_grid_tiled(const SceneInfo* __restrict__ scenes, int nscenes,
+__global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes, int nscenes, int slots,
+                                                    const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
+                                                    const float4* __restrict__ wbuf, int wcap,
                                                     float4* __restrict__ mp_all, float4* __restrict__ gv_all,
-                                                    uint32_t* __restrict__ flags) {
-    const int b = blockIdx.x;
-    if (flags[b] == 0u) return;
-    const SceneInfo& sc = scenes[scene_of(scenes, nscenes, (uint32_t)b, [](const SceneInfo& s) { return s.boff; })];
+                                                    uint32_t* __restrict__ flags8) {
+    const int B = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (B >= slots) return;
+    const int lane = threadIdx.x & 63;
+    const SceneInfo sc = scenes[scene_of(scenes, nscenes, (uint32_t)B, [](const SceneInfo& t) { return t.toff; })];
+    const uint32_t lb = (uint32_t)B - sc.toff;
+    const int bc[3] = {(int)compact3(lb), (int)compact3(lb >> 1), (int)compact3(lb >> 2)};
+    const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
+    if (bc[0] >= nt[0] || bc[1] >= nt[1] || bc[2] >= nt[2]) return;
+    auto slot_of = [&](int tx, int ty, int tz) -> uint32_t {
+        if (tx < 0 || ty < 0 || tz < 0 || tx >= nt[0] || ty >= nt[1] || tz >= nt[2]) return 0xFFFFFFFFu;
+        return tslot[sc.toff + morton_tile(tx, ty, tz)];
+    };
+    // any covering tile active, or global-atomic P2G here?
+    const bool fb = nflag[B] != 0u;
+    bool mine = false;
+    if (lane < 27) mine = slot_of(bc[0] + lane % 3 - 1, bc[1] + (lane / 3) % 3 - 1, bc[2] + lane / 9 - 1) != 0xFFFFFFFFu;
+    if (!fb && __ballot(mine) == 0ull) return;
     const MlsGeom g = sc.g;
+    const int lc[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
+    const int i = bc[0] * TT + lc[0], j = bc[1] * TT + lc[1], k = bc[2] * TT + lc[2];
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        int dT[3], w[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int bit = (e >> ax) & 1;
+            dT[ax] = lc[ax] < 3 ? bit - 1 : bit;
+            w[ax] = lc[ax] + 1 - TT * dT[ax];
+        }
+        const uint32_t sl = slot_of(bc[0] + dT[0], bc[1] + dT[1], bc[2] + dT[2]);
+        if (sl < (uint32_t)wcap) {
+            const float4 u = wbuf[(size_t)sl * PV + (w[2] * PN + w[1]) * PN + w[0]];
+            q.x += u.x;
+            q.y += u.y;
+            q.z += u.z;
+            q.w += u.w;
+        }
+    }
     float4* __restrict__ mp = mp_all + sc.goff;
     float4* __restrict__ gv = gv_all + sc.goff;
-    const int lb = b - (int)sc.boff;
-    const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
-    const int i = bx * GB + (threadIdx.x & 7), j = by * GB + ((threadIdx.x >> 3) & 7), k = bz * GB + (threadIdx.x >> 6);
     if (i < g.nx && j < g.ny && k < g.nz) {
         const size_t id = node_id(g, i, j, k);
-        const float4 q = mp[id];
+        if (fb) {
+            const float4 u = mp[id];
+            q.x += u.x;
+            q.y += u.y;
+            q.z += u.z;
+            q.w += u.w;
+            if (!TAPE) mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         float4 r = make_float4(0.f, 0.f, 0.f, q.x);
         if (q.x > 0.f && !is_sticky(g, i, j, k)) {
             const float im = 1.f / q.x;
@@ -513,11 +753,12 @@ __global__ void __launch_bounds__(512) k_grid_tiled(const SceneInfo* __restrict_
             r.z = q.w * im + g.dt * g.g[2];
         }
         gv[id] = r;
-        if (!TAPE) mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (TAPE) mp[id] = q;
     }
-    if (!TAPE) {
-        __syncthreads();
-        if (threadIdx.x == 0) flags[b] = 0u;
+    if (lane == 0) {
+        if (fb) nflag[B] = 0u;
+        if (TAPE)
+            flags8[sc.boff + ((bc[2] * TT / GB) * sc.gby + bc[1] * TT / GB) * sc.gbx + bc[0] * TT / GB] = 1u;
     }
 }
 
@@ -601,7 +842,8 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ 
 __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
                                                    uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
-                                                   const SceneInfo* __restrict__ scenes, int nscenes) {
+                                                   uint32_t* __restrict__ tslot, const SceneInfo* __restrict__ scenes,
+                                                   int nscenes) {
     __shared__ uint32_t sh[SCAN_T];
     // totals of the preceding blocks
     uint32_t ps = 0, pa = 0;
@@ -627,6 +869,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
     for (int k = 0; k < 4; ++k) {
         if (base + k < slots) {
             start[base + k] = run;
+            tslot[base + k] = v[k] ? pos : 0xFFFFFFFFu;
             if (v[k]) {
                 const int sc = scene_of(scenes, nscenes, (uint32_t)(base + k), [](const SceneInfo& t) { return t.toff; });
                 alist[pos++] = make_uint4((uint32_t)(base + k), run, v[k], (uint32_t)sc);
@@ -650,7 +893,7 @@ hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, w.scenes, w.nscenes);
+                                    nactive, w.tslot, w.scenes, w.nscenes);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
 }
@@ -662,8 +905,18 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.rank_out = w.rank;
     a.count = w.count;
     a.src_out = nullptr;
+    a.nflag = w.nflag;
+    a.wbuf = w.wbuf;
+    a.wcap = w.wcap;
     a.scenes = w.scenes;
     a.cap = cap;
+}
+
+template <bool TAPE>
+void grid_update(const TiledWork& w, float4* mp, float4* gv, uint32_t* flags8, hipStream_t st) {
+    ProfScope ps(w.prof, PK_GRID, st);
+    k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, w.tslot, w.nflag,
+                                                                w.wbuf, w.wcap, mp, gv, flags8);
 }
 
 }  // namespace
@@ -678,7 +931,6 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
                        uint32_t*& pid2, float*& E, float*& E2, float*& mass, float*& mass2, float4* mp, float4* gv,
                        int nsteps, hipStream_t st) {
     if (nsteps <= 0) return hipSuccess;
-    const int nblk = w.total_blocks;
     TileArgs a;
     fill_args(a, w, cap);
     a.perm = w.perm;
@@ -686,7 +938,6 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
     a.nactive = w.nactive;
     a.gv = gv;
     a.mp = mp;
-    a.flags = w.flags;
     auto set_io = [&]() {
         a.Sin = S; a.Sout = S2;
         a.pid_in = pid; a.pid_out = pid2;
@@ -700,20 +951,16 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
         ProfScope ps(w.prof, PK_TILE_P2G, st);
         k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
     }
-    {
-        ProfScope ps(w.prof, PK_GRID, st);
-        k_grid_tiled<false><<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, mp, gv, w.flags);
-    }
+    // the grid update reads the window buffers by this binning's active list,
+    // so it runs before the next binning
+    grid_update<false>(w, mp, gv, nullptr, st);
     for (int s = 0; s + 1 < nsteps; ++s) {
         set_io();
         {
             ProfScope ps(w.prof, PK_TILE_G2P2G, st);
             k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
         }
-        {
-            ProfScope ps(w.prof, PK_GRID, st);
-            k_grid_tiled<false><<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, mp, gv, w.flags);
-        }
+        grid_update<false>(w, mp, gv, nullptr, st);
         e = bin(w, n, st);
         if (e != hipSuccess) return e;
         std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
@@ -729,7 +976,6 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
 
 hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, int L, hipStream_t st) {
     if (L <= 0) return hipSuccess;
-    const int nblk = w.total_blocks;
     TileArgs a;
     fill_args(a, w, cap);
     if (n > 0)
@@ -737,6 +983,20 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
                                                      w.count);
     hipError_t e = bin_into(w, n, tape[0].perm, tape[0].alist, tape[0].nactive, st);
     if (e != hipSuccess) return e;
+    // P2G of step 0 and its grid
+    a.Sin = tape[0].S;
+    a.pid_in = tape[0].pid;
+    a.E_in = tape[0].E;
+    a.m_in = tape[0].mass;
+    a.perm = tape[0].perm;
+    a.alist = reinterpret_cast<const uint4*>(tape[0].alist);
+    a.nactive = tape[0].nactive;
+    a.mp = tape[0].mp;
+    {
+        ProfScope ps(w.prof, PK_TILE_P2G, st);
+        k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
+    }
+    grid_update<true>(w, tape[0].mp, tape[0].gv, tape[0].flags, st);
     for (int s = 0; s < L; ++s) {
         TapeStep& cur = tape[s];
         TapeStep& nxt = tape[s + 1];
@@ -748,28 +1008,20 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
         a.alist = reinterpret_cast<const uint4*>(cur.alist);
         a.nactive = cur.nactive;
         a.gv = cur.gv;
-        if (s == 0) {
-            a.mp = cur.mp;
-            a.flags = cur.flags;
-            ProfScope ps(w.prof, PK_TILE_P2G, st);
-            k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
-        }
-        {
-            ProfScope ps(w.prof, PK_GRID, st);
-            k_grid_tiled<true><<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, cur.mp, cur.gv, cur.flags);
-        }
         a.Sout = nxt.S;
         a.pid_out = nxt.pid;
         a.E_out = nxt.E;
         a.m_out = nxt.mass;
         a.src_out = cur.src;
         if (s + 1 < L) {
+            // G2P of step s, P2G of step s+1 (window buffers by this binning), its grid,
+            // then the binning of step s+1
             a.mp = nxt.mp;
-            a.flags = nxt.flags;
             {
                 ProfScope ps(w.prof, PK_TILE_G2P2G, st);
                 k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
             }
+            grid_update<true>(w, nxt.mp, nxt.gv, nxt.flags, st);
             e = bin_into(w, n, nxt.perm, nxt.alist, nxt.nactive, st);
             if (e != hipSuccess) return e;
         } else {

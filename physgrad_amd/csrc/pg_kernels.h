@@ -126,8 +126,12 @@ struct TiledWork {
     uint32_t* key;         // [cap] tile id per storage slot
     uint32_t* rank;        // [cap] rank within tile
     uint32_t* perm;        // [cap] sorted position -> storage slot
-    uint32_t* flags;       // [total_blocks] touched grid blocks
+    uint32_t* flags;       // [total_blocks] 8^3 block flags (unused by the forward step)
     uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
+    uint32_t* tslot;       // [tile_slots] index of the tile in the active list, or ~0
+    uint32_t* nflag;       // [tile_slots] 4^3 node blocks hit by out-of-window P2G (global atomics)
+    float4* wbuf;          // [wcap][216] P2G window sums of the active tiles
+    int wcap;              // active tiles with a window buffer; later ones add with atomics
     int nblocks_launch;    // persistent grid size of the tile kernels
 };
 
@@ -142,9 +146,9 @@ struct TapeStep {
     uint32_t* src;      // output position of this step -> storage slot of this step
     uint32_t* alist;    // uint4 (tile, start, count, scene) per active tile
     uint32_t* nactive;
-    float4* mp;         // P2G accumulation (m, p); cleared by the reverse pass
+    float4* mp;         // P2G result (m, p) of the step; cleared by the reverse pass
     float4* gv;         // (v, m) after the grid update
-    uint32_t* flags;    // touched blocks; cleared by the reverse pass
+    uint32_t* flags;    // 8^3 blocks holding the step's grid; cleared by the reverse pass
 };
 
 // Recompute L steps from tape[0].S (tape[0].pid/E/mass set), filling tape[0..L-1]

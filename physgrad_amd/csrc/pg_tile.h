@@ -13,6 +13,9 @@ namespace pg {
 constexpr int TT = 4;            // tile edge (cells)
 constexpr int WN = TT + 2;       // window edge (nodes)
 constexpr int WV = WN * WN * WN; // window volume (216)
+constexpr int PN = TT + 4;       // P2G window edge (nodes [t0 - 1, t0 + TT + 3)): also takes
+                                 // particles that moved one cell out of the tile
+constexpr int PV = PN * PN * PN; // P2G window volume (512)
 constexpr int TB = 256;          // threads per tile workgroup
 constexpr int GB = 8;            // grid-update block edge (nodes)
 

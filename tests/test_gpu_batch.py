@@ -7,11 +7,11 @@ separate single-scene contexts and eager against graph replay, relative 1e-4
 (FP32 trajectories whose atomics sum in different orders); the batch adjoint
 against the FP64 oracle's reverse steps per scene, relative 1e-3.
 """
-import os
-
 import numpy as np
 import pytest
 import torch
+
+from test_gpu_mls import check_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -89,8 +89,7 @@ def test_batch_step_parity_vs_fp64_oracle(gpu, oracle_lib):
                 for key in ("x", "v", "C", "F"):
                     err = rel(got[key][p0:p0 + n].reshape(-1), want[key].reshape(-1))
                     assert err <= TOL_STEP, (s, key, err)
-                assert rel(gm[g0:g0 + G], want["grid_m"]) <= TOL_STEP, s
-                assert rel(gv[g0:g0 + G].reshape(-1), want["grid_v"].reshape(-1)) <= TOL_STEP, s
+                check_grid(gm[g0:g0 + G], gv[g0:g0 + G], want["grid_m"], want["grid_v"])
             else:
                 assert not gm[g0:g0 + G].any()
             p0 += n

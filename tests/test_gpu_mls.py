@@ -15,6 +15,20 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
+def check_grid(gm, gv, wm, wv):
+    """Grid of one step against the FP64 oracle: mass and the updated momentum
+    m v at TOL_STEP; the velocity itself at 10 TOL_STEP (at nearly empty nodes
+    it is the ratio of two small sums, whose FP32 rounding depends on the
+    summation order)."""
+    gm = np.asarray(gm, np.float64).reshape(-1)
+    wm = np.asarray(wm, np.float64).reshape(-1)
+    gv = np.asarray(gv, np.float64).reshape(-1, 3)
+    wv = np.asarray(wv, np.float64).reshape(-1, 3)
+    assert rel(gm, wm) <= TOL_STEP
+    assert rel((gv * gm[:, None]).reshape(-1), (wv * wm[:, None]).reshape(-1)) <= TOL_STEP
+    assert rel(gv.reshape(-1), wv.reshape(-1)) <= 10 * TOL_STEP
+
+
 def small_scene(rng, n=512, material=0, per_particle_E=True):
     from physgrad_amd.scenes import Scene
     x = (0.3 + 0.4 * rng.random((n, 3))).astype(np.float32)
@@ -60,8 +74,7 @@ def test_step_parity_vs_fp64_oracle(gpu, oracle_lib, impl, material):
         for key in ("x", "v", "C", "F"):
             err = rel(got[key].reshape(-1), want[key].reshape(-1))
             assert err <= TOL_STEP, (key, k, err)
-        assert rel(got["grid_m"], want["grid_m"]) <= TOL_STEP
-        assert rel(got["grid_v"].reshape(-1), want["grid_v"].reshape(-1)) <= TOL_STEP
+        check_grid(got["grid_m"], got["grid_v"], want["grid_m"], want["grid_v"])
         x, v, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
 
 
