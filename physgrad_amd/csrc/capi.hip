@@ -206,7 +206,7 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
                    (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
                    (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, toff)) ||
-                   (rc = dalloc(c, &w.nflag, toff)))))
+                   (rc = dalloc(c, &w.nflag, toff)) || (rc = dalloc(c, &w.wflag, toff)))))
         return rc;
     if (tiled) {
         // P2G window buffers (8^3 nodes, 8 KiB each) for up to min(tile slots,
@@ -222,6 +222,7 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
         hipMemset(w.flags, 0, boff * 4);
         hipMemset(w.nactive, 0, 4);
         hipMemset(w.nflag, 0, toff * 4);
+        hipMemset(w.wflag, 0, toff * 4);
         hipMemset(w.tslot, 0xFF, toff * 4);
     }
     hipError_t e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);

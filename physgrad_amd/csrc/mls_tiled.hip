@@ -41,7 +41,7 @@
 namespace pg {
 namespace {
 
-constexpr int PASS = 512;        // particles staged per LDS pass of a tile
+constexpr int PASS = 1280;       // particles staged per LDS pass of a tile (> most full tiles)
 constexpr int NW = TB / 64;      // waves per workgroup
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 
@@ -64,6 +64,7 @@ struct TileArgs {
     const float4* gv;        // concatenated grids of all scenes
     float4* mp;
     uint32_t* nflag;         // 4^3 node blocks hit by global-atomic P2G (by tile slot)
+    uint32_t* wflag;         // 4^3 node blocks covered by a buffered window (by tile slot)
     float4* wbuf;            // window sums by active-list index
     int wcap;
     const SceneInfo* scenes;
@@ -84,7 +85,7 @@ __device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, in
     const int nt[3] = {ntx, nty, ntz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int b = (int)floorf((x[a] - g.o[a]) * g.inv_dx - 0.5f);
+        int b = stencil_base(grid_coord(x[a], g.o[a], g.inv_dx));
         int tc = b >= 0 ? b / TT : 0;
         t[a] = tc < nt[a] ? tc : nt[a] - 1;
     }
@@ -203,7 +204,10 @@ __device__ __forceinline__ void p2g_visit(const Stencil& st, const P2GVal& p, Em
 }
 
 // Rare paths, written as rolled loops so that they stay out of the register
-// budget of the rounds.
+// budget of the rounds.  Stencil values are picked with selects rather than by
+// indexing, which would put the stencil in scratch memory.
+__device__ __forceinline__ float pick3(const float* w, int i) { return i == 0 ? w[0] : (i == 1 ? w[1] : w[2]); }
+
 // G2P from the global grid (particle whose stencil is not its lane's cell).
 __device__ __forceinline__ void g2p_global(const float4* __restrict__ gvs, const MlsGeom& g, const Stencil& st,
                                            float* v, float* C) {
@@ -217,7 +221,7 @@ __device__ __forceinline__ void g2p_global(const float4* __restrict__ gvs, const
         const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
         if (!in_grid(g, gi, gj, gk)) continue;
         const float4 u = gvs[node_id(g, gi, gj, gk)];
-        const float w = st.w[0][ix] * st.w[1][iy] * st.w[2][iz];
+        const float w = pick3(st.w[0], ix) * pick3(st.w[1], iy) * pick3(st.w[2], iz);
         const float d[3] = {(float)ix - st.f[0], (float)iy - st.f[1], (float)iz - st.f[2]};
         const float uu[3] = {u.x, u.y, u.z};
 #pragma unroll
@@ -233,24 +237,21 @@ __device__ __forceinline__ void g2p_global(const float4* __restrict__ gvs, const
 }
 
 __device__ __forceinline__ float4 p2g_contrib(const Stencil& st, const P2GVal& p, int ix, int iy, int iz) {
-    const float w = st.w[0][ix] * st.w[1][iy] * st.w[2][iz];
+    const float w = pick3(st.w[0], ix) * pick3(st.w[1], iy) * pick3(st.w[2], iz);
     return make_float4(w * p.m, w * (p.q[0] + p.a0[0] * (float)ix + p.a1[0] * (float)iy + p.a2[0] * (float)iz),
                        w * (p.q[1] + p.a0[1] * (float)ix + p.a1[1] * (float)iy + p.a2[1] * (float)iz),
                        w * (p.q[2] + p.a0[2] * (float)ix + p.a1[2] * (float)iy + p.a2[2] * (float)iz));
 }
 
-// P2G into the LDS P2G window with atomics (nb = window index of the stencil base).
+// P2G into the wide LDS window with atomics (nb = window index of the stencil base).
 __device__ __forceinline__ void p2g_window_atomic(float* wf, int nb, const Stencil& st, const P2GVal& p) {
-#pragma unroll 1
-    for (int n = 0; n < 27; ++n) {
-        const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
-        const float4 c = p2g_contrib(st, p, ix, iy, iz);
+    p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
         float* t = wf + 4 * (nb + (iz * PN + iy) * PN + ix);
         atomicAdd(t + 0, c.x);
         atomicAdd(t + 1, c.y);
         atomicAdd(t + 2, c.z);
         atomicAdd(t + 3, c.w);
-    }
+    });
 }
 
 // P2G into the global grid with atomics; the node blocks hit are flagged for
@@ -276,41 +277,41 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* _
 // Tile kernel.  MODE bit 1: G2P (reads x, F; writes the new state);
 // MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state).
 //
-// Per pass of up to PASS particles of the tile: the particles' x, F (and
-// slot, pid) are loaded in bulk and counted by stencil-base cell (64 cells,
-// plus "outside" for clamped keys), then staged in LDS in cell order.  Then
+// Per pass of up to PASS particles of the tile: the particles' (x, F00) and
+// storage slots are loaded in bulk, counted by stencil-base cell (64 cells,
+// plus "outside" for clamped keys) and staged in LDS in cell order.  Then
 // rounds: in round r lane c of a wave takes the r-th particle of cell c, waves
-// take rounds in turn; a round touches global memory only to store.  All
-// lanes of a round sit in distinct cells, so the 27 node updates of a lane's
-// P2G never meet another lane's in the same instruction, and each wave
-// accumulates into its own LDS window with plain read-modify-writes in
-// program order.  A particle that left its cell during G2P adds with LDS
-// atomics (inside the window) or global atomics (outside).  Outputs are
+// take rounds in turn.  All lanes of a round sit in distinct cells, so the 27
+// node updates of a lane's P2G never meet another lane's in the same
+// instruction, and each wave accumulates into its own LDS window with plain
+// read-modify-writes in program order.  A particle that left its cell during
+// G2P adds into a shared, one node wider window with LDS atomics; one that
+// left even that (more than a cell per step) with global atomics.  Outputs are
 // written round by round, so stores and the next step's loads coalesce.  At
-// the end of the tile the four windows are summed into the tile's buffer.
+// the end of the tile the windows are summed into the tile's buffer.
 template <int MODE>
 __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     constexpr bool G2P = (MODE & 1) != 0;
     constexpr bool P2G = (MODE & 2) != 0;
     constexpr int PT = PASS / TB;                 // particles per thread in the bulk phases
-    __shared__ float4 sv[G2P ? WV : 1];          // grid (v, m) window
-    __shared__ float4 pw[P2G ? NW : 1][PV];      // per-wave P2G windows (m, p)
-    __shared__ float4 lq0[PASS];                 // (x, F00) of the pass in cell order
+    static_assert(PASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
+    __shared__ float4 sv[G2P ? WV : 1];          // G2P window: grid (v, m) on nodes [t0, t0 + WN)
+    __shared__ float4 pw[P2G ? NW : 1][WV];      // per-wave P2G windows (m, p) on nodes [t0, t0 + WN)
+    __shared__ float4 mv[P2G ? PV : 1];          // P2G of moved particles on nodes [t0 - 1, t0 + PN - 1)
+    __shared__ float lx[3][PASS];                // x of the pass in cell order
     __shared__ uint32_t lslot[PASS];             // storage slot, cell order
-    __shared__ uint32_t lgen[PASS];              // (j | oo << 16) of round particles sent to the generic path
-    __shared__ uint16_t lqueue[(G2P && P2G) ? PASS : 1];  // output offsets of P2G that left the window
+    __shared__ uint32_t lqueue[(G2P && P2G) ? PASS / 32 : 1];  // bit oo: P2G outside both windows
     __shared__ uint16_t rs[PASS];                // first output offset of each round
     __shared__ uint16_t ldir[(G2P && P2G) ? PASS : 1];  // (direction << 11 | rank) of the new tile, or FAR
     __shared__ uint32_t lcnt[65], lstart[65];
     __shared__ uint32_t dcnt[27], dbase[27];
-    __shared__ uint32_t s_max, s_tot, s_ngen, s_nq;
+    __shared__ uint32_t s_max, s_tot;
 
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // this lane's cell in a round: cell index x + 4 z + 16 y
     const int ccx = lane & 3, ccz = (lane >> 2) & 3, ccy = lane >> 4;
-    const int cnb = (ccz * WN + ccy) * WN + ccx;                          // its base node in the G2P window
-    const int cnp = ((ccz + 1) * PN + ccy + 1) * PN + ccx + 1;            // ... and in the P2G window
+    const int cnb = (ccz * WN + ccy) * WN + ccx;                          // its base node in the windows
 
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
@@ -331,10 +332,11 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             if (gi < g.nx && gj < g.ny && gk < g.nz) wvel = gvs[node_id(g, gi, gj, gk)];
         }
         if (P2G) {
-            for (int l = tid; l < PV; l += TB) {
+            for (int l = tid; l < WV; l += TB) {
 #pragma unroll
                 for (int w = 0; w < NW; ++w) pw[w][l] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
+            for (int l = tid; l < PV; l += TB) mv[l] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
 
         for (uint32_t pb = 0; pb < cnt; pb += PASS) {
@@ -342,10 +344,8 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             // ---- bulk load and count by stencil-base cell
             if (tid < 65) lcnt[tid] = 0;
             if (tid < 27) dcnt[tid] = 0;
-            if (tid == 0) {
-                s_ngen = 0;
-                s_nq = 0;
-            }
+            if (G2P && P2G)
+                for (int q = tid; q < PASS / 32; q += TB) lqueue[q] = 0u;
             __syncthreads();
             uint32_t key[PT], ks[PT];
             float4 k0[PT];
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     bool in = true;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        l3[d] = (int)floorf((xx[d] - g.o[d]) * g.inv_dx - 0.5f) - t0[d];
+                        l3[d] = stencil_base(grid_coord(xx[d], g.o[d], g.inv_dx)) - t0[d];
                         in = in && l3[d] >= 0 && l3[d] < TT;
                     }
                     const int cell = in ? l3[0] + TT * l3[2] + TT * TT * l3[1] : 64;
@@ -411,7 +411,9 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 const uint32_t r = tid + k * TB;
                 if (r < pc) {
                     const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
-                    lq0[j] = k0[k];
+                    lx[0][j] = k0[k].x;
+                    lx[1][j] = k0[k].y;
+                    lx[2][j] = k0[k].z;
                     lslot[j] = ks[k];
                 }
             }
@@ -450,26 +452,65 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 }
             };
 
-            // One particle of a round: j = its cell-order index, oo = its output offset.
-            // Returns false if its stencil base is not this lane's cell (the sort and the
-            // stencil could round differently); it then takes the generic path.  A
-            // particle whose P2G leaves the window is queued for the global scatter.
-            auto round_particle = [&](uint32_t j, uint32_t oo) -> bool {
-                float x[3], F[9], v[3], C[9];
-                const float4 q0 = lq0[j];
-                x[0] = q0.x; x[1] = q0.y; x[2] = q0.z; F[0] = q0.w;
+            // P2G of a particle with its (new) state: into the wave's window by
+            // read-modify-write when its base is this lane's cell (own), into the
+            // shared wide window with LDS atomics when it moved to another cell
+            // nearby, else queued for the global scatter (MODE 3) or added with
+            // global atomics.
+            auto scatter = [&](const float* x, const float* F, const float* C, const float* v, float Eq, float mq,
+                               bool own_ok, uint32_t oo) {
                 Stencil st;
                 make_stencil(x, g.o, g.inv_dx, st);
-                if (st.b[0] - t0[0] != ccx || st.b[1] - t0[1] != ccy || st.b[2] - t0[2] != ccz) return false;
+                const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
+                const bool own = own_ok && l0 == ccx && l1 == ccy && l2 == ccz;
+                const bool near = l0 >= -1 && l0 < TT + 1 && l1 >= -1 && l1 < TT + 1 && l2 >= -1 && l2 < TT + 1;
+                if (!near && G2P) {
+                    atomicOr(&lqueue[oo >> 5], 1u << (oo & 31));   // scattered after the rounds
+                    return;
+                }
+                P2GVal p;
+                p2g_prep(g, Eq, mq, F, C, v, st, p);
+                if (own) {
+                    float4* __restrict__ win = pw[wave];
+                    p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                        float4* t = win + cnb + (iz * WN + iy) * WN + ix;
+                        float4 u = *t;
+                        u.x += c.x;
+                        u.y += c.y;
+                        u.z += c.z;
+                        u.w += c.w;
+                        *t = u;
+                        lds_order();
+                    });
+                } else if (near) {
+                    p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st, p);
+                } else {
+                    p2g_global(mps, a.nflag, sc, g, st, p);
+                }
+                lds_order();
+            };
+
+            // one particle: j = its cell-order index, oo = its output offset, own_ok = it
+            // is this lane's cell in a round (G2P from the window)
+            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok) {
+                float x[3], F[9], v[3], C[9];
+                x[0] = lx[0][j]; x[1] = lx[1][j]; x[2] = lx[2][j];
                 const uint32_t s = lslot[j];
                 const uint32_t o = beg + pb + oo;
-                const float4 q1 = a.Sin[a.cap + s], q2 = a.Sin[2 * a.cap + s];
+                // F from the state planes (Q0 was read by the bulk phase and is in cache)
+                const float4 q0 = a.Sin[s], q1 = a.Sin[a.cap + s], q2 = a.Sin[2 * a.cap + s];
+                F[0] = q0.w;
                 float Eq = g.E, mq = g.p_mass;
                 if (a.E_in) Eq = a.E_in[s];
                 if (a.m_in) mq = a.m_in[s];
                 if (G2P) {
-                    g2p_sum([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; }, st,
-                            g.inv_dx, v, C);
+                    Stencil st;
+                    make_stencil(x, g.o, g.inv_dx, st);
+                    if (own_ok)
+                        g2p_sum([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; }, st,
+                                g.inv_dx, v, C);
+                    else
+                        g2p_global(gvs, g, st, v, C);
                     F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
                     F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
                     float Fn[9];
@@ -488,7 +529,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     if (a.E_out && a.E_in) a.E_out[o] = Eq;
                     if (a.m_out && a.m_in) a.m_out[o] = mq;
                     if (a.src_out) a.src_out[o] = s;
-                    if (P2G) bin_out(x, o, oo, true);
+                    if (P2G) bin_out(x, o, oo, own_ok);
                 } else {
                     F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
                     F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
@@ -497,122 +538,29 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     C[1] = q4.x; C[2] = q4.y; C[3] = q4.z; C[4] = q4.w;
                     C[5] = q5.x; C[6] = q5.y; C[7] = q5.z; C[8] = q5.w;
                 }
-                if (P2G) {
-                    if (G2P) make_stencil(x, g.o, g.inv_dx, st);
-                    // base relative to the P2G window origin t0 - 1
-                    const int l0 = st.b[0] - t0[0] + 1, l1 = st.b[1] - t0[1] + 1, l2 = st.b[2] - t0[2] + 1;
-                    const bool inwin = l0 >= 0 && l0 < PN - 2 && l1 >= 0 && l1 < PN - 2 && l2 >= 0 && l2 < PN - 2;
-                    const int nb = (l2 * PN + l1) * PN + l0;
-                    if (!inwin) {
-                        lqueue[atomicAdd(&s_nq, 1u)] = (uint16_t)oo;   // scattered after the rounds
-                    } else {
-                        P2GVal p;
-                        p2g_prep(g, Eq, mq, F, C, v, st, p);
-                        if (nb == cnp) {
-                            // the lanes of a round sit in distinct cells: plain read-modify-writes
-                            // of the wave's window, kept in program order
-                            float4* __restrict__ win = pw[wave];
-                            p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                                float4* t = win + nb + (iz * PN + iy) * PN + ix;
-                                float4 u = *t;
-                                u.x += c.x;
-                                u.y += c.y;
-                                u.z += c.z;
-                                u.w += c.w;
-                                *t = u;
-                                lds_order();
-                            });
-                        } else {
-                            // moved to another cell of the tile: may share its base with another lane
-                            float* wf = reinterpret_cast<float*>(pw[wave]);
-                            p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                                float* t = wf + 4 * (nb + (iz * PN + iy) * PN + ix);
-                                atomicAdd(t + 0, c.x);
-                                atomicAdd(t + 1, c.y);
-                                atomicAdd(t + 2, c.z);
-                                atomicAdd(t + 3, c.w);
-                            });
-                        }
-                    }
-                }
-                return true;
+                if (P2G) scatter(x, F, C, v, Eq, mq, own_ok, oo);
             };
 
-            // ---- rounds: lane = cell, round r = the r-th particle of every cell
+            // ---- rounds: lane = cell, round r = the r-th particle of every cell (the sort
+            // and the stencil compute the base identically, so it is the lane's cell)
             const uint32_t maxcnt = s_max, tot = s_tot;
             const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
             for (uint32_t r = wave; r < maxcnt; r += NW) {
                 const bool act = r < mycnt;
                 const uint64_t am = __ballot(act);
                 const uint32_t oo = rs[r] + lanes_below(am);
-                if (act && !round_particle(mystart + r, oo))
-                    lgen[atomicAdd(&s_ngen, 1u)] = (mystart + r) | (oo << 16);
+                if (act) particle(mystart + r, oo, true);
             }
-
-            // ---- generic path: particles whose stencil base is outside the tile (clamped
-            // keys) and the listed ones; global G2P, LDS atomics or global P2G
-            __syncthreads();
+            // ---- particles whose stencil base is outside the tile (clamped keys)
             const uint32_t nout = lcnt[64], sout = lstart[64];
-            const uint32_t ngen = nout + s_ngen;
-            for (uint32_t t = tid; t < ngen; t += TB) {
-                uint32_t j, oo;
-                if (t < nout) {
-                    j = sout + t;
-                    oo = tot + t;
-                } else {
-                    const uint32_t e = lgen[t - nout];
-                    j = e & 0xFFFFu;
-                    oo = e >> 16;
-                }
-                const uint32_t s = lslot[j];
-                const uint32_t o = beg + pb + oo;
-                float Eq = g.E, mq = g.p_mass;
-                if (a.E_in) Eq = a.E_in[s];
-                if (a.m_in) mq = a.m_in[s];
-                float x[3], F[9], v[3], C[9];
-                load_state(a.Sin, a.cap, s, x, F, v, C);
-                Stencil st;
-                make_stencil(x, g.o, g.inv_dx, st);
-                if (G2P) {
-                    g2p_global(gvs, g, st, v, C);
-                    float Fn[9];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int jj = 0; jj < 3; ++jj)
-                            Fn[3 * i + jj] = F[3 * i + jj] + g.dt * (C[3 * i] * F[jj] + C[3 * i + 1] * F[3 + jj] +
-                                                                     C[3 * i + 2] * F[6 + jj]);
-#pragma unroll
-                    for (int c = 0; c < 9; ++c) F[c] = Fn[c];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
-                    store_state(a.Sout, a.cap, o, x, F, v, C);
-                    a.pid_out[o] = a.pid_in[s];
-                    if (a.E_out && a.E_in) a.E_out[o] = Eq;
-                    if (a.m_out && a.m_in) a.m_out[o] = mq;
-                    if (a.src_out) a.src_out[o] = s;
-                    if (P2G) bin_out(x, o, oo, false);
-                    make_stencil(x, g.o, g.inv_dx, st);
-                }
-                if (P2G) {
-                    P2GVal p;
-                    p2g_prep(g, Eq, mq, F, C, v, st, p);
-                    const int l0 = st.b[0] - t0[0] + 1, l1 = st.b[1] - t0[1] + 1, l2 = st.b[2] - t0[2] + 1;
-                    const bool inwin = l0 >= 0 && l0 < PN - 2 && l1 >= 0 && l1 < PN - 2 && l2 >= 0 && l2 < PN - 2;
-                    if (inwin) {
-                        float* wf = reinterpret_cast<float*>(pw[wave]);
-                        p2g_window_atomic(wf, (l2 * PN + l1) * PN + l0, st, p);
-                    } else {
-                        p2g_global(mps, a.nflag, sc, g, st, p);
-                    }
-                }
-            }
-            // ---- queued P2G of particles that left the window: global atomics, from the
-            // state just written
+            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, tot + t, false);
+            __syncthreads();
+            // ---- queued P2G of particles that left both windows: global atomics,
+            // from the state just written
             if (P2G && G2P) {
-                const uint32_t nq = s_nq;
-                for (uint32_t t = tid; t < nq; t += TB) {
-                    const uint32_t o = beg + pb + lqueue[t];
+                for (uint32_t t = tid; t < pc; t += TB) {
+                    if (!(lqueue[t >> 5] & (1u << (t & 31)))) continue;
+                    const uint32_t o = beg + pb + t;
                     float x[3], F[9], v[3], C[9];
                     load_state(a.Sout, a.cap, o, x, F, v, C);
                     float Eq = g.E, mq = g.p_mass;
@@ -625,7 +573,6 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     p2g_global(mps, a.nflag, sc, g, st, p);
                 }
             }
-            __syncthreads();
             // ---- ranks in the new tiles: one global atomic per (tile, direction)
             if (G2P && P2G) {
                 if (tid < 27) {
@@ -641,29 +588,33 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     const uint16_t dr = ldir[jj];
                     if (dr != FAR) a.rank_out[beg + pb + jj] = dbase[dr >> 11] + (dr & 0x7FFu);
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
 
         if (P2G) {
-            __syncthreads();
-            // the tile's window sum goes to its buffer (atomics once the buffers are used up)
+            // the tile's window sum goes to its buffer (atomics once the buffers are used up);
+            // node l of the wide window is node (l - (1,1,1)) of the per-wave windows
             const bool buffered = (int)ti < a.wcap;
             float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
             for (int l = tid; l < PV; l += TB) {
-                float4 m = pw[0][l];
+                const int px = l % PN, py = (l / PN) % PN, pz = l / (PN * PN);
+                float4 m = mv[l];
+                if (px >= 1 && px <= WN && py >= 1 && py <= WN && pz >= 1 && pz <= WN) {
+                    const int l6 = ((pz - 1) * WN + py - 1) * WN + px - 1;
 #pragma unroll
-                for (int w = 1; w < NW; ++w) {
-                    const float4 u = pw[w][l];
-                    m.x += u.x;
-                    m.y += u.y;
-                    m.z += u.z;
-                    m.w += u.w;
+                    for (int w = 0; w < NW; ++w) {
+                        const float4 u = pw[w][l6];
+                        m.x += u.x;
+                        m.y += u.y;
+                        m.z += u.z;
+                        m.w += u.w;
+                    }
                 }
                 if (buffered) {
                     wb[l] = m;
                 } else if (m.x != 0.f) {
-                    const int gi = t0[0] - 1 + l % PN, gj = t0[1] - 1 + (l / PN) % PN, gk = t0[2] - 1 + l / (PN * PN);
+                    const int gi = t0[0] - 1 + px, gj = t0[1] - 1 + py, gk = t0[2] - 1 + pz;
                     if (in_grid(g, gi, gj, gk)) {
                         float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
                         atomic_add_f32(cp + 0, m.x);
@@ -673,6 +624,12 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                         flag_node(a.nflag, sc, gi, gj, gk);
                     }
                 }
+            }
+            if (buffered && tid < 27) {
+                // node blocks this window covers: the tile's and its 26 neighbours
+                const int bx = tc[0] + tid % 3 - 1, by = tc[1] + (tid / 3) % 3 - 1, bz = tc[2] + tid / 9 - 1;
+                if (bx >= 0 && by >= 0 && bz >= 0 && bx < sc.ntx && by < sc.nty && bz < sc.ntz)
+                    a.wflag[sc.toff + morton_tile(bx, by, bz)] = 1u;
             }
         }
         __syncthreads();
@@ -691,6 +648,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 template <bool TAPE>
 __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes, int nscenes, int slots,
                                                     const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
+                                                    uint32_t* __restrict__ wflag,
                                                     const float4* __restrict__ wbuf, int wcap,
                                                     float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                     uint32_t* __restrict__ flags8) {
@@ -706,11 +664,9 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
         if (tx < 0 || ty < 0 || tz < 0 || tx >= nt[0] || ty >= nt[1] || tz >= nt[2]) return 0xFFFFFFFFu;
         return tslot[sc.toff + morton_tile(tx, ty, tz)];
     };
-    // any covering tile active, or global-atomic P2G here?
-    const bool fb = nflag[B] != 0u;
-    bool mine = false;
-    if (lane < 27) mine = slot_of(bc[0] + lane % 3 - 1, bc[1] + (lane / 3) % 3 - 1, bc[2] + lane / 9 - 1) != 0xFFFFFFFFu;
-    if (!fb && __ballot(mine) == 0ull) return;
+    // covered by a buffered window, or global-atomic P2G here?
+    const bool fb = nflag[B] != 0u, wb = wflag[B] != 0u;
+    if (!fb && !wb) return;
     const MlsGeom g = sc.g;
     const int lc[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
     const int i = bc[0] * TT + lc[0], j = bc[1] * TT + lc[1], k = bc[2] * TT + lc[2];
@@ -757,6 +713,7 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
     }
     if (lane == 0) {
         if (fb) nflag[B] = 0u;
+        if (wb) wflag[B] = 0u;
         if (TAPE)
             flags8[sc.boff + ((bc[2] * TT / GB) * sc.gby + bc[1] * TT / GB) * sc.gbx + bc[0] * TT / GB] = 1u;
     }
@@ -906,6 +863,7 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.count = w.count;
     a.src_out = nullptr;
     a.nflag = w.nflag;
+    a.wflag = w.wflag;
     a.wbuf = w.wbuf;
     a.wcap = w.wcap;
     a.scenes = w.scenes;
@@ -916,7 +874,7 @@ template <bool TAPE>
 void grid_update(const TiledWork& w, float4* mp, float4* gv, uint32_t* flags8, hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
     k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, w.tslot, w.nflag,
-                                                                w.wbuf, w.wcap, mp, gv, flags8);
+                                                                w.wflag, w.wbuf, w.wcap, mp, gv, flags8);
 }
 
 }  // namespace

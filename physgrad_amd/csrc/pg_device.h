@@ -180,11 +180,23 @@ struct Stencil {
     float w[3][3];
 };
 
+// Position in cell units and the quadratic stencil's base cell, with explicit
+// IEEE rounding (no contraction into an FMA) so that every kernel that bins or
+// stencils a particle computes the same base for it.
+PG_DEV float grid_coord(float x, float o, float inv_dx) {
+#pragma clang fp contract(off)
+    return (x - o) * inv_dx;
+}
+PG_DEV int stencil_base(float xs) {
+#pragma clang fp contract(off)
+    return (int)floorf(xs - 0.5f);
+}
+
 PG_DEV void make_stencil(const float* x, const float* o, float inv_dx, Stencil& s) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float xs = (x[a] - o[a]) * inv_dx;
-        int b = (int)floorf(xs - 0.5f);
+        const float xs = grid_coord(x[a], o[a], inv_dx);
+        const int b = stencil_base(xs);
         float f = xs - (float)b;
         s.b[a] = b;
         s.f[a] = f;

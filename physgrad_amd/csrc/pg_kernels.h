@@ -130,6 +130,7 @@ struct TiledWork {
     uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
     uint32_t* tslot;       // [tile_slots] index of the tile in the active list, or ~0
     uint32_t* nflag;       // [tile_slots] 4^3 node blocks hit by out-of-window P2G (global atomics)
+    uint32_t* wflag;       // [tile_slots] 4^3 node blocks covered by a buffered P2G window
     float4* wbuf;          // [wcap][216] P2G window sums of the active tiles
     int wcap;              // active tiles with a window buffer; later ones add with atomics
     int nblocks_launch;    // persistent grid size of the tile kernels

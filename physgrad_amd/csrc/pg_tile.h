@@ -41,7 +41,7 @@ __device__ __forceinline__ uint32_t tile_key(const float* x, const MlsGeom& g, i
     const int nt[3] = {ntx, nty, ntz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int b = (int)floorf((x[a] - g.o[a]) * g.inv_dx - 0.5f);
+        int b = stencil_base(grid_coord(x[a], g.o[a], g.inv_dx));
         int tc = b >= 0 ? b / TT : 0;
         t[a] = tc < nt[a] ? tc : nt[a] - 1;
     }
