@@ -722,6 +722,7 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
             (c->have_mass && (rc = valloc(c, A, &t.mass, c->cap, false))) ||
             (rc = valloc(c, A, &t.perm, c->cap, false)) || (rc = valloc(c, A, &t.src, c->cap, false)) ||
             (rc = valloc(c, A, &t.alist, 4 * (size_t)w.tile_slots, false)) ||
+            (rc = valloc(c, A, &t.tslot, (size_t)w.tile_slots, false)) ||
             (rc = valloc(c, A, &t.nactive, 1, true)) || (rc = valloc(c, A, &t.mp, G, true)) ||
             (rc = valloc(c, A, &t.gv, G, true)) || (rc = valloc(c, A, &t.flags, nblk, true)))
             goto fail;

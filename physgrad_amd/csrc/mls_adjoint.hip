@@ -45,6 +45,8 @@ struct AdjArgs {
     float4* agv;            // grid accumulator of the velocity adjoint (x,y,z used)
     const float4* apam;     // (am, ap) per node
     float* aE_user;         // dL/dE in user order (or NULL)
+    float4* wbuf;           // per-tile window sums of the grid adjoint (by index in alist)
+    int wcap;
     const SceneInfo* scenes;
     size_t cap;
 };
@@ -63,190 +65,326 @@ PG_DEV void weight_grads(const Stencil& st, float dw[3][3]) {
     }
 }
 
-__global__ void __launch_bounds__(TB) k_adj_g2p(AdjArgs a) {
-    constexpr int NP = 27;               // 9 nodes x 3 channels per slice
+constexpr int APASS = 1280;   // particles per LDS pass of a tile (above most full tiles)
+
+// The reverse of one particle's G2P and state update, given its stencil, F and
+// the adjoints of its new state (ax, aF, av, aC):
+//   C' = 4/dx sum_n w u_n (n - f)^T,  x' = x + dt v',  F' = (I + dt C') F
+// emits the velocity adjoint g_n = w_n (anv + 4/dx^2 anc d_n) of each stencil
+// node (d_n = (n - f) dx) and returns the adjoints of x and F.
+__device__ __forceinline__ float pick3(const float* w, int i) { return i == 0 ? w[0] : (i == 1 ? w[1] : w[2]); }
+
+template <bool ROLLED, class Load, class Emit>
+__device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const MlsGeom& g, const Stencil& st,
+                                                 const float* F, const float* ax, const float* aF,
+                                                 const float* av, const float* aC, float* axo, float* aFo) {
+    const float s4 = 4.f * g.inv_dx * g.inv_dx;
+    float v[3], nc[9];
+    if (ROLLED) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) nc[c] = 0.f;
+#pragma unroll 1
+        for (int n = 0; n < 27; ++n) {
+            const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
+            const float4 u = load(ix, iy, iz);
+            const float w = pick3(st.w[0], ix) * pick3(st.w[1], iy) * pick3(st.w[2], iz);
+            const float d[3] = {(float)ix - st.f[0], (float)iy - st.f[1], (float)iz - st.f[2]};
+            const float uu[3] = {u.x, u.y, u.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int e = 0; e < 3; ++e) nc[3 * c + e] += 4.f * g.inv_dx * w * uu[c] * d[e];
+        }
+    } else {
+        g2p_sum(load, st, g.inv_dx, v, nc);
+    }
+    float dw[3][3];
+    weight_grads(st, dw);
+    float anv[3], anc[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) anv[c] = av[c] + g.dt * ax[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            anc[3 * i + j] = aC[3 * i + j] + g.dt * (aF[3 * i] * F[3 * j] + aF[3 * i + 1] * F[3 * j + 1] +
+                                                     aF[3 * i + 2] * F[3 * j + 2]);
+            aFo[3 * i + j] = aF[3 * i + j] + g.dt * (nc[i] * aF[j] + nc[3 + i] * aF[3 + j] + nc[6 + i] * aF[6 + j]);
+        }
+    float af[3] = {0.f, 0.f, 0.f};
+    auto node = [&](int ix, int iy, int iz, float wx, float wy, float wz, float dwx, float dwy, float dwz) {
+                const float w = wx * wy * wz;
+                const float d[3] = {((float)ix - st.f[0]) * g.dx, ((float)iy - st.f[1]) * g.dx,
+                                    ((float)iz - st.f[2]) * g.dx};
+                const float4 u4 = load(ix, iy, iz);
+                const float u[3] = {u4.x, u4.y, u4.z};
+                float t[3], aw = 0.f;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    t[c] = anv[c] + s4 * (anc[3 * c] * d[0] + anc[3 * c + 1] * d[1] + anc[3 * c + 2] * d[2]);
+                    aw += u[c] * t[c];
+                }
+#pragma unroll
+                for (int e = 0; e < 3; ++e) af[e] -= g.dx * s4 * w * (anc[e] * u[0] + anc[3 + e] * u[1] + anc[6 + e] * u[2]);
+                af[0] += aw * dwx * wy * wz;
+                af[1] += aw * wx * dwy * wz;
+                af[2] += aw * wx * wy * dwz;
+                emit(ix, iy, iz, make_float4(w * t[0], w * t[1], w * t[2], 0.f));
+    };
+    if (ROLLED) {
+        // rare path: a rolled loop, stencil values picked by selects
+#pragma unroll 1
+        for (int n = 0; n < 27; ++n) {
+            const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
+            node(ix, iy, iz, pick3(st.w[0], ix), pick3(st.w[1], iy), pick3(st.w[2], iz), pick3(dw[0], ix),
+                 pick3(dw[1], iy), pick3(dw[2], iz));
+        }
+    } else {
+#pragma unroll
+        for (int iz = 0; iz < 3; ++iz) {
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix)
+                    node(ix, iy, iz, st.w[0][ix], st.w[1][iy], st.w[2][iz], dw[0][ix], dw[1][iy], dw[2][iz]);
+            __builtin_amdgcn_sched_barrier(0);   // one 9-node slice in flight: bounds the registers
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) axo[c] = ax[c] + af[c] * g.inv_dx;
+}
+
+// A1: tiles of binning n; the particles of a pass are counted by stencil-base
+// cell and processed in rounds (lane c of a wave takes the next particle of
+// cell c), exactly like the forward tile kernel, so the velocity adjoint of a
+// round scatters into the wave's LDS window with plain read-modify-writes.
+// The windows are summed into the tile's buffer; the reverse grid update
+// gathers them.  A particle whose stencil base lies outside its tile (a
+// clamped key) scatters with global atomics into agv.
+__global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
+    constexpr int PT = APASS / TB;
+    static_assert(APASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
     __shared__ float4 sv[WV];
-    __shared__ float sa[3 * WV];         // velocity-adjoint window, channel-major
-    __shared__ float scr[TB / 64][64 * NP];
+    __shared__ float4 pw[NW][WV];
+    __shared__ float lx[3][APASS];
+    __shared__ uint32_t lo[APASS];            // output position (the adjoint's index), cell order
+    __shared__ uint32_t lcnt[65], lstart[65];
+    __shared__ uint32_t s_max, s_tot;
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float* myscr = scr[wave];
+    const int ccx = lane & 3, ccz = (lane >> 2) & 3, ccy = lane >> 4;
+    const int cnb = (ccz * WN + ccy) * WN + ccx;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
-        const SceneInfo sc = a.scenes[te.w];   // by value: kept in registers across the stores
+        const SceneInfo sc = a.scenes[te.w];
         const MlsGeom g = sc.g;
         const float4* __restrict__ gvs = a.gv + sc.goff;
         float4* __restrict__ agvs = a.agv + sc.goff;
-        const float s4 = 4.f * g.inv_dx * g.inv_dx;
         const uint32_t beg = te.y, cnt = te.z;
         int t0[3];
         tile_origin(te.x - sc.toff, t0);
-        for (int l = tid; l < WV; l += TB) {
-            int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
+        static_assert(WV <= TB, "one window node per thread");
+        if (tid < WV) {
+            const int gi = t0[0] + tid % WN, gj = t0[1] + (tid / WN) % WN, gk = t0[2] + tid / (WN * WN);
             float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
-            sv[l] = u;
-            sa[l] = 0.f; sa[WV + l] = 0.f; sa[2 * WV + l] = 0.f;
+            sv[tid] = u;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) pw[w][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        __syncthreads();
-        // output positions of the step in cell order; src gives the input slot
-        for (uint32_t c0 = 0; c0 < cnt; c0 += TB) {
-            const uint32_t r = c0 + tid;
-            const bool act = r < cnt;
-            const uint32_t o = beg + r;
-            const uint32_t s = act ? a.src[o] : 0u;
-            float x[3] = {0.f, 0.f, 0.f}, F[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
-            float ax[3] = {0.f, 0.f, 0.f}, aF[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            float av[3] = {0.f, 0.f, 0.f}, aC[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (act) {
-                load_xF(a.S, a.cap, s, x, F);
-                load_adj(a.abar, a.cap, o, ax, aF, av, aC);
+        for (uint32_t pb = 0; pb < cnt; pb += APASS) {
+            const uint32_t pc = min((uint32_t)APASS, cnt - pb);
+            if (tid < 65) lcnt[tid] = 0;
+            __syncthreads();
+            uint32_t key[PT];
+            float4 k0[PT];
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                const uint32_t r = tid + k * TB;
+                if (r < pc) k0[k] = a.S[a.src[beg + pb + r]];
             }
-            Stencil st;
-            make_stencil(x, g.o, g.inv_dx, st);
-            float dw[3][3];
-            weight_grads(st, dw);
-            const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-            const bool inwin = act && l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
-            // recompute C' = 4/dx^2 sum w v_i d^T
-            float nc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ii = 0; ii < 3; ++ii)
+            for (int k = 0; k < PT; ++k) {
+                const uint32_t r = tid + k * TB;
+                key[k] = 0u;
+                if (r < pc) {
+                    const float xx[3] = {k0[k].x, k0[k].y, k0[k].z};
+                    int l3[3];
+                    bool in = true;
 #pragma unroll
-                for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) {
-                        const float w = st.w[0][ii] * st.w[1][jj] * st.w[2][kk];
-                        const float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
-                                            ((float)kk - st.f[2]) * g.dx};
-                        float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (inwin) {
-                            u = sv[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
-                        } else if (act) {
-                            const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                            if (in_grid(g, gi, gj, gk)) u = gvs[node_id(g, gi, gj, gk)];
-                        }
-                        const float uv[3] = {u.x, u.y, u.z};
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-#pragma unroll
-                            for (int e = 0; e < 3; ++e) nc[3 * c + e] += w * uv[c] * d[e];
+                    for (int d = 0; d < 3; ++d) {
+                        l3[d] = stencil_base(grid_coord(xx[d], g.o[d], g.inv_dx)) - t0[d];
+                        in = in && l3[d] >= 0 && l3[d] < TT;
                     }
-#pragma unroll
-            for (int c = 0; c < 9; ++c) nc[c] *= s4;
-            // x' = x + dt v', F' = (I + dt C') F
-            float anv[3], anc[9], aFo[9];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) anv[c] = av[c] + g.dt * ax[c];
-#pragma unroll
-            for (int ii = 0; ii < 3; ++ii)
-#pragma unroll
-                for (int jj = 0; jj < 3; ++jj) {
-                    anc[3 * ii + jj] = aC[3 * ii + jj] + g.dt * (aF[3 * ii] * F[3 * jj] + aF[3 * ii + 1] * F[3 * jj + 1] +
-                                                                 aF[3 * ii + 2] * F[3 * jj + 2]);
-                    aFo[3 * ii + jj] = aF[3 * ii + jj] + g.dt * (nc[ii] * aF[jj] + nc[3 + ii] * aF[3 + jj] +
-                                                                 nc[6 + ii] * aF[6 + jj]);
+                    const int cell = in ? l3[0] + TT * l3[2] + TT * TT * l3[1] : 64;
+                    key[k] = ((uint32_t)cell << 16) | atomicAdd(&lcnt[cell], 1u);
                 }
-            float af[3] = {0.f, 0.f, 0.f};
-            const int base = inwin ? (l2 * WN + l1) * WN + l0 : -1;
-#pragma unroll
-            for (int ii = 0; ii < 3; ++ii) {
-#pragma unroll
-                for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) {
-                        const float wx = st.w[0][ii], wy = st.w[1][jj], wz = st.w[2][kk];
-                        const float w = wx * wy * wz;
-                        const float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
-                                            ((float)kk - st.f[2]) * g.dx};
-                        float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-                        bool ok = false;
-                        int gi = 0, gj = 0, gk = 0;
-                        if (inwin) {
-                            u = sv[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
-                            ok = true;
-                        } else if (act) {
-                            gi = st.b[0] + ii; gj = st.b[1] + jj; gk = st.b[2] + kk;
-                            ok = in_grid(g, gi, gj, gk);
-                            if (ok) u = gvs[node_id(g, gi, gj, gk)];
-                        }
-                        const float uv[3] = {u.x, u.y, u.z};
-                        float g3[3] = {0.f, 0.f, 0.f};
-                        if (ok) {
-                            float Cd[3];
-#pragma unroll
-                            for (int c = 0; c < 3; ++c)
-                                Cd[c] = anc[3 * c] * d[0] + anc[3 * c + 1] * d[1] + anc[3 * c + 2] * d[2];
-                            float aw = 0.f;
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) {
-                                g3[c] = w * (anv[c] + s4 * Cd[c]);
-                                aw += uv[c] * (anv[c] + s4 * Cd[c]);
-                            }
-#pragma unroll
-                            for (int e = 0; e < 3; ++e)
-                                af[e] -= g.dx * s4 * w * (anc[e] * uv[0] + anc[3 + e] * uv[1] + anc[6 + e] * uv[2]);
-                            af[0] += aw * dw[0][ii] * wy * wz;
-                            af[1] += aw * wx * dw[1][jj] * wz;
-                            af[2] += aw * wx * wy * dw[2][kk];
-                            if (!inwin) {
-                                float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
-                                atomic_add_f32(cp + 0, g3[0]);
-                                atomic_add_f32(cp + 1, g3[1]);
-                                atomic_add_f32(cp + 2, g3[2]);
-                                g3[0] = g3[1] = g3[2] = 0.f;
-                            }
-                        }
-                        float* row = &myscr[lane * NP + 3 * (3 * jj + kk)];
-                        row[0] = g3[0]; row[1] = g3[1]; row[2] = g3[2];
-                    }
-                __builtin_amdgcn_wave_barrier();
-                wave_slice_reduce<3>(myscr, base, ii, sa);
-                __builtin_amdgcn_wave_barrier();
             }
-            if (act) {
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t cl = lcnt[lane];
+                uint32_t inc = cl;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) ax[c] += af[c] * g.inv_dx;
-                a.part[s] = make_float4(ax[0], ax[1], ax[2], aFo[0]);
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t t = __shfl_up(inc, off);
+                    if (lane >= off) inc += t;
+                }
+                lstart[lane] = inc - cl;
+                uint32_t mx = cl;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+                if (lane == 63) {
+                    lstart[64] = inc;
+                    s_tot = inc;
+                    s_max = mx;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                const uint32_t r = tid + k * TB;
+                if (r < pc) {
+                    const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
+                    lx[0][j] = k0[k].x;
+                    lx[1][j] = k0[k].y;
+                    lx[2][j] = k0[k].z;
+                    lo[j] = beg + pb + r;
+                }
+            }
+            __syncthreads();
+
+            auto particle = [&](uint32_t j, bool own) {
+                const uint32_t o = lo[j];
+                const uint32_t s = a.src[o];
+                float x[3] = {lx[0][j], lx[1][j], lx[2][j]};
+                float F[9], ax[3], aF[9], av[3], aC[9];
+                {
+                    const float4 q0 = a.S[s], q1 = a.S[a.cap + s], q2 = a.S[2 * a.cap + s];
+                    F[0] = q0.w; F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
+                    F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
+                }
+                load_adj(a.abar, a.cap, o, ax, aF, av, aC);
+                Stencil st;
+                make_stencil(x, g.o, g.inv_dx, st);
+                float axo[3], aFo[9];
+                if (own) {
+                    float4* __restrict__ win = pw[wave];
+                    adj_g2p_particle<false>([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; },
+                                     [&](int ix, int iy, int iz, float4 c) {
+                                         float4* t = win + cnb + (iz * WN + iy) * WN + ix;
+                                         float4 u = *t;
+                                         u.x += c.x;
+                                         u.y += c.y;
+                                         u.z += c.z;
+                                         *t = u;
+                                         lds_order();
+                                     },
+                                     g, st, F, ax, aF, av, aC, axo, aFo);
+                } else {
+                    adj_g2p_particle<true>(
+                        [&](int ix, int iy, int iz) {
+                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+                            return in_grid(g, gi, gj, gk) ? gvs[node_id(g, gi, gj, gk)]
+                                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+                        },
+                        [&](int ix, int iy, int iz, float4 c) {
+                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+                            if (!in_grid(g, gi, gj, gk)) return;
+                            float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
+                            atomic_add_f32(cp + 0, c.x);
+                            atomic_add_f32(cp + 1, c.y);
+                            atomic_add_f32(cp + 2, c.z);
+                        },
+                        g, st, F, ax, aF, av, aC, axo, aFo);
+                }
+                a.part[s] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
                 a.part[a.cap + s] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
                 a.part[2 * a.cap + s] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
-            }
+            };
+            const uint32_t maxcnt = s_max;
+            const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
+            for (uint32_t r = wave; r < maxcnt; r += NW)
+                if (r < mycnt) particle(mystart + r, true);
+            const uint32_t nout = lcnt[64], sout = lstart[64];
+            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, false);
+            __syncthreads();
         }
-        __syncthreads();
-        for (int l = tid; l < WV; l += TB) {
-            int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
-            float s0 = sa[l], s1 = sa[WV + l], s2 = sa[2 * WV + l];
-            if ((s0 != 0.f || s1 != 0.f || s2 != 0.f) && gi < g.nx && gj < g.ny && gk < g.nz) {
-                float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
-                atomic_add_f32(cp + 0, s0);
-                atomic_add_f32(cp + 1, s1);
-                atomic_add_f32(cp + 2, s2);
+        // the tile's velocity-adjoint window goes to its buffer (atomics once the buffers are used up)
+        const bool buffered = (int)ti < a.wcap;
+        float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
+        if (tid < WV) {
+            float4 m = pw[0][tid];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) {
+                m.x += pw[w][tid].x;
+                m.y += pw[w][tid].y;
+                m.z += pw[w][tid].z;
+            }
+            if (buffered) {
+                wb[tid] = m;
+            } else if (m.x != 0.f || m.y != 0.f || m.z != 0.f) {
+                const int gi = t0[0] + tid % WN, gj = t0[1] + (tid / WN) % WN, gk = t0[2] + tid / (WN * WN);
+                if (gi < g.nx && gj < g.ny && gk < g.nz) {
+                    float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
+                    atomic_add_f32(cp + 0, m.x);
+                    atomic_add_f32(cp + 1, m.y);
+                    atomic_add_f32(cp + 2, m.z);
+                }
             }
         }
         __syncthreads();
     }
 }
 
-// reverse of v_i = (p_i/m_i + dt g) * mask_i
+// A2: reverse of v_i = (p_i/m_i + dt g) * mask_i over the step's 8^3 blocks.
+// The velocity adjoint of node i is the sum of the buffered windows of the
+// tiles of binning n that cover it (the window of tile T spans nodes
+// [4T, 4T + 6)) plus the global accumulator.  Clears the accumulator, the
+// step's grid and flags.
 __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes, int nscenes,
-                                                  float4* __restrict__ agv_all, float4* __restrict__ apam_all,
+                                                  const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
+                                                  int wcap, float4* __restrict__ agv_all, float4* __restrict__ apam_all,
                                                   float4* __restrict__ mp_all, uint32_t* __restrict__ flags) {
     const int b = blockIdx.x;
     if (flags[b] == 0u) return;
-    const SceneInfo& sc = scenes[scene_of(scenes, nscenes, (uint32_t)b, [](const SceneInfo& s) { return s.boff; })];
+    const SceneInfo sc = scenes[scene_of(scenes, nscenes, (uint32_t)b, [](const SceneInfo& s) { return s.boff; })];
     const MlsGeom g = sc.g;
     float4* __restrict__ agv = agv_all + sc.goff;
     float4* __restrict__ apam = apam_all + sc.goff;
     float4* __restrict__ mp = mp_all + sc.goff;
     const int lb = b - (int)sc.boff;
     const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
-    const int i = bx * GB + (threadIdx.x & 7), j = by * GB + ((threadIdx.x >> 3) & 7), k = bz * GB + (threadIdx.x >> 6);
-    if (i < g.nx && j < g.ny && k < g.nz) {
-        const size_t id = node_id(g, i, j, k);
+    const int n3[3] = {bx * GB + (int)(threadIdx.x & 7), by * GB + (int)((threadIdx.x >> 3) & 7),
+                       bz * GB + (int)(threadIdx.x >> 6)};
+    const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
+    if (n3[0] < g.nx && n3[1] < g.ny && n3[2] < g.nz) {
+        const size_t id = node_id(g, n3[0], n3[1], n3[2]);
+        float4 av = agv[id];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int T[3], wl[3];
+            bool ok = true;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                T[ax] = n3[ax] / TT - ((e >> ax) & 1);
+                wl[ax] = n3[ax] - TT * T[ax];
+                ok = ok && T[ax] >= 0 && T[ax] < nt[ax] && wl[ax] < WN;
+            }
+            if (ok) {
+                const uint32_t sl = tslot[sc.toff + morton_tile(T[0], T[1], T[2])];
+                if (sl < (uint32_t)wcap) {
+                    const float4 u = wbuf[(size_t)sl * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
+                    av.x += u.x;
+                    av.y += u.y;
+                    av.z += u.z;
+                }
+            }
+        }
         const float4 q = mp[id];
-        const float4 av = agv[id];
         float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q.x > 0.f && !is_sticky(g, i, j, k)) {
+        if (q.x > 0.f && !is_sticky(g, n3[0], n3[1], n3[2])) {
             const float im = 1.f / q.x;
             r.y = av.x * im;
             r.z = av.y * im;
@@ -414,6 +552,8 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.gv = t.gv;
         a.abar = abar; a.part = part; a.agv = agv; a.apam = apam;
         a.aE_user = aE_user;
+        a.wbuf = w.wbuf;
+        a.wcap = w.wcap;
         a.scenes = w.scenes;
         a.cap = cap;
         {
@@ -422,7 +562,7 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         }
         {
             ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, agv, apam, t.mp, t.flags);
+            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, t.tslot, w.wbuf, w.wcap, agv, apam, t.mp, t.flags);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);

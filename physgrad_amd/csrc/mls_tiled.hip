@@ -42,7 +42,6 @@ namespace pg {
 namespace {
 
 constexpr int PASS = 1280;       // particles staged per LDS pass of a tile (> most full tiles)
-constexpr int NW = TB / 64;      // waves per workgroup
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 
 struct TileArgs {
@@ -71,9 +70,6 @@ struct TileArgs {
     size_t cap;
 };
 
-__device__ __forceinline__ uint32_t morton_tile(int tx, int ty, int tz) {
-    return expand3((uint32_t)tx) | (expand3((uint32_t)ty) << 1) | (expand3((uint32_t)tz) << 2);
-}
 
 // marks the 4^3 node block of node (gi, gj, gk) as holding global-atomic P2G
 __device__ __forceinline__ void flag_node(uint32_t* nflag, const SceneInfo& sc, int gi, int gj, int gk) {
@@ -88,71 +84,6 @@ __device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, in
         int b = stencil_base(grid_coord(x[a], g.o[a], g.inv_dx));
         int tc = b >= 0 ? b / TT : 0;
         t[a] = tc < nt[a] ? tc : nt[a] - 1;
-    }
-}
-
-// number of set bits of a wave mask below this lane
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Keeps the compiler from moving LDS accesses across this point.  The window
-// updates of one wave are read-modify-writes whose lanes touch overlapping
-// nodes in successive instructions; the LDS executes a wave's instructions in
-// issue order, so program order is all they need.
-__device__ __forceinline__ void lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
-
-// G2P sums v = sum_n w u_n and C = 4/dx sum_n w u_n (n - f)^T over the 27
-// stencil nodes, factored along the axes (x innermost).  load(x, y, z) gives
-// the (v, m) of stencil node (x, y, z).
-template <class Load>
-__device__ __forceinline__ void g2p_sum(Load load, const Stencil& st, float inv_dx, float* v, float* C) {
-    float wd[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int n = 0; n < 3; ++n) wd[a][n] = st.w[a][n] * ((float)n - st.f[a]);
-    float vv[3] = {0.f, 0.f, 0.f}, cx[3] = {0.f, 0.f, 0.f}, cy[3] = {0.f, 0.f, 0.f}, cz[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int z = 0; z < 3; ++z) {
-        float R[3] = {0.f, 0.f, 0.f}, Ry[3] = {0.f, 0.f, 0.f}, Rx[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int y = 0; y < 3; ++y) {
-            float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                const float4 u = load(x, y, z);
-                const float uu[3] = {u.x, u.y, u.z};
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    P[c] += st.w[0][x] * uu[c];
-                    Q[c] += wd[0][x] * uu[c];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                R[c] += st.w[1][y] * P[c];
-                Ry[c] += wd[1][y] * P[c];
-                Rx[c] += st.w[1][y] * Q[c];
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            vv[c] += st.w[2][z] * R[c];
-            cz[c] += wd[2][z] * R[c];
-            cy[c] += st.w[2][z] * Ry[c];
-            cx[c] += st.w[2][z] * Rx[c];
-        }
-        // at most one 9-node slice of loads in flight: bounds the registers
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    const float s = 4.f * inv_dx;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        v[c] = vv[c];
-        C[3 * c + 0] = s * cx[c];
-        C[3 * c + 1] = s * cy[c];
-        C[3 * c + 2] = s * cz[c];
     }
 }
 
@@ -845,17 +776,20 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
     perm[start[key[s]] + rank[s]] = (uint32_t)s;
 }
 
-hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32_t* nactive, hipStream_t st) {
+hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32_t* nactive, uint32_t* tslot,
+                    hipStream_t st) {
     ProfScope ps(w.prof, PK_BIN, st);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, w.tslot, w.scenes, w.nscenes);
+                                    nactive, tslot, w.scenes, w.nscenes);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
 }
 
-hipError_t bin(TiledWork& w, int n, hipStream_t st) { return bin_into(w, n, w.perm, w.active, w.nactive, st); }
+hipError_t bin(TiledWork& w, int n, hipStream_t st) {
+    return bin_into(w, n, w.perm, w.active, w.nactive, w.tslot, st);
+}
 
 void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.key_out = w.key;
@@ -870,10 +804,12 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.cap = cap;
 }
 
+// tslot: the binning whose tile kernel filled the window buffers
 template <bool TAPE>
-void grid_update(const TiledWork& w, float4* mp, float4* gv, uint32_t* flags8, hipStream_t st) {
+void grid_update(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* gv, uint32_t* flags8,
+                 hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
-    k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, w.tslot, w.nflag,
+    k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, tslot, w.nflag,
                                                                 w.wflag, w.wbuf, w.wcap, mp, gv, flags8);
 }
 
@@ -911,14 +847,14 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
     }
     // the grid update reads the window buffers by this binning's active list,
     // so it runs before the next binning
-    grid_update<false>(w, mp, gv, nullptr, st);
+    grid_update<false>(w, w.tslot, mp, gv, nullptr, st);
     for (int s = 0; s + 1 < nsteps; ++s) {
         set_io();
         {
             ProfScope ps(w.prof, PK_TILE_G2P2G, st);
             k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
         }
-        grid_update<false>(w, mp, gv, nullptr, st);
+        grid_update<false>(w, w.tslot, mp, gv, nullptr, st);
         e = bin(w, n, st);
         if (e != hipSuccess) return e;
         std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
@@ -939,7 +875,7 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
     if (n > 0)
         k_init_keys<<<(n + 255) / 256, 256, 0, st>>>(n, tape[0].S, tape[0].pid, w.scenes, w.nscenes, w.key, w.rank,
                                                      w.count);
-    hipError_t e = bin_into(w, n, tape[0].perm, tape[0].alist, tape[0].nactive, st);
+    hipError_t e = bin_into(w, n, tape[0].perm, tape[0].alist, tape[0].nactive, tape[0].tslot, st);
     if (e != hipSuccess) return e;
     // P2G of step 0 and its grid
     a.Sin = tape[0].S;
@@ -954,7 +890,7 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
         ProfScope ps(w.prof, PK_TILE_P2G, st);
         k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
     }
-    grid_update<true>(w, tape[0].mp, tape[0].gv, tape[0].flags, st);
+    grid_update<true>(w, tape[0].tslot, tape[0].mp, tape[0].gv, tape[0].flags, st);
     for (int s = 0; s < L; ++s) {
         TapeStep& cur = tape[s];
         TapeStep& nxt = tape[s + 1];
@@ -979,8 +915,8 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
                 ProfScope ps(w.prof, PK_TILE_G2P2G, st);
                 k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
             }
-            grid_update<true>(w, nxt.mp, nxt.gv, nxt.flags, st);
-            e = bin_into(w, n, nxt.perm, nxt.alist, nxt.nactive, st);
+            grid_update<true>(w, cur.tslot, nxt.mp, nxt.gv, nxt.flags, st);
+            e = bin_into(w, n, nxt.perm, nxt.alist, nxt.nactive, nxt.tslot, st);
             if (e != hipSuccess) return e;
         } else {
             // last step of the segment: G2P only, to fix the end state's storage order

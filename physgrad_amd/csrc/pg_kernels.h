@@ -147,6 +147,7 @@ struct TapeStep {
     uint32_t* src;      // output position of this step -> storage slot of this step
     uint32_t* alist;    // uint4 (tile, start, count, scene) per active tile
     uint32_t* nactive;
+    uint32_t* tslot;    // tile slot -> index in alist, or ~0
     float4* mp;         // P2G result (m, p) of the step; cleared by the reverse pass
     float4* gv;         // (v, m) after the grid update
     uint32_t* flags;    // 8^3 blocks holding the step's grid; cleared by the reverse pass

@@ -17,6 +17,7 @@ constexpr int PN = TT + 4;       // P2G window edge (nodes [t0 - 1, t0 + TT + 3)
                                  // particles that moved one cell out of the tile
 constexpr int PV = PN * PN * PN; // P2G window volume (512)
 constexpr int TB = 256;          // threads per tile workgroup
+constexpr int NW = TB / 64;      // waves per tile workgroup
 constexpr int GB = 8;            // grid-update block edge (nodes)
 
 __device__ __forceinline__ uint32_t expand3(uint32_t v) {
@@ -46,6 +47,10 @@ __device__ __forceinline__ uint32_t tile_key(const float* x, const MlsGeom& g, i
         t[a] = tc < nt[a] ? tc : nt[a] - 1;
     }
     return expand3((uint32_t)t[0]) | (expand3((uint32_t)t[1]) << 1) | (expand3((uint32_t)t[2]) << 2);
+}
+
+__device__ __forceinline__ uint32_t morton_tile(int tx, int ty, int tz) {
+    return expand3((uint32_t)tx) | (expand3((uint32_t)ty) << 1) | (expand3((uint32_t)tz) << 2);
 }
 
 __device__ __forceinline__ void tile_origin(uint32_t tile, int* t0) {
@@ -108,6 +113,71 @@ __device__ __forceinline__ void wave_slice_reduce(const float* scr, int base, in
         }
     }
     if (cur >= 0 && pl) atomicAdd(&win[c * WV + cur + off], acc);
+}
+
+// number of set bits of a wave mask below this lane
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Keeps the compiler from moving LDS accesses across this point.  The window
+// updates of one wave are read-modify-writes whose lanes touch overlapping
+// nodes in successive instructions; the LDS executes a wave's instructions in
+// issue order, so program order is all they need.
+__device__ __forceinline__ void lds_order() { __atomic_signal_fence(__ATOMIC_SEQ_CST); }
+
+// G2P sums v = sum_n w u_n and C = 4/dx sum_n w u_n (n - f)^T over the 27
+// stencil nodes, factored along the axes (x innermost).  load(x, y, z) gives
+// the (v, m) of stencil node (x, y, z).
+template <class Load>
+__device__ __forceinline__ void g2p_sum(Load load, const Stencil& st, float inv_dx, float* v, float* C) {
+    float wd[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) wd[a][n] = st.w[a][n] * ((float)n - st.f[a]);
+    float vv[3] = {0.f, 0.f, 0.f}, cx[3] = {0.f, 0.f, 0.f}, cy[3] = {0.f, 0.f, 0.f}, cz[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int z = 0; z < 3; ++z) {
+        float R[3] = {0.f, 0.f, 0.f}, Ry[3] = {0.f, 0.f, 0.f}, Rx[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const float4 u = load(x, y, z);
+                const float uu[3] = {u.x, u.y, u.z};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    P[c] += st.w[0][x] * uu[c];
+                    Q[c] += wd[0][x] * uu[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                R[c] += st.w[1][y] * P[c];
+                Ry[c] += wd[1][y] * P[c];
+                Rx[c] += st.w[1][y] * Q[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vv[c] += st.w[2][z] * R[c];
+            cz[c] += wd[2][z] * R[c];
+            cy[c] += st.w[2][z] * Ry[c];
+            cx[c] += st.w[2][z] * Rx[c];
+        }
+        // at most one 9-node slice of loads in flight: bounds the registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const float s = 4.f * inv_dx;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v[c] = vv[c];
+        C[3 * c + 0] = s * cx[c];
+        C[3 * c + 1] = s * cy[c];
+        C[3 * c + 2] = s * cz[c];
+    }
 }
 
 // scene owning an index, by binary search of a monotone offset field
