@@ -7,7 +7,7 @@ over all particles with the particle state resident in HBM.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
 Prints one JSON line: particle-steps/s of the tiled path (value), the plain
-GPU port measured in the same run, the roofline of the dominant kernel (the
+GPU port measured in the same run (forward, and forward+backward), the roofline of the dominant kernel (the
 fused G2P+P2G tile kernel, timed with HIP events on its launch stream) and the
 CPU baseline (FP32 port of the same step on one host core, bounded sample).
 """
@@ -60,14 +60,13 @@ def time_steps(ctx, steps, warmup):
     return a.elapsed_time(b) / 1e3, wall
 
 
-def fwd_bwd_leg(scene, steps, ckpt):
+def fwd_bwd_leg(scene, steps, ckpt, impl):
     """Forward with a checkpoint every `ckpt` steps plus the full backward (recompute
     and reverse of every segment) on the same 1M-particle workload; the adjoint
     seed is that of the centre-of-mass loss."""
     import torch
-    from physgrad_amd import _lib
     from physgrad_amd.solver import MPMContext
-    ctx = MPMContext(scene.params(impl=_lib.IMPL_TILED))
+    ctx = MPMContext(scene.params(impl=impl))
     ctx.set_particles(scene.x, scene.v)
     n = scene.n
     seed = torch.full((n, 3), 1.0 / n, dtype=torch.float32, device="cuda")
@@ -218,7 +217,11 @@ def main():
         pctx.close()
         plain = {"value": N * kpl / t_plain, "steps": kpl, "ms_per_step": 1e3 * t_plain / kpl}
 
-    fwd_bwd = None if args.no_bwd else fwd_bwd_leg(scene, args.bwd_steps, args.bwd_ckpt)
+    fwd_bwd = None if args.no_bwd else fwd_bwd_leg(scene, args.bwd_steps, args.bwd_ckpt, _lib.IMPL_TILED)
+    plain_fwd_bwd = None
+    if not args.no_bwd and not args.no_plain:
+        k = min(args.bwd_steps, args.bwd_ckpt)
+        plain_fwd_bwd = fwd_bwd_leg(scene, k, k, _lib.IMPL_PLAIN)
     grad = None if args.no_bwd else grad_leg()
 
     cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline(scene, args.cpu_steps)
@@ -249,6 +252,8 @@ def main():
         "grad": grad,
         "plain_port": plain,
         "speedup_vs_plain": (N * K / t_tiled) / plain["value"] if plain else None,
+        "plain_port_fwd_bwd": plain_fwd_bwd,
+        "speedup_vs_plain_fwd_bwd": fwd_bwd["value"] / plain_fwd_bwd["value"] if fwd_bwd and plain_fwd_bwd else None,
         "cpu_baseline": cpu,
         "cpu_reference": cpu_ref,
         "state_finite": finite,

@@ -705,7 +705,9 @@ int ensure_checkpoints(pg_mpm_ctx* c, int ncp) {
 }
 
 int ensure_tape(pg_mpm_ctx* c, int K) {
-    if (c->tape_len >= K + 1 && c->abar && (!c->have_E || c->tape[0].E) && (!c->have_mass || c->tape[0].mass))
+    const bool plain = c->p.impl == PG_IMPL_PLAIN;   // plain tapes hold no E / mass copies
+    if (c->tape_len >= K + 1 && c->abar && (plain || !c->have_E || c->tape[0].E) &&
+        (plain || !c->have_mass || c->tape[0].mass))
         return PG_OK;
     hipDeviceSynchronize();
     free_all(c->tape_allocs);
@@ -717,6 +719,12 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     auto& A = c->tape_allocs;
     int rc;
     for (auto& t : c->tape) {
+        if (plain) {   // user order throughout: states and dense grids only
+            if ((rc = valloc(c, A, &t.S, 6 * c->cap, false)) || (rc = valloc(c, A, &t.mp, G, true)) ||
+                (rc = valloc(c, A, &t.gv, G, true)))
+                goto fail;
+            continue;
+        }
         if ((rc = valloc(c, A, &t.S, 6 * c->cap, false)) || (rc = valloc(c, A, &t.pid, c->cap, false)) ||
             (c->have_E && (rc = valloc(c, A, &t.E, c->cap, false))) ||
             (c->have_mass && (rc = valloc(c, A, &t.mass, c->cap, false))) ||
@@ -749,8 +757,7 @@ extern "C" {
 
 int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every, void* stream) {
     if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
-    if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
-        return fail(c, PG_ERR_INVALID, "checkpointed rollouts need mode MLS_MPM with the tiled implementation");
+    if (c->p.mode != PG_MODE_MLS_MPM) return fail(c, PG_ERR_INVALID, "checkpointed rollouts need mode MLS_MPM");
     if (nsteps <= 0 || checkpoint_every <= 0) return fail(c, PG_ERR_INVALID, "nsteps and checkpoint_every must be > 0");
     const int K = checkpoint_every;
     const int ncp = (nsteps + K - 1) / K;
@@ -758,9 +765,18 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
     if ((rc = ensure_checkpoints(c, ncp)) || (rc = ensure_tape(c, K))) return rc;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e = hipSuccess;
+    const bool plain = c->p.impl == PG_IMPL_PLAIN;
+    const pg::MlsGeom g = mls_geom(c);
     for (int k = 0; k < ncp && e == hipSuccess; ++k) {
         pg::TapeStep& ck = c->ckpt[k];
         e = copy_planes(ck.S, c->S, c->cap, c->n, 6, st);
+        if (plain) {   // E and mass stay in user order: only the state moves
+            const int L = std::min(K, nsteps - k * K);
+            for (int s = 0; s < L && e == hipSuccess; ++s)
+                e = pg::mls_plain_step(g, c->n, c->cap, c->S, c->have_mass ? c->mass : nullptr,
+                                       c->have_E ? c->E : nullptr, c->mp, c->gv, st, &c->prof);
+            continue;
+        }
         if (e == hipSuccess) e = copy_u32(ck.pid, c->pid, c->n, st);
         if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(ck.E, c->E, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && c->have_mass)
@@ -805,11 +821,29 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     seed(c->uC, aC, 9 * (size_t)n);
     seed(c->uF, aF, 9 * (size_t)n);
     seed(c->aE, nullptr, (size_t)n);
+    const bool plain = c->p.impl == PG_IMPL_PLAIN;
+    const pg::MlsGeom g = mls_geom(c);
+    const float* pE = c->have_E ? c->E : nullptr;
+    const float* pm = c->have_mass ? c->mass : nullptr;
     for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
         const int L = std::min(K, c->ck_steps - k * K);
         pg::TapeStep* T = c->tape.data();
         const pg::TapeStep& ck = c->ckpt[k];
         e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
+        if (plain) {
+            // recompute the segment: tape[s].S is the state before step s, tape[s].mp/gv its grid
+            for (int s = 0; s < L && e == hipSuccess; ++s) {
+                e = copy_planes(T[s + 1].S, T[s].S, c->cap, n, 6, st);
+                if (e == hipSuccess)
+                    e = pg::mls_plain_step(g, n, c->cap, T[s + 1].S, pm, pE, T[s].mp, T[s].gv, st, &c->prof);
+            }
+            if (e == hipSuccess) e = pg::adjoint_from_user(n, c->cap, nullptr, c->ux, c->uv, c->uC, c->uF, c->abar, st);
+            if (e == hipSuccess)
+                e = pg::adjoint_segment_plain(g, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, pE, pm,
+                                              st, &c->prof);
+            if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, nullptr, c->abar, c->ux, c->uv, c->uC, c->uF, st);
+            continue;
+        }
         if (e == hipSuccess) e = copy_u32(T[0].pid, ck.pid, n, st);
         if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(T[0].E, ck.E, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess && c->have_mass)

@@ -25,6 +25,8 @@
 // particle's slot in step n); A3 writes the adjoint of the state before step n
 // at its slot of step n, which is what A1 of step n-1 reads by its output
 // position.
+#include <algorithm>
+
 #include "pg_tile.h"
 
 namespace pg {
@@ -399,6 +401,73 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
     if (threadIdx.x == 0) flags[b] = 0u;
 }
 
+// A3 for one particle: the reverse of its P2G (stress included) given the
+// grid adjoints of its stencil nodes (load(x, y, z, am, ap) -> false where the
+// node is outside the grid) and its partial adjoints (ax, aF) from A1
+// (read by part() after the node loop, so they are not live across it).
+// Returns the adjoints of x, F, v, C before the step and dL/dE.
+template <class Load, class Part>
+__device__ __forceinline__ void adj_p2g_particle(Load load, Part part, const MlsGeom& g, const Stencil& st,
+                                                 const float* F, const float* v, const float* C, float Eq,
+                                                 float mq, float* ax, float* aF, float* avq, float* aC, float& dE) {
+    const float s4 = 4.f * g.inv_dx * g.inv_dx;
+    const float kscale = -g.dt * g.p_vol * s4;
+    const float emu = 1.f / (2.f * (1.f + g.nu));
+    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+    const float mu = Eq * emu, lam = Eq * elam;
+    float tau[9], A[9];
+    kirchhoff(g.material, mu, lam, F, tau);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
+    float dw[3][3];
+    weight_grads(st, dw);
+    float aA[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float af[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) avq[c] = 0.f;
+    for (int ii = 0; ii < 3; ++ii)
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                const float wx = st.w[0][ii], wy = st.w[1][jj], wz = st.w[2][kk];
+                const float w = wx * wy * wz;
+                const float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
+                                    ((float)kk - st.f[2]) * g.dx};
+                float am, ap[3];
+                if (!load(ii, jj, kk, am, ap)) continue;
+                float aw = am * mq;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float contrib = mq * v[c] + A[3 * c] * d[0] + A[3 * c + 1] * d[1] + A[3 * c + 2] * d[2];
+                    aw += ap[c] * contrib;
+                    avq[c] += w * mq * ap[c];
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) aA[3 * c + e] += w * ap[c] * d[e];
+                }
+#pragma unroll
+                for (int e = 0; e < 3; ++e) af[e] -= g.dx * w * (A[e] * ap[0] + A[3 + e] * ap[1] + A[6 + e] * ap[2]);
+                af[0] += aw * dw[0][ii] * wy * wz;
+                af[1] += aw * wx * dw[1][jj] * wz;
+                af[2] += aw * wx * wy * dw[2][kk];
+            }
+    float4 p0, p1, p2;
+    part(p0, p1, p2);
+    ax[0] = p0.x + af[0] * g.inv_dx;
+    ax[1] = p0.y + af[1] * g.inv_dx;
+    ax[2] = p0.z + af[2] * g.inv_dx;
+    aF[0] = p0.w; aF[1] = p1.x; aF[2] = p1.y; aF[3] = p1.z; aF[4] = p1.w;
+    aF[5] = p2.x; aF[6] = p2.y; aF[7] = p2.z; aF[8] = p2.w;
+    float at[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        at[c] = kscale * aA[c];
+        aC[c] = mq * aA[c];
+    }
+    float amu = 0.f, alam = 0.f;
+    kirchhoff_vjp(g.material, mu, lam, F, at, aF, amu, alam);
+    dE = amu * emu + alam * elam;
+}
+
 __global__ void __launch_bounds__(TB) k_adj_p2g(AdjArgs a) {
     __shared__ float sp[4][WV];
     const XcdChunk ch(*a.nactive);
@@ -408,10 +477,6 @@ __global__ void __launch_bounds__(TB) k_adj_p2g(AdjArgs a) {
         const SceneInfo sc = a.scenes[te.w];   // by value: kept in registers across the stores
         const MlsGeom g = sc.g;
         const float4* __restrict__ apams = a.apam + sc.goff;
-        const float s4 = 4.f * g.inv_dx * g.inv_dx;
-        const float kscale = -g.dt * g.p_vol * s4;
-        const float emu = 1.f / (2.f * (1.f + g.nu));
-        const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
         const uint32_t beg = te.y, cnt = te.z;
         int t0[3];
         tile_origin(te.x - sc.toff, t0);
@@ -423,75 +488,119 @@ __global__ void __launch_bounds__(TB) k_adj_p2g(AdjArgs a) {
         }
         __syncthreads();
         for (uint32_t r = tid; r < cnt; r += TB) {
-            const uint32_t i = beg + r;
-            const uint32_t s = a.perm[i];
+            const uint32_t s = a.perm[beg + r];
             float x[3], F[9], v[3], C[9];
             load_state(a.S, a.cap, s, x, F, v, C);
             const float Eq = a.E ? a.E[s] : g.E;
             const float mq = a.mass ? a.mass[s] : g.p_mass;
-            const float mu = Eq * emu, lam = Eq * elam;
-            float tau[9], A[9];
-            kirchhoff(g.material, mu, lam, F, tau);
-#pragma unroll
-            for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
             Stencil st;
             make_stencil(x, g.o, g.inv_dx, st);
-            float dw[3][3];
-            weight_grads(st, dw);
             const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
             const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
-            float aA[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            float af[3] = {0.f, 0.f, 0.f}, avq[3] = {0.f, 0.f, 0.f};
-            for (int ii = 0; ii < 3; ++ii)
-                for (int jj = 0; jj < 3; ++jj)
-#pragma unroll
-                    for (int kk = 0; kk < 3; ++kk) {
-                        const float wx = st.w[0][ii], wy = st.w[1][jj], wz = st.w[2][kk];
-                        const float w = wx * wy * wz;
-                        float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
-                                      ((float)kk - st.f[2]) * g.dx};
-                        float am, ap[3];
-                        if (inwin) {
-                            int l = ((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii);
-                            am = sp[0][l]; ap[0] = sp[1][l]; ap[1] = sp[2][l]; ap[2] = sp[3][l];
-                        } else {
-                            int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                            if (!in_grid(g, gi, gj, gk)) continue;
-                            float4 q = apams[node_id(g, gi, gj, gk)];
-                            am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
-                        }
-                        float aw = am * mq;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            const float contrib = mq * v[c] + A[3 * c] * d[0] + A[3 * c + 1] * d[1] + A[3 * c + 2] * d[2];
-                            aw += ap[c] * contrib;
-                            avq[c] += w * mq * ap[c];
-#pragma unroll
-                            for (int e = 0; e < 3; ++e) aA[3 * c + e] += w * ap[c] * d[e];
-                        }
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) af[e] -= g.dx * w * (A[e] * ap[0] + A[3 + e] * ap[1] + A[6 + e] * ap[2]);
-                        af[0] += aw * dw[0][ii] * wy * wz;
-                        af[1] += aw * wx * dw[1][jj] * wz;
-                        af[2] += aw * wx * wy * dw[2][kk];
+            float ax[3], aF[9], avq[3], aC[9], dE;
+            adj_p2g_particle(
+                [&](int ii, int jj, int kk, float& am, float* ap) {
+                    if (inwin) {
+                        const int l = ((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii);
+                        am = sp[0][l]; ap[0] = sp[1][l]; ap[1] = sp[2][l]; ap[2] = sp[3][l];
+                        return true;
                     }
-            // partial adjoints from A1 (ax, aF after the G2P reverse)
-            const float4 p0 = a.part[s], p1 = a.part[a.cap + s], p2 = a.part[2 * a.cap + s];
-            float ax[3] = {p0.x + af[0] * g.inv_dx, p0.y + af[1] * g.inv_dx, p0.z + af[2] * g.inv_dx};
-            float aF[9] = {p0.w, p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w};
-            float at[9], aC[9];
-#pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                at[c] = kscale * aA[c];
-                aC[c] = mq * aA[c];
-            }
-            float amu = 0.f, alam = 0.f;
-            kirchhoff_vjp(g.material, mu, lam, F, at, aF, amu, alam);
+                    const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
+                    if (!in_grid(g, gi, gj, gk)) return false;
+                    const float4 q = apams[node_id(g, gi, gj, gk)];
+                    am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
+                    return true;
+                },
+                [&](float4& p0, float4& p1, float4& p2) {
+                    p0 = a.part[s]; p1 = a.part[a.cap + s]; p2 = a.part[2 * a.cap + s];
+                },
+                g, st, F, v, C, Eq, mq, ax, aF, avq, aC, dE);
             store_state(a.abar, a.cap, s, ax, aF, avq, aC);
-            if (a.aE_user) a.aE_user[a.pid[s]] += amu * emu + alam * elam;
+            if (a.aE_user) a.aE_user[a.pid[s]] += dE;
         }
         __syncthreads();
     }
+}
+
+// ---- the plain port of the reverse step: one thread per particle in user
+// order, global atomics into a dense velocity-adjoint grid, dense grid pass.
+// It is the baseline the tiled reverse is measured against.
+__global__ void __launch_bounds__(256) k_adj_g2p_plain(int n, size_t cap, MlsGeom g, const float4* __restrict__ S,
+                                                       const float4* __restrict__ gv, const float4* __restrict__ abar,
+                                                       float4* __restrict__ part, float4* __restrict__ agv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[3], F[9], ax[3], aF[9], av[3], aC[9];
+    load_xF(S, cap, i, x, F);
+    load_adj(abar, cap, i, ax, aF, av, aC);
+    Stencil st;
+    make_stencil(x, g.o, g.inv_dx, st);
+    float axo[3], aFo[9];
+    adj_g2p_particle<false>(
+        [&](int ix, int iy, int iz) {
+            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+            return in_grid(g, gi, gj, gk) ? gv[node_id(g, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        },
+        [&](int ix, int iy, int iz, float4 c) {
+            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+            if (!in_grid(g, gi, gj, gk)) return;
+            float* cp = reinterpret_cast<float*>(agv + node_id(g, gi, gj, gk));
+            atomic_add_f32(cp + 0, c.x);
+            atomic_add_f32(cp + 1, c.y);
+            atomic_add_f32(cp + 2, c.z);
+        },
+        g, st, F, ax, aF, av, aC, axo, aFo);
+    part[i] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
+    part[cap + i] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
+    part[2 * cap + i] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
+}
+
+__global__ void k_adj_grid_plain(MlsGeom g, float4* __restrict__ agv, float4* __restrict__ apam,
+                                 const float4* __restrict__ mp) {
+    const size_t G = (size_t)g.nx * g.ny * g.nz;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < G; id += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((size_t)g.nx * g.ny));
+        const float4 q = mp[id], av = agv[id];
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (q.x > 0.f && !is_sticky(g, i, j, k)) {
+            const float im = 1.f / q.x;
+            r.y = av.x * im;
+            r.z = av.y * im;
+            r.w = av.z * im;
+            r.x = -(av.x * q.y + av.y * q.z + av.z * q.w) * im * im;
+        }
+        apam[id] = r;
+        agv[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_adj_p2g_plain(int n, size_t cap, MlsGeom g, const float4* __restrict__ S,
+                                                       const float* __restrict__ E, const float* __restrict__ mass,
+                                                       const float4* __restrict__ apam, const float4* __restrict__ part,
+                                                       float4* __restrict__ abar, float* __restrict__ aE) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[3], F[9], v[3], C[9];
+    load_state(S, cap, i, x, F, v, C);
+    const float Eq = E ? E[i] : g.E;
+    const float mq = mass ? mass[i] : g.p_mass;
+    Stencil st;
+    make_stencil(x, g.o, g.inv_dx, st);
+    float ax[3], aF[9], avq[3], aC[9], dE;
+    adj_p2g_particle(
+        [&](int ii, int jj, int kk, float& am, float* ap) {
+            const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
+            if (!in_grid(g, gi, gj, gk)) return false;
+            const float4 q = apam[node_id(g, gi, gj, gk)];
+            am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
+            return true;
+        },
+        [&](float4& p0, float4& p1, float4& p2) {
+            p0 = part[i]; p1 = part[cap + i]; p2 = part[2 * cap + i];
+        },
+        g, st, F, v, C, Eq, mq, ax, aF, avq, aC, dE);
+    store_state(abar, cap, i, ax, aF, avq, aC);
+    if (aE) aE[i] += dE;
 }
 
 // user order <-> storage order conversions of the adjoint planes
@@ -500,7 +609,7 @@ __global__ void k_gather_user(int n, size_t cap, const uint32_t* __restrict__ pi
                               float4* __restrict__ abar) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const size_t u = pid[s];
+    const size_t u = pid ? pid[s] : (uint32_t)s;
     float x[3], v[3], C[9], F[9];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -520,7 +629,7 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
                                float* __restrict__ aF) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const size_t u = pid[s];
+    const size_t u = pid ? pid[s] : (uint32_t)s;
     float x[3], v[3], C[9], F[9];
     load_state(abar, cap, s, x, F, v, C);
 #pragma unroll
@@ -567,6 +676,30 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);
             k_adj_p2g<<<w.nblocks_launch, TB, 0, st>>>(a);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* tape, int L, float4* abar,
+                                 float4* part, float4* agv, float4* apam, float* aE_user, const float* E,
+                                 const float* mass, hipStream_t st, KernelProf* prof) {
+    const size_t G = (size_t)g.nx * g.ny * g.nz;
+    const int nb = (n + 255) / 256;
+    const int gb = (int)std::min<size_t>((G + 255) / 256, 16384);
+    for (int s = L - 1; s >= 0; --s) {
+        TapeStep& t = tape[s];
+        if (n > 0) {
+            ProfScope ps(prof, PK_ADJ_G2P, st);
+            k_adj_g2p_plain<<<nb, 256, 0, st>>>(n, cap, g, t.S, t.gv, abar, part, agv);
+        }
+        {
+            ProfScope ps(prof, PK_ADJ_GRID, st);
+            k_adj_grid_plain<<<gb, 256, 0, st>>>(g, agv, apam, t.mp);
+        }
+        if (n > 0) {
+            ProfScope ps(prof, PK_ADJ_P2G, st);
+            k_adj_p2g_plain<<<nb, 256, 0, st>>>(n, cap, g, t.S, E, mass, apam, part, abar, aE_user);
         }
     }
     return hipGetLastError();

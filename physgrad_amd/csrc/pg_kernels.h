@@ -161,7 +161,14 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
 // Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
                            float4* abar, float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st);
-// user-order adjoint arrays (x,v [3n], C,F [9n]; NULL = 0) -> adjoint planes in storage order (pid)
+// The plain port of the reverse pass over a segment recomputed with
+// mls_plain_step: tape[s].S is the state before step s (user order),
+// tape[s].mp / .gv the dense grid of step s.
+hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* tape, int L, float4* abar,
+                                 float4* part, float4* agv, float4* apam, float* aE_user, const float* E,
+                                 const float* mass, hipStream_t st, KernelProf* prof);
+// user-order adjoint arrays (x,v [3n], C,F [9n]; NULL = 0) -> adjoint planes in storage order (pid;
+// NULL = identity)
 hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float* ax, const float* av,
                              const float* aC, const float* aF, float4* abar, hipStream_t st);
 hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,

@@ -42,9 +42,10 @@ def oracle_reverse(O, sc, x, v, C, F, E, nsteps, seeds):
     return adj, aE
 
 
+@pytest.mark.parametrize("impl", [1, 0], ids=["tiled", "plain"])
 @pytest.mark.parametrize("material", [0, 1], ids=["fixed_corotated", "neo_hookean"])
 @pytest.mark.parametrize("nsteps,K", [(1, 1), (7, 3)], ids=["one_step", "three_segments"])
-def test_backward_vs_oracle_reverse(gpu, oracle_lib, material, nsteps, K):
+def test_backward_vs_oracle_reverse(gpu, oracle_lib, material, nsteps, K, impl):
     from physgrad_amd.solver import MPMContext
     O = oracle_lib
     rng = np.random.default_rng(7 + material + 10 * nsteps)
@@ -53,8 +54,8 @@ def test_backward_vs_oracle_reverse(gpu, oracle_lib, material, nsteps, K):
     C = rng.normal(0, 0.3, (n, 9)).astype(np.float32)
     F = (np.eye(3).reshape(1, 9) + rng.normal(0, 0.03, (n, 9))).astype(np.float32)
     seeds = [rng.normal(size=(n, c)) for c in (3, 3, 9, 9)]
-    print("material", material, "steps", nsteps, "K", K, flush=True)
-    ctx = MPMContext(sc.params(impl=1))
+    print("impl", impl, "material", material, "steps", nsteps, "K", K, flush=True)
+    ctx = MPMContext(sc.params(impl=impl))
     ctx.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
     ctx.forward_checkpointed(nsteps, K)
     got = ctx.backward(*[torch.as_tensor(s, dtype=torch.float32) for s in seeds])
