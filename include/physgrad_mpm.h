@@ -102,7 +102,7 @@ int pg_mpm_step(pg_mpm_ctx* ctx, int nsteps, void* stream);
 /* Simulated time and step count (reference getCurrentTime/getStepCount). */
 int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);
 
-/* Differentiable rollout (MLS-MPM, tiled implementation).
+/* Differentiable rollout (MLS-MPM, tiled or plain implementation).
  * pg_mpm_forward_checkpointed advances nsteps steps like pg_mpm_step and keeps
  * the state every checkpoint_every steps in device memory (the reference's
  * CheckpointStrategy::UNIFORM, src/differentiable_mpm.h:63-68).
@@ -110,8 +110,10 @@ int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);
  * segment from the last checkpoint: each segment is recomputed into a tape,
  * then reversed (G2P, grid update with sticky walls, P2G with elasticity).
  * Inputs: adjoints dL/d(final x, v, C, F) in the caller's particle order
- * (device arrays [3n],[3n],[9n],[9n]; NULL = zero).  Outputs: dL/d(initial
- * x, v, C, F) and dL/dE per particle ([n]); NULL outputs are skipped.  The
+ * ([3n],[3n],[9n],[9n]; NULL = zero).  Outputs: dL/d(initial x, v, C, F) and
+ * dL/dE per particle ([n]); NULL outputs are skipped.  Arrays may be in host
+ * or device memory; when any is host memory the call returns after the
+ * outputs are written, otherwise it is asynchronous on the stream.  The
  * reference's computeGradients (src/differentiable_mpm.h:772-783) reverses an
  * empty tape; this is the working replacement. */
 int pg_mpm_forward_checkpointed(pg_mpm_ctx* ctx, int nsteps, int checkpoint_every, void* stream);
@@ -197,6 +199,21 @@ int pg_launch_optimized_g2p(float* positions, float* velocities, const float* gr
 int pg_launch_fully_fused_g2p2g(float* positions, float* velocities, float* grid_mass, float* grid_momentum,
                                 float* grid_velocity, const float* origin, const float* cell, const int* dims, int n,
                                 float dt, const float* gravity, float flip_ratio, int order, void* stream);
+
+/* G2P2GKernelLauncher<T>::launchG2P2G (src/mpm_g2p2g_kernels.h:494-547) with
+ * the header kernels it launches (optimizedP2GKernel :343-413,
+ * calculateGridVelocityKernel :419-437, fusedG2P2GKernel :211-335): clear the
+ * grid, scatter mass and momentum with gravity added to the particle velocity,
+ * v = p/m on nodes with mass > 1e-10 (0 elsewhere), gather the plain weighted
+ * sum of node velocities (not normalised), v = flip v_old + (1-flip) v_grid
+ * when use_affine_pic is set and v = v_grid otherwise, x += v dt.
+ * Arrays as for pg_launch_fully_fused_g2p2g; memory = PG_HOST stages host
+ * arrays through device copies and returns after the results are back
+ * (the reference passes host containers to this launcher). */
+int pg_launch_g2p2g(float* positions, float* velocities, float* grid_mass, float* grid_momentum,
+                    float* grid_velocity, const float* origin, const float* cell, const int* dims, int n, float dt,
+                    const float* gravity, int use_affine_pic, float flip_ratio, int order, int memory,
+                    void* stream);
 
 /* Spatial hash: cell ids (Morton code of the clamped truncated cell coordinate,
  * modulo the cell count), stable sort of particle indices by cell id, per-cell

@@ -4,7 +4,9 @@
  *
  * Restates src/mpm_g2p2g_kernels.cu: evalQuadraticGPU/evalCubicGPU (:27-66),
  * optimizedP2GKernel (:72-157), optimizedG2PKernel (:163-271) and the phases
- * of fullyFusedG2P2GKernel (:277-500), particle by particle in index order.
+ * of fullyFusedG2P2GKernel (:277-500), particle by particle in index order,
+ * and the header launcher G2P2GKernelLauncher<T>::launchG2P2G
+ * (src/mpm_g2p2g_kernels.h:494-547).
  * The CUDA file does not compile and no NVIDIA toolchain exists here, so
  * these are pinned by closed forms (tests/test_oracle_launchers.py), not by
  * an executable reference: parity with the CUDA source is "unpinned" beyond
@@ -68,7 +70,7 @@ static void p2g(const float* pos, const float* vel, float* gm, float* gp, const 
 }
 
 static void g2p(float* pos, float* vel, const float* gm, const float* gv, const float* o, const float* h,
-                const int* d, int n, float dt, float flip, int order) {
+                const int* d, int n, float dt, float flip, int order, int normalise) {
     const int ks = order == 2 ? 3 : 4, off = ks / 2;
     for (int q = 0; q < n; ++q) {
         float* pm = pos + 4 * (size_t)q;
@@ -91,7 +93,7 @@ static void g2p(float* pos, float* vel, const float* gm, const float* gv, const 
                         }
                     }
                 }
-        if (tw > 1e-10f)
+        if (normalise && tw > 1e-10f)
             for (int c = 0; c < 3; ++c) nv[c] /= tw;
         float bv[3];
         for (int c = 0; c < 3; ++c) bv[c] = vo[c] * flip + nv[c] * (1.0f - flip);
@@ -109,7 +111,7 @@ void orc_launch_p2g(const float* pos, const float* vel, float* gm, float* gp, co
 
 void orc_launch_g2p(float* pos, float* vel, const float* gm, const float* gv, const float* o, const float* h,
                     const int* d, int n, float dt, float flip, int order) {
-    g2p(pos, vel, gm, gv, o, h, d, n, dt, flip, order);
+    g2p(pos, vel, gm, gv, o, h, d, n, dt, flip, order, 1);
 }
 
 void orc_launch_fused(float* pos, float* vel, float* gm, float* gp, float* gv, const float* o, const float* h,
@@ -123,5 +125,23 @@ void orc_launch_fused(float* pos, float* vel, float* gm, float* gp, float* gv, c
     for (size_t i = 0; i < G; ++i)
         if (gm[i] > 1e-10f)
             for (int c = 0; c < 3; ++c) gv[3 * i + c] = gp[3 * i + c] / gm[i];
-    g2p(pos, vel, gm, gv, o, h, d, n, dt, flip, order);
+    g2p(pos, vel, gm, gv, o, h, d, n, dt, flip, order, 1);
+}
+
+/* G2P2GKernelLauncher<T>::launchG2P2G, src/mpm_g2p2g_kernels.h:494-547:
+ * clear (:520-521), optimizedP2GKernel with gravity on the particle velocity
+ * (:343-413), calculateGridVelocityKernel (:419-437), fusedG2P2GKernel's
+ * un-normalised gather and PIC / blended update (:211-335). */
+void orc_launch_g2p2g(float* pos, float* vel, float* gm, float* gp, float* gv, const float* o, const float* h,
+                      const int* d, int n, float dt, const float* grav, int use_affine_pic, float flip, int order) {
+    size_t G = (size_t)d[0] * d[1] * d[2];
+    memset(gm, 0, G * sizeof(float));
+    memset(gp, 0, 3 * G * sizeof(float));
+    memset(gv, 0, 3 * G * sizeof(float));
+    float gdt[3] = {grav[0] * dt, grav[1] * dt, grav[2] * dt};
+    p2g(pos, vel, gm, gp, o, h, d, n, order, gdt);
+    for (size_t i = 0; i < G; ++i)
+        if (gm[i] > 1e-10f)
+            for (int c = 0; c < 3; ++c) gv[3 * i + c] = gp[3 * i + c] / gm[i];
+    g2p(pos, vel, gm, gv, o, h, d, n, dt, use_affine_pic ? flip : 0.0f, order, 0);
 }

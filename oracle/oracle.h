@@ -88,6 +88,8 @@ void orc_launch_g2p(float* pos, float* vel, const float* gm, const float* gv, co
                     const int* d, int n, float dt, float flip, int order);
 void orc_launch_fused(float* pos, float* vel, float* gm, float* gp, float* gv, const float* o, const float* h,
                       const int* d, int n, float dt, const float* grav, float flip, int order);
+void orc_launch_g2p2g(float* pos, float* vel, float* gm, float* gp, float* gv, const float* o, const float* h,
+                      const int* d, int n, float dt, const float* grav, int use_affine_pic, float flip, int order);
 
 /* ---------------- cell keys + counting sort ---------------- */
 unsigned orc_morton3(unsigned x, unsigned y, unsigned z);

@@ -90,6 +90,8 @@ _SIGNATURES = {
                                                ctypes.c_float, ctypes.c_int, _vp]),
     "pg_launch_fully_fused_g2p2g": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
                                                    ctypes.c_float, _vp, ctypes.c_float, ctypes.c_int, _vp]),
+    "pg_launch_g2p2g": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float, _vp,
+                                       ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, _vp]),
     "pg_spatial_hash_workspace_size": (ctypes.c_size_t, [ctypes.c_uint]),
     "pg_spatial_hash_build": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp,
                                              _vp, ctypes.c_size_t, _vp]),

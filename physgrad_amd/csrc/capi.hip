@@ -680,6 +680,16 @@ hipError_t copy_planes(float4* dst, const float4* src, size_t cap, int n, int pl
     return hipSuccess;
 }
 
+// true for device (or managed) memory; pageable or pinned host memory is host
+bool is_device_ptr(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // unregistered host memory: clear the error it leaves
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
 hipError_t copy_u32(uint32_t* dst, const uint32_t* src, int n, hipStream_t st) {
     return hipMemcpyAsync(dst, src, (size_t)n * 4, hipMemcpyDeviceToDevice, st);
 }
@@ -811,9 +821,15 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     const int n = c->n, K = c->ck_every;
     const int ncp = (c->ck_steps + K - 1) / K;
     hipError_t e = hipSuccess;
+    // seeds and outputs may be host or device arrays (copies use hipMemcpyDefault);
+    // with any host array the call returns after the outputs are written
+    bool any_host = false;
+    for (const void* q : {(const void*)ax, (const void*)av, (const void*)aC, (const void*)aF, (const void*)gx,
+                          (const void*)gv, (const void*)gC, (const void*)gF, (const void*)gE})
+        if (q && !is_device_ptr(q)) any_host = true;
     auto seed = [&](float* dst, const float* src, size_t count) {
         if (e != hipSuccess) return;
-        if (src) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToDevice, st);
+        if (src) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDefault, st);
         else e = pg::fill_u32(reinterpret_cast<uint32_t*>(dst), count, 0u, st);
     };
     seed(c->ux, ax, 3 * (size_t)n);
@@ -855,13 +871,14 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, st);
     }
     auto out = [&](float* dst, const float* src, size_t count) {
-        if (e == hipSuccess && dst) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && dst) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDefault, st);
     };
     out(gx, c->ux, 3 * (size_t)n);
     out(gv, c->uv, 3 * (size_t)n);
     out(gC, c->uC, 9 * (size_t)n);
     out(gF, c->uF, 9 * (size_t)n);
     out(gE, c->aE, (size_t)n);
+    if (e == hipSuccess && any_host) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_backward");
     return PG_OK;
 }

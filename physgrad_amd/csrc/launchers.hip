@@ -7,7 +7,11 @@
 //   G2P  normalised gather over nodes with mass > 1e-10, FLIP/PIC blend with
 //        the old velocity, x += v dt;
 //   fused clear -> P2G with gravity on the particle velocity -> v = p/m ->
-//        G2P.  The reference needs a cooperative launch with grid.sync() for
+//        G2P;
+//   the launcher class G2P2GKernelLauncher<T>::launchG2P2G
+//        (src/mpm_g2p2g_kernels.h:494-547 with the header kernels :211-453):
+//        the fused sequence with an un-normalised G2P sum and the blend
+//        applied only when use_affine_pic is set (pure PIC otherwise).  The reference needs a cooperative launch with grid.sync() for
 //        this; here the phases are stream-ordered kernels on the caller's
 //        stream, which gives the same ordering without co-residency limits.
 // Launch shape: 256 threads, one particle per thread.
@@ -104,9 +108,11 @@ __global__ void k_l_vel(const float* __restrict__ gm, const float* __restrict__ 
     }
 }
 
-// optimizedG2PKernel (:163-271)
+// optimizedG2PKernel (:163-271, normalise = true) and the header launcher's
+// fusedG2P2GKernel (src/mpm_g2p2g_kernels.h:211-335, normalise = false: the
+// plain weighted sum over nodes with mass > 1e-10)
 __global__ void k_l_g2p(float4* __restrict__ pos, float4* __restrict__ vel, const float* __restrict__ gm,
-                        const float* __restrict__ gv, LGeom g, int n, float dt, float flip) {
+                        const float* __restrict__ gv, LGeom g, int n, float dt, float flip, bool normalise) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const float4 pm = pos[q], vo = vel[q];
@@ -132,7 +138,7 @@ __global__ void k_l_g2p(float4* __restrict__ pos, float4* __restrict__ vel, cons
                     }
                 }
             }
-    if (tw > 1e-10f) {
+    if (normalise && tw > 1e-10f) {
         nv[0] /= tw;
         nv[1] /= tw;
         nv[2] /= tw;
@@ -182,7 +188,7 @@ extern "C" int pg_launch_optimized_g2p(float* positions, float* velocities, cons
     if (n > 0)
         pg::k_l_g2p<<<pg::blocks(n), 256, 0, (hipStream_t)stream>>>(
             reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities), grid_mass, grid_velocity, g, n,
-            dt, flip_ratio);
+            dt, flip_ratio, true);
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }
 
@@ -207,6 +213,70 @@ extern "C" int pg_launch_fully_fused_g2p2g(float* positions, float* velocities, 
     if (n > 0)
         pg::k_l_g2p<<<pg::blocks(n), 256, 0, st>>>(reinterpret_cast<float4*>(positions),
                                                    reinterpret_cast<float4*>(velocities), grid_mass, grid_velocity, g,
-                                                   n, dt, flip_ratio);
+                                                   n, dt, flip_ratio, true);
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
+}
+
+namespace pg {
+namespace {
+
+// clear -> P2G (gravity on the particle velocity) -> v = p/m -> G2P (plain
+// weighted sum) on device arrays
+hipError_t header_g2p2g(float4* pos, float4* vel, float* gm, float* gp, float* gv, const LGeom& g, int n, float dt,
+                        const float* gravity, float flip, hipStream_t st) {
+    const size_t G = (size_t)g.d[0] * g.d[1] * g.d[2];
+    const int gb = std::min(blocks(G), 16384);
+    k_l_clear<<<gb, 256, 0, st>>>(gm, gp, gv, G);
+    if (n > 0)
+        k_l_p2g<<<blocks(n), 256, 0, st>>>(pos, vel, gm, gp, g, n, gravity[0] * dt, gravity[1] * dt, gravity[2] * dt);
+    k_l_vel<<<gb, 256, 0, st>>>(gm, gp, gv, G);
+    if (n > 0) k_l_g2p<<<blocks(n), 256, 0, st>>>(pos, vel, gm, gv, g, n, dt, flip, false);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace pg
+
+extern "C" int pg_launch_g2p2g(float* positions, float* velocities, float* grid_mass, float* grid_momentum,
+                               float* grid_velocity, const float* origin, const float* cell, const int* dims, int n,
+                               float dt, const float* gravity, int use_affine_pic, float flip_ratio, int order,
+                               int memory, void* stream) {
+    pg::LGeom g;
+    if (pg::check_geom(origin, cell, dims, n, order, g) != PG_OK || !gravity) return PG_ERR_INVALID;
+    if (!grid_mass || !grid_momentum || !grid_velocity || (n > 0 && (!positions || !velocities)))
+        return PG_ERR_INVALID;
+    if (memory != PG_HOST && memory != PG_DEVICE) return PG_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    const float flip = use_affine_pic ? flip_ratio : 0.f;
+    if (memory == PG_DEVICE)
+        return pg::header_g2p2g(reinterpret_cast<float4*>(positions), reinterpret_cast<float4*>(velocities),
+                                grid_mass, grid_momentum, grid_velocity, g, n, dt, gravity, flip, st) == hipSuccess
+                   ? PG_OK
+                   : PG_ERR_HIP;
+    // host arrays: stage through device copies for the duration of the call
+    const size_t G = (size_t)dims[0] * dims[1] * dims[2];
+    const size_t pb = 16 * (size_t)std::max(n, 1), gbytes = 4 * G;
+    char* buf = nullptr;
+    if (hipMalloc(&buf, 2 * pb + 7 * gbytes) != hipSuccess) return PG_ERR_NOMEM;
+    float4* dp = reinterpret_cast<float4*>(buf);
+    float4* dv = reinterpret_cast<float4*>(buf + pb);
+    float* dm = reinterpret_cast<float*>(buf + 2 * pb);
+    float* dmom = dm + G;
+    float* dvel = dmom + 3 * G;
+    hipError_t e = hipSuccess;
+    if (n > 0) {
+        e = hipMemcpyAsync(dp, positions, 16 * (size_t)n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dv, velocities, 16 * (size_t)n, hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) e = pg::header_g2p2g(dp, dv, dm, dmom, dvel, g, n, dt, gravity, flip, st);
+    if (e == hipSuccess && n > 0) {
+        e = hipMemcpyAsync(positions, dp, 16 * (size_t)n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(velocities, dv, 16 * (size_t)n, hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(grid_mass, dm, gbytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(grid_momentum, dmom, 3 * gbytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(grid_velocity, dvel, 3 * gbytes, hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    hipFree(buf);
+    return e == hipSuccess && es == hipSuccess ? PG_OK : PG_ERR_HIP;
 }

@@ -146,3 +146,125 @@ def test_gpu_launchers_vs_oracle(gpu, oracle_lib, order):
     assert rel(dgm.cpu().numpy(), gm3) <= 1e-5
     assert rel(dvel.cpu().numpy(), v3) <= 1e-5
     assert rel(dpos.cpu().numpy(), p3) <= 1e-6
+
+
+def _olib_g2p2g(O):
+    L = olib(O)
+    L.orc_launch_g2p2g.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p, ctypes.c_int,
+                                   ctypes.c_float, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+    return L
+
+
+def _fixture_particles(f):
+    n = len(f["m"])
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = f["x0"]
+    pos[:, 3] = f["m"]
+    vel = np.zeros((n, 4), np.float32)
+    vel[:, :3] = f["v0"]
+    return pos, vel
+
+
+@pytest.mark.parametrize("name", ["box512_o2", "box512_o3", "two_o2", "two_o3"])
+def test_launcher_class_pic_matches_reference_solver(oracle_lib, name):
+    """G2P2GKernelLauncher::launchG2P2G in pure-PIC mode is the reference CPU
+    MPMSolver::step up to where gravity enters (particle velocity instead of
+    grid momentum): the oracle restatement must reproduce the reference
+    solver's fixtures (written by src/material_point_method.h itself) after
+    1, 2 and 10 steps."""
+    import os
+    from conftest import GOLDEN
+    L = _olib_g2p2g(oracle_lib)
+    f = np.load(os.path.join(GOLDEN, f"refcompat_{name}.npz"))
+    pos, vel = _fixture_particles(f)
+    o, h, d = (f["origin"].astype(np.float32), f["cell"].astype(np.float32), f["dims"].astype(np.int32))
+    grav = f["gravity"].astype(np.float32)
+    G = int(np.prod(d))
+    gm, gp, gv = np.zeros(G, np.float32), np.zeros(3 * G, np.float32), np.zeros(3 * G, np.float32)
+    n = len(f["m"])
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+    for s in range(1, 11):
+        L.orc_launch_g2p2g(pos.reshape(-1), vel.reshape(-1), gm, gp, gv, o, h, d, n, float(f["dt"]), grav, 0,
+                           0.95, int(f["order"]))
+        if s in (1, 2, 10):
+            assert rel(pos[:, :3], f[f"x_{s}"]) <= 1e-6, s
+            assert rel(vel[:, :3], f[f"v_{s}"]) <= 1e-5, s
+            assert rel(gm, f[f"grid_mass_{s}"]) <= 1e-6, s
+            assert rel(gv.reshape(-1, 3), f[f"grid_vel_{s}"]) <= 1e-5, s
+
+
+def test_launcher_class_blend(oracle_lib):
+    """use_affine_pic blends the old velocity with the un-normalised gather: for
+    one particle the gather returns S (v + g dt), S = sum of weights."""
+    L = _olib_g2p2g(oracle_lib)
+    rng = np.random.default_rng(5)
+    pos, vel, o, h, d = scene(rng, 1)
+    G = int(np.prod(d))
+    gm, gp, gv = np.zeros(G, np.float32), np.zeros(3 * G, np.float32), np.zeros(3 * G, np.float32)
+    g = np.array([0.0, -9.8, 0.0], np.float32)
+    flip, dt = 0.8, 1e-3
+    f = (pos[0, :3].astype(np.float64) - o) / h
+    f -= np.floor(f)
+
+    def w(x):
+        a = abs(x)
+        return 0.75 - x * x if a < 0.5 else (0.5 * (1.5 - a) ** 2 if a < 1.5 else 0.0)
+    S = np.prod([sum(w(f[a] - k) for k in (-1, 0, 1)) for a in range(3)])
+    v0 = vel[0, :3].astype(np.float64)
+    x0 = pos[0, :3].astype(np.float64)
+    L.orc_launch_g2p2g(pos.reshape(-1), vel.reshape(-1), gm, gp, gv, o, h, d, 1, dt, g, 1, flip, 2)
+    want_v = flip * v0 + (1 - flip) * S * (v0 + g * dt)
+    np.testing.assert_allclose(vel[0, :3], want_v, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(pos[0, :3], x0 + dt * want_v, rtol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("memory", [0, 1], ids=["host", "device"])
+@pytest.mark.parametrize("affine", [0, 1], ids=["pic", "blend"])
+def test_gpu_launcher_class_vs_oracle(gpu, oracle_lib, order, memory, affine):
+    import torch
+    from physgrad_amd import _lib
+    Lg = _lib.load()
+    L = _olib_g2p2g(oracle_lib)
+    rng = np.random.default_rng(40 + order + 2 * memory + 4 * affine)
+    n = 3000
+    pos, vel, o, h, d = scene(rng, n)
+    G = int(np.prod(d))
+    grav = np.array([0.0, -9.8, 0.0], np.float32)
+    print("order", order, "memory", memory, "affine", affine, flush=True)
+    p1, v1 = pos.copy(), vel.copy()
+    gm, gp, gv = np.zeros(G, np.float32), np.zeros(3 * G, np.float32), np.zeros(3 * G, np.float32)
+    L.orc_launch_g2p2g(p1.reshape(-1), v1.reshape(-1), gm, gp, gv, o, h, d, n, 1e-3, grav, affine, 0.9, order)
+    o_c = (ctypes.c_float * 3)(*o)
+    h_c = (ctypes.c_float * 3)(*h)
+    d_c = (ctypes.c_int * 3)(*d)
+    g_c = (ctypes.c_float * 3)(*grav)
+    if memory == 1:
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        dp, dv = torch.as_tensor(pos).cuda(), torch.as_tensor(vel).cuda()
+        dgm, dgp, dgv = (torch.full((G,), 3.0, device="cuda"), torch.full((3 * G,), 3.0, device="cuda"),
+                         torch.full((3 * G,), 3.0, device="cuda"))
+        _lib.check(Lg.pg_launch_g2p2g(P(dp), P(dv), P(dgm), P(dgp), P(dgv), o_c, h_c, d_c, n, 1e-3, g_c, affine, 0.9,
+                                      order, 1, stream))
+        torch.cuda.synchronize()
+        got = dp.cpu().numpy(), dv.cpu().numpy(), dgm.cpu().numpy(), dgv.cpu().numpy()
+    else:
+        hp, hv = pos.copy(), vel.copy()
+        hgm, hgp, hgv = np.full(G, 3.0, np.float32), np.full(3 * G, 3.0, np.float32), np.full(3 * G, 3.0, np.float32)
+        A = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        _lib.check(Lg.pg_launch_g2p2g(A(hp), A(hv), A(hgm), A(hgp), A(hgv), o_c, h_c, d_c, n, 1e-3, g_c, affine,
+                                      0.9, order, 0, None))
+        got = hp, hv, hgm, hgv
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+    assert rel(got[0], p1) <= 1e-6
+    assert rel(got[1], v1) <= 1e-5
+    assert rel(got[2], gm) <= 1e-5
+    assert rel(got[3], gv) <= 1e-5
