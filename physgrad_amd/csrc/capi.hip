@@ -196,22 +196,25 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
     w.tile_slots = (int)toff;
     w.total_blocks = (int)boff;
     w.nblocks_launch = 2048;
+    // active-list entries: one per chunk of at most CHUNK particles of a tile
+    const uint64_t max_entries = toff + c->cap / pg::CHUNK + 1;
+    w.max_entries = (int)std::min<uint64_t>(max_entries, 1ull << 31);
     w.prof = &c->prof;
     int rc;
     if ((rc = dalloc(c, &c->scenes_d, (size_t)ns)) || (rc = dalloc(c, &c->S, 6 * c->cap)) ||
         (rc = dalloc(c, &c->pid, c->cap)) || (rc = dalloc(c, &c->mp, goff)) || (rc = dalloc(c, &c->gv, goff)) ||
         (tiled && ((rc = dalloc(c, &c->S2, 6 * c->cap)) || (rc = dalloc(c, &c->pid2, c->cap)) ||
                    (rc = dalloc(c, &w.count, toff)) || (rc = dalloc(c, &w.start, toff)) ||
-                   (rc = dalloc(c, &w.active, 4 * toff)) || (rc = dalloc(c, &w.nactive, 1)) ||
+                   (rc = dalloc(c, &w.active, 4 * max_entries)) || (rc = dalloc(c, &w.nactive, 1)) ||
                    (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
                    (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
-                   (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, toff)) ||
+                   (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, 2 * toff)) ||
                    (rc = dalloc(c, &w.nflag, toff)) || (rc = dalloc(c, &w.wflag, toff)))))
         return rc;
     if (tiled) {
-        // P2G window buffers (8^3 nodes, 8 KiB each) for up to min(tile slots,
-        // capacity, 2^16) active tiles; tiles beyond that add with atomics
-        w.wcap = (int)std::min<uint64_t>(std::min<uint64_t>(toff, c->cap), 1u << 16);
+        // P2G window buffers (8^3 nodes, 8 KiB each) for up to min(active-list
+        // capacity, particle capacity, 2^16) entries; later entries add with atomics
+        w.wcap = (int)std::min<uint64_t>(std::min<uint64_t>(max_entries, c->cap), 1u << 16);
         if ((rc = dalloc(c, &w.wbuf, (size_t)w.wcap * 512))) return rc;
     }
     w.scenes = c->scenes_d;
@@ -224,6 +227,7 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
         hipMemset(w.nflag, 0, toff * 4);
         hipMemset(w.wflag, 0, toff * 4);
         hipMemset(w.tslot, 0xFF, toff * 4);
+        hipMemset(w.tslot + toff, 0, toff * 4);
     }
     hipError_t e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);
     return e == hipSuccess ? PG_OK : hip_fail(c, e, "scene table upload");
@@ -739,8 +743,8 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
             (c->have_E && (rc = valloc(c, A, &t.E, c->cap, false))) ||
             (c->have_mass && (rc = valloc(c, A, &t.mass, c->cap, false))) ||
             (rc = valloc(c, A, &t.perm, c->cap, false)) || (rc = valloc(c, A, &t.src, c->cap, false)) ||
-            (rc = valloc(c, A, &t.alist, 4 * (size_t)w.tile_slots, false)) ||
-            (rc = valloc(c, A, &t.tslot, (size_t)w.tile_slots, false)) ||
+            (rc = valloc(c, A, &t.alist, 4 * (size_t)w.max_entries, false)) ||
+            (rc = valloc(c, A, &t.tslot, 2 * (size_t)w.tile_slots, false)) ||
             (rc = valloc(c, A, &t.nactive, 1, true)) || (rc = valloc(c, A, &t.mp, G, true)) ||
             (rc = valloc(c, A, &t.gv, G, true)) || (rc = valloc(c, A, &t.flags, nblk, true)))
             goto fail;

@@ -67,7 +67,8 @@ PG_DEV void weight_grads(const Stencil& st, float dw[3][3]) {
     }
 }
 
-constexpr int APASS = 1280;   // particles per LDS pass of a tile (above most full tiles)
+constexpr int APASS = 1280;   // particles per LDS pass of a work item
+static_assert(CHUNK % APASS == 0, "a work item is a whole number of passes");
 
 // The reverse of one particle's G2P and state update, given its stencil, F and
 // the adjoints of its new state (ax, aF, av, aC):
@@ -345,7 +346,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
 // tiles of binning n that cover it (the window of tile T spans nodes
 // [4T, 4T + 6)) plus the global accumulator.  Clears the accumulator, the
 // step's grid and flags.
-__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes, int nscenes,
+__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes, int nscenes, int slots,
                                                   const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
                                                   int wcap, float4* __restrict__ agv_all, float4* __restrict__ apam_all,
                                                   float4* __restrict__ mp_all, uint32_t* __restrict__ flags) {
@@ -375,9 +376,12 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
                 ok = ok && T[ax] >= 0 && T[ax] < nt[ax] && wl[ax] < WN;
             }
             if (ok) {
-                const uint32_t sl = tslot[sc.toff + morton_tile(T[0], T[1], T[2])];
-                if (sl < (uint32_t)wcap) {
-                    const float4 u = wbuf[(size_t)sl * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
+                // every work item (chunk) of tile T
+                const uint32_t t = sc.toff + morton_tile(T[0], T[1], T[2]);
+                const uint32_t sl = tslot[t];
+                const uint32_t nb = sl < (uint32_t)wcap ? min(tslot[slots + t], (uint32_t)wcap - sl) : 0u;
+                for (uint32_t c = 0; c < nb; ++c) {
+                    const float4 u = wbuf[(size_t)(sl + c) * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
                     av.x += u.x;
                     av.y += u.y;
                     av.z += u.z;
@@ -671,7 +675,8 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         }
         {
             ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, t.tslot, w.wbuf, w.wcap, agv, apam, t.mp, t.flags);
+            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, t.tslot, w.wbuf, w.wcap, agv, apam,
+                                             t.mp, t.flags);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);

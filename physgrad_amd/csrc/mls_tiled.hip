@@ -9,18 +9,20 @@
 // starts and the list of non-empty tiles, and a scatter builds
 // perm = sorted position -> storage slot.
 //
-// Transfers.  One workgroup processes one tile at a time.  Per pass of up to
-// PASS particles, x and the storage slots are loaded in bulk and ordered by
-// stencil-base cell with an LDS counting sort (64 cells), then processed in
-// rounds: lane c of a wave takes the next particle of cell c.
+// Transfers.  One workgroup processes one work item at a time: a chunk of at
+// most CHUNK particles of one tile (dense tiles have several).  Per pass of up
+// to PASS particles, x and the storage slots are loaded in bulk and ordered by
+// stencil-base cell with an LDS counting sort (64 cells).  Layer L holds the
+// L-th particle of every cell; layers are processed in pairs, cut into rounds
+// of up to 64 particles (one per lane), so the particles of one layer in a
+// round sit in distinct cells.
 //   G2P  gathers the 27 node velocities from the tile's (TT+2)^3 window of
 //        grid velocity held in LDS;
-//   P2G  all particles of lane c's cell scatter to the same 27 nodes, so the
-//        lane accumulates their contributions in registers over the rounds
-//        and adds them to its wave's LDS window once per pass (plain
-//        read-modify-writes: the lanes of one instruction touch distinct
-//        nodes, and a wave's LDS instructions execute in order).  A particle
-//        that moved to another cell adds with LDS atomics.
+//   P2G  each wave owns two LDS windows, one per layer of the pair; lanes of
+//        one layer touch distinct nodes in each instruction, so they add with
+//        plain read-modify-writes (a wave's LDS instructions execute in
+//        order).  A particle that moved to another cell adds into a shared,
+//        one node wider window with LDS atomics.
 // G2P of step n and P2G of step n+1 are fused (the new state never leaves
 // registers); the new state is written round by round at consecutive
 // positions of the tile's output range, so stores and the next step's loads
@@ -28,20 +30,23 @@
 // leaves the P2G window (more than one cell per step) scatters with global
 // atomics, so correctness never depends on the binning.
 //
-// Grid.  Each tile accumulates P2G in an (TT+4)^3-node window (one node
-// wider than the stencils of its particles on every side, so a particle that
-// moved one cell out of the tile still lands in it) and stores the window in
-// a buffer indexed by its place in the active list.  The grid update runs
-// over 4^3 node blocks: each node gathers the windows of the 8 tiles that
-// cover it (plus the global accumulator where the rare out-of-window P2G
-// landed), computes v = p/m + dt g with sticky walls, and clears what it
+// Grid.  Each work item accumulates P2G in an (TT+4)^3-node window of its
+// tile (one node wider than the stencils of its particles on every side, so a
+// particle that moved one cell out of the tile still lands in it) and stores
+// the window in a buffer indexed by its place in the active list.  The grid
+// update runs over 4^3 node blocks: each node gathers the windows of every
+// work item of the 8 tiles that cover it (plus the global accumulator where
+// the rare out-of-window P2G landed), computes v = p/m + dt g with sticky walls, and clears what it
 // read.  There are no atomics on the regular path and no dense grid clear.
+#include <algorithm>
+
 #include "pg_tile.h"
 
 namespace pg {
 namespace {
 
-constexpr int PASS = 1280;       // particles staged per LDS pass of a tile (> most full tiles)
+constexpr int PASS = 1280;       // particles staged per LDS pass of a work item
+static_assert(CHUNK % PASS == 0, "a work item is a whole number of passes");
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 
 struct TileArgs {
@@ -208,14 +213,16 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* _
 // Tile kernel.  MODE bit 1: G2P (reads x, F; writes the new state);
 // MODE bit 2: P2G (scatters the current (MODE 2) or updated (MODE 3) state).
 //
-// Per pass of up to PASS particles of the tile: the particles' (x, F00) and
-// storage slots are loaded in bulk, counted by stencil-base cell (64 cells,
-// plus "outside" for clamped keys) and staged in LDS in cell order.  Then
-// rounds: in round r lane c of a wave takes the r-th particle of cell c, waves
-// take rounds in turn.  All lanes of a round sit in distinct cells, so the 27
-// node updates of a lane's P2G never meet another lane's in the same
-// instruction, and each wave accumulates into its own LDS window with plain
-// read-modify-writes in program order.  A particle that left its cell during
+// Per pass of up to PASS particles of the work item: the particles' (x, F00)
+// and storage slots are loaded in bulk, counted by stencil-base cell (64
+// cells, plus "outside" for clamped keys) and staged in LDS in cell order.
+// Layer L = the L-th particle of every cell with more than L; output offsets
+// run layer by layer.  Rounds: each pair of layers is cut into groups of up to
+// 64 consecutive offsets, one particle per lane, and waves take groups in
+// turn.  The particles of one layer in a group sit in distinct cells, so their
+// 27 node updates never meet in one instruction; the two layers of the pair
+// accumulate into the wave's two LDS windows with plain read-modify-writes in
+// program order.  A particle that left its cell during
 // G2P adds into a shared, one node wider window with LDS atomics; one that
 // left even that (more than a cell per step) with global atomics.  Outputs are
 // written round by round, so stores and the next step's loads coalesce.  At
@@ -227,10 +234,11 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     constexpr int PT = PASS / TB;                 // particles per thread in the bulk phases
     static_assert(PASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
     __shared__ float4 sv[G2P ? WV : 1];          // G2P window: grid (v, m) on nodes [t0, t0 + WN)
-    __shared__ float4 pw[P2G ? NW : 1][WV];      // per-wave P2G windows (m, p) on nodes [t0, t0 + WN)
+    __shared__ float4 pw[P2G ? NW : 1][P2G ? 2 : 1][WV];  // per-wave P2G windows (m, p) on nodes [t0, t0 + WN),
+                                                        // one per layer parity
     __shared__ float4 mv[P2G ? PV : 1];          // P2G of moved particles on nodes [t0 - 1, t0 + PN - 1)
-    __shared__ float lx[3][PASS];                // x of the pass in cell order
     __shared__ uint32_t lslot[PASS];             // storage slot, cell order
+    __shared__ uint16_t ent2j[PASS];             // output offset -> cell-order index
     __shared__ uint32_t lqueue[(G2P && P2G) ? PASS / 32 : 1];  // bit oo: P2G outside both windows
     __shared__ uint16_t rs[PASS];                // first output offset of each round
     __shared__ uint16_t ldir[(G2P && P2G) ? PASS : 1];  // (direction << 11 | rank) of the new tile, or FAR
@@ -240,9 +248,6 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // this lane's cell in a round: cell index x + 4 z + 16 y
-    const int ccx = lane & 3, ccz = (lane >> 2) & 3, ccy = lane >> 4;
-    const int cnb = (ccz * WN + ccy) * WN + ccx;                          // its base node in the windows
 
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
@@ -265,7 +270,10 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
         if (P2G) {
             for (int l = tid; l < WV; l += TB) {
 #pragma unroll
-                for (int w = 0; w < NW; ++w) pw[w][l] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int w = 0; w < NW; ++w) {
+                    pw[w][0][l] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    pw[w][1][l] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
             for (int l = tid; l < PV; l += TB) mv[l] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
@@ -342,9 +350,6 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 const uint32_t r = tid + k * TB;
                 if (r < pc) {
                     const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
-                    lx[0][j] = k0[k].x;
-                    lx[1][j] = k0[k].y;
-                    lx[2][j] = k0[k].z;
                     lslot[j] = ks[k];
                 }
             }
@@ -388,12 +393,17 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             // shared wide window with LDS atomics when it moved to another cell
             // nearby, else queued for the global scatter (MODE 3) or added with
             // global atomics.
+            // ob = window index of the particle's base before its G2P (own_ok only);
+            // the second layer of a round's pair (offsets from e1 on) uses the wave's
+            // second window
             auto scatter = [&](const float* x, const float* F, const float* C, const float* v, float Eq, float mq,
-                               bool own_ok, uint32_t oo) {
+                               bool own_ok, uint32_t oo, int ob, uint32_t e1) {
                 Stencil st;
                 make_stencil(x, g.o, g.inv_dx, st);
                 const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
-                const bool own = own_ok && l0 == ccx && l1 == ccy && l2 == ccz;
+                const int nb = (l2 * WN + l1) * WN + l0;
+                const bool inside = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
+                const bool own = own_ok && inside && nb == ob;
                 const bool near = l0 >= -1 && l0 < TT + 1 && l1 >= -1 && l1 < TT + 1 && l2 >= -1 && l2 < TT + 1;
                 if (!near && G2P) {
                     atomicOr(&lqueue[oo >> 5], 1u << (oo & 31));   // scattered after the rounds
@@ -402,9 +412,9 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 P2GVal p;
                 p2g_prep(g, Eq, mq, F, C, v, st, p);
                 if (own) {
-                    float4* __restrict__ win = pw[wave];
+                    float4* __restrict__ win = pw[wave][oo >= e1 ? 1 : 0];
                     p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                        float4* t = win + cnb + (iz * WN + iy) * WN + ix;
+                        float4* t = win + nb + (iz * WN + iy) * WN + ix;
                         float4 u = *t;
                         u.x += c.x;
                         u.y += c.y;
@@ -422,23 +432,26 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             };
 
             // one particle: j = its cell-order index, oo = its output offset, own_ok = it
-            // is this lane's cell in a round (G2P from the window)
-            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok) {
+            // is in a round (its base is a cell of the tile: G2P from the window),
+            // e1 = first offset of the second layer of its round's pair
+            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok, uint32_t e1) {
                 float x[3], F[9], v[3], C[9];
-                x[0] = lx[0][j]; x[1] = lx[1][j]; x[2] = lx[2][j];
                 const uint32_t s = lslot[j];
                 const uint32_t o = beg + pb + oo;
-                // F from the state planes (Q0 was read by the bulk phase and is in cache)
+                // x, F from the state planes (Q0 was read by the bulk phase and is in cache)
                 const float4 q0 = a.Sin[s], q1 = a.Sin[a.cap + s], q2 = a.Sin[2 * a.cap + s];
+                x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
                 F[0] = q0.w;
+                int ob = -1;
                 float Eq = g.E, mq = g.p_mass;
                 if (a.E_in) Eq = a.E_in[s];
                 if (a.m_in) mq = a.m_in[s];
                 if (G2P) {
                     Stencil st;
                     make_stencil(x, g.o, g.inv_dx, st);
+                    ob = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
                     if (own_ok)
-                        g2p_sum([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; }, st,
+                        g2p_sum([&](int ix, int iy, int iz) { return sv[ob + (iz * WN + iy) * WN + ix]; }, st,
                                 g.inv_dx, v, C);
                     else
                         g2p_global(gvs, g, st, v, C);
@@ -468,23 +481,47 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     v[0] = q3.x; v[1] = q3.y; v[2] = q3.z; C[0] = q3.w;
                     C[1] = q4.x; C[2] = q4.y; C[3] = q4.z; C[4] = q4.w;
                     C[5] = q5.x; C[6] = q5.y; C[7] = q5.z; C[8] = q5.w;
+                    // no G2P: the base is still the staged cell
+                    Stencil st;
+                    make_stencil(x, g.o, g.inv_dx, st);
+                    ob = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
                 }
-                if (P2G) scatter(x, F, C, v, Eq, mq, own_ok, oo);
+                if (P2G) scatter(x, F, C, v, Eq, mq, own_ok, oo, ob, e1);
             };
 
-            // ---- rounds: lane = cell, round r = the r-th particle of every cell (the sort
-            // and the stencil compute the base identically, so it is the lane's cell)
+            // ---- layers: layer L holds the L-th particle of every cell with more than L,
+            // in cell order, at output offsets [rs[L], rs[L + 1]); ent2j inverts the
+            // (layer, cell) -> offset map
             const uint32_t maxcnt = s_max, tot = s_tot;
-            const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
-            for (uint32_t r = wave; r < maxcnt; r += NW) {
-                const bool act = r < mycnt;
-                const uint64_t am = __ballot(act);
-                const uint32_t oo = rs[r] + lanes_below(am);
-                if (act) particle(mystart + r, oo, true);
+            {
+                const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
+                for (uint32_t L = wave; L < maxcnt; L += NW) {
+                    const bool act = L < mycnt;
+                    const uint64_t am = __ballot(act);
+                    if (act) ent2j[rs[L] + lanes_below(am)] = (uint16_t)(mystart + L);
+                }
+            }
+            __syncthreads();
+            // ---- rounds: layers are taken in pairs and each pair is cut into groups of
+            // up to 64 consecutive offsets, one particle per lane; waves take groups in
+            // turn.  In a group, the particles of one layer sit in distinct cells (so
+            // their 27 node updates never meet in one instruction) and the two layers
+            // of the pair accumulate into the wave's two windows.  A cell's particles
+            // in the same group are thus in different windows, and a group is nearly
+            // full even where few cells are still non-empty.
+            auto rsx = [&](uint32_t L) -> uint32_t { return L < maxcnt ? (uint32_t)rs[L] : tot; };
+            uint32_t grp = 0;
+            for (uint32_t L = 0; L < maxcnt; L += 2) {
+                const uint32_t e1 = rsx(L + 1), e2 = rsx(L + 2);
+                for (uint32_t e0 = rs[L]; e0 < e2; e0 += 64, ++grp) {
+                    if ((int)(grp % NW) != wave) continue;
+                    const uint32_t e = e0 + lane;
+                    if (e < e2) particle(ent2j[e], e, true, e1);
+                }
             }
             // ---- particles whose stencil base is outside the tile (clamped keys)
             const uint32_t nout = lcnt[64], sout = lstart[64];
-            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, tot + t, false);
+            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, tot + t, false, 0u);
             __syncthreads();
             // ---- queued P2G of particles that left both windows: global atomics,
             // from the state just written
@@ -534,13 +571,15 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 if (px >= 1 && px <= WN && py >= 1 && py <= WN && pz >= 1 && pz <= WN) {
                     const int l6 = ((pz - 1) * WN + py - 1) * WN + px - 1;
 #pragma unroll
-                    for (int w = 0; w < NW; ++w) {
-                        const float4 u = pw[w][l6];
-                        m.x += u.x;
-                        m.y += u.y;
-                        m.z += u.z;
-                        m.w += u.w;
-                    }
+                    for (int w = 0; w < NW; ++w)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const float4 u = pw[w][h][l6];
+                            m.x += u.x;
+                            m.y += u.y;
+                            m.z += u.z;
+                            m.w += u.w;
+                        }
                 }
                 if (buffered) {
                     wb[l] = m;
@@ -570,12 +609,13 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 // Grid update over 4^3 node blocks (one wave per block, lane = node).  The
 // P2G window of tile T covers nodes [4T - 1, 4T + 7) per axis, so node block
 // B is covered by tiles B + {-1, 0, 1}^3, each node by 8 of them; the node's
-// (m, p) is the sum of those tiles' buffered windows plus the global
-// accumulator where the rare global-atomic P2G landed.  Writes
-// gv = (p/m + dt g, m) with sticky walls and clears the accumulator.  TAPE
-// (segment recompute for the adjoint): also stores the summed (m, p) in the
-// step's grid and flags its 8^3 block, both of which the reverse pass uses
-// and clears.
+// (m, p) is the sum of the buffered windows of every work item of those tiles
+// plus the global accumulator where the rare global-atomic P2G landed.
+// Writes gv = (p/m + dt g, m) with sticky walls and clears the accumulator.
+// TAPE (segment recompute for the adjoint): also stores the summed (m, p) in
+// the step's grid and flags its 8^3 block, both of which the reverse pass
+// uses and clears.  A block without flags returns after two loads, before the
+// scene lookup (a search over the scene table).
 template <bool TAPE>
 __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes, int nscenes, int slots,
                                                     const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
@@ -585,39 +625,48 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
                                                     uint32_t* __restrict__ flags8) {
     const int B = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (B >= slots) return;
+    // covered by a buffered window, or global-atomic P2G here?
+    const bool fb = nflag[B] != 0u, wb = wflag[B] != 0u;
+    if (!fb && !wb) return;
     const int lane = threadIdx.x & 63;
     const SceneInfo sc = scenes[scene_of(scenes, nscenes, (uint32_t)B, [](const SceneInfo& t) { return t.toff; })];
     const uint32_t lb = (uint32_t)B - sc.toff;
     const int bc[3] = {(int)compact3(lb), (int)compact3(lb >> 1), (int)compact3(lb >> 2)};
     const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
     if (bc[0] >= nt[0] || bc[1] >= nt[1] || bc[2] >= nt[2]) return;
-    auto slot_of = [&](int tx, int ty, int tz) -> uint32_t {
+    // first active-list entry of a tile and its number of entries (chunks)
+    auto slot_of = [&](int tx, int ty, int tz, uint32_t& ne) -> uint32_t {
+        ne = 0u;
         if (tx < 0 || ty < 0 || tz < 0 || tx >= nt[0] || ty >= nt[1] || tz >= nt[2]) return 0xFFFFFFFFu;
-        return tslot[sc.toff + morton_tile(tx, ty, tz)];
+        const uint32_t t = sc.toff + morton_tile(tx, ty, tz);
+        const uint32_t f = tslot[t];
+        if (f != 0xFFFFFFFFu) ne = tslot[slots + t];
+        return f;
     };
-    // covered by a buffered window, or global-atomic P2G here?
-    const bool fb = nflag[B] != 0u, wb = wflag[B] != 0u;
-    if (!fb && !wb) return;
     const MlsGeom g = sc.g;
     const int lc[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
     const int i = bc[0] * TT + lc[0], j = bc[1] * TT + lc[1], k = bc[2] * TT + lc[2];
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (wb) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        int dT[3], w[3];
+        for (int e = 0; e < 8; ++e) {
+            int dT[3], w[3];
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const int bit = (e >> ax) & 1;
-            dT[ax] = lc[ax] < 3 ? bit - 1 : bit;
-            w[ax] = lc[ax] + 1 - TT * dT[ax];
-        }
-        const uint32_t sl = slot_of(bc[0] + dT[0], bc[1] + dT[1], bc[2] + dT[2]);
-        if (sl < (uint32_t)wcap) {
-            const float4 u = wbuf[(size_t)sl * PV + (w[2] * PN + w[1]) * PN + w[0]];
-            q.x += u.x;
-            q.y += u.y;
-            q.z += u.z;
-            q.w += u.w;
+            for (int ax = 0; ax < 3; ++ax) {
+                const int bit = (e >> ax) & 1;
+                dT[ax] = lc[ax] < 3 ? bit - 1 : bit;
+                w[ax] = lc[ax] + 1 - TT * dT[ax];
+            }
+            uint32_t ne;
+            const uint32_t sl = slot_of(bc[0] + dT[0], bc[1] + dT[1], bc[2] + dT[2], ne);
+            const uint32_t nb = sl < (uint32_t)wcap ? min(ne, (uint32_t)wcap - sl) : 0u;
+            for (uint32_t c = 0; c < nb; ++c) {
+                const float4 u = wbuf[(size_t)(sl + c) * PV + (w[2] * PN + w[1]) * PN + w[0]];
+                q.x += u.x;
+                q.y += u.y;
+                q.z += u.z;
+                q.w += u.w;
+            }
         }
     }
     float4* __restrict__ mp = mp_all + sc.goff;
@@ -686,8 +735,10 @@ __global__ void k_init_keys(int n, const float4* __restrict__ S, const uint32_t*
 }
 
 // Scan of the tile counts in two kernels over blocks of SCAN_B slots:
-// block sums and active counts, then per-block exclusive scan with the
-// preceding blocks' totals; writes starts, the active tile list, clears counts.
+// block sums and active-list entry counts, then per-block exclusive scan with
+// the preceding blocks' totals; writes starts, the active list (one entry per
+// chunk of at most CHUNK particles of a tile), the tile -> entries map, and
+// clears the counts.
 constexpr int SCAN_T = 256;
 constexpr int SCAN_B = 1024;
 
@@ -716,7 +767,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ 
     for (int k = 0; k < 4; ++k) {
         const uint32_t v = base + k < slots ? count[base + k] : 0u;
         s += v;
-        act += v ? 1u : 0u;
+        act += (v + CHUNK - 1) / CHUNK;
     }
     uint32_t ts, ta;
     block_excl_scan(s, sh, &ts);
@@ -748,7 +799,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
     for (int k = 0; k < 4; ++k) {
         v[k] = base + k < slots ? count[base + k] : 0u;
         s += v[k];
-        act += v[k] ? 1u : 0u;
+        act += (v[k] + CHUNK - 1) / CHUNK;
     }
     uint32_t ts, ta;
     uint32_t run = block_excl_scan(s, sh, &ts) + pre_s;
@@ -757,10 +808,14 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
     for (int k = 0; k < 4; ++k) {
         if (base + k < slots) {
             start[base + k] = run;
+            const uint32_t ne = (v[k] + CHUNK - 1) / CHUNK;
             tslot[base + k] = v[k] ? pos : 0xFFFFFFFFu;
+            tslot[slots + base + k] = ne;
             if (v[k]) {
                 const int sc = scene_of(scenes, nscenes, (uint32_t)(base + k), [](const SceneInfo& t) { return t.toff; });
-                alist[pos++] = make_uint4((uint32_t)(base + k), run, v[k], (uint32_t)sc);
+                for (uint32_t c = 0; c < ne; ++c)
+                    alist[pos++] = make_uint4((uint32_t)(base + k), run + c * CHUNK, min((uint32_t)CHUNK, v[k] - c * CHUNK),
+                                              (uint32_t)sc);
             }
             run += v[k];
             count[base + k] = 0u;

@@ -113,6 +113,11 @@ struct SceneInfo {
     int pad;
 };
 
+// Work items of the tile kernels: the particles of a tile in chunks of at
+// most CHUNK, so that a dense tile is spread over several workgroups instead
+// of serialising the step on one.
+constexpr int CHUNK = 1280;
+
 struct TiledWork {
     KernelProf* prof;      // optional per-kernel timing
     const SceneInfo* scenes;  // device scene table
@@ -121,18 +126,19 @@ struct TiledWork {
     int total_blocks;      // total 8^3 grid blocks over all scenes
     uint32_t* count;       // [tile_slots]
     uint32_t* start;       // [tile_slots]
-    uint32_t* active;      // [tile_slots] active tile list
+    uint32_t* active;      // [4 max_entries] active list: (tile, start, count, scene) per chunk of a tile
     uint32_t* nactive;     // [1]
     uint32_t* key;         // [cap] tile id per storage slot
     uint32_t* rank;        // [cap] rank within tile
     uint32_t* perm;        // [cap] sorted position -> storage slot
     uint32_t* flags;       // [total_blocks] 8^3 block flags (unused by the forward step)
     uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
-    uint32_t* tslot;       // [tile_slots] index of the tile in the active list, or ~0
+    uint32_t* tslot;       // [2 tile_slots] first active-list entry of the tile (or ~0), then its entry count
     uint32_t* nflag;       // [tile_slots] 4^3 node blocks hit by out-of-window P2G (global atomics)
     uint32_t* wflag;       // [tile_slots] 4^3 node blocks covered by a buffered P2G window
-    float4* wbuf;          // [wcap][216] P2G window sums of the active tiles
-    int wcap;              // active tiles with a window buffer; later ones add with atomics
+    float4* wbuf;          // [wcap][512] P2G window sums by active-list entry
+    int wcap;              // entries with a window buffer; later ones add with atomics
+    int max_entries;       // capacity of the active list
     int nblocks_launch;    // persistent grid size of the tile kernels
 };
 
@@ -145,9 +151,9 @@ struct TapeStep {
     float* mass;        // per-particle mass in storage order (or NULL)
     uint32_t* perm;     // sorted position -> storage slot
     uint32_t* src;      // output position of this step -> storage slot of this step
-    uint32_t* alist;    // uint4 (tile, start, count, scene) per active tile
+    uint32_t* alist;    // uint4 (tile, start, count, scene) per chunk of an active tile
     uint32_t* nactive;
-    uint32_t* tslot;    // tile slot -> index in alist, or ~0
+    uint32_t* tslot;    // tile slot -> first entry in alist (or ~0), then the entry count
     float4* mp;         // P2G result (m, p) of the step; cleared by the reverse pass
     float4* gv;         // (v, m) after the grid update
     uint32_t* flags;    // 8^3 blocks holding the step's grid; cleared by the reverse pass

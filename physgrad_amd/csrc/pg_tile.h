@@ -74,47 +74,6 @@ struct XcdChunk {
     }
 };
 
-// Wave-level transpose-reduce scatter into an LDS node window.
-//
-// LDS float atomics cost ~3 clocks per lane on gfx950, so a per-contribution
-// ds_add (108 per particle in P2G) is the bottleneck of a naive LDS-staged
-// scatter.  Instead each lane writes the NCH-channel contributions of its
-// particle to the 9 nodes of one stencil slice (fixed first offset ii) into a
-// per-wave scratch row; then lane (q, c) = (node-in-slice, channel) walks the
-// 64 rows in order, summing runs of consecutive particles that share a
-// stencil base (particles are sorted by cell inside a tile, so runs are long)
-// and issuing one LDS atomic per run.  Rows of particles that do not scatter
-// into this window carry base = -1 and break runs without contributing.
-//   scr    this wave's scratch, 64 rows of 9*NCH floats, row p = lane p
-//   base   the lane's window index of its stencil base node, or -1
-//   win    channel-major window, win[c*WV + node]
-template <int NCH>
-__device__ __forceinline__ void wave_slice_reduce(const float* scr, int base, int ii, float* win) {
-    constexpr int NP = 9 * NCH;
-    const int lane = threadIdx.x & 63;
-    const bool pl = lane < NP;
-    const int q = lane / NCH, c = lane - (lane / NCH) * NCH;
-    const int off = ii + WN * (q / 3) + WN * WN * (q % 3);  // (ii, jj, kk) -> window offset
-    float acc = 0.f;
-    int cur = -1;
-    for (int p0 = 0; p0 < 64; p0 += 8) {
-        float v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = pl ? scr[(p0 + u) * NP + lane] : 0.f;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int cp = __builtin_amdgcn_readlane(base, p0 + u);
-            if (cp != cur) {
-                if (cur >= 0 && pl) atomicAdd(&win[c * WV + cur + off], acc);
-                acc = 0.f;
-                cur = cp;
-            }
-            acc += v[u];
-        }
-    }
-    if (cur >= 0 && pl) atomicAdd(&win[c * WV + cur + off], acc);
-}
-
 // number of set bits of a wave mask below this lane
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -18,7 +18,10 @@
                       seeded random box per scene, fixed-corotated or
                       neo-Hookean per scene, E log-uniform in [5e3,5e4],
                       nu 0.2..0.35, rho 1000, dt 1e-4, 500 steps, all advanced
-                      by the same launches.
+                      by the same launches.  The workload fixes neither box
+                      size nor density: each box holds 8 particles per cell
+                      (2 per axis, the usual MPM sampling) with aspect ratios
+                      in [0.7, 1.4], placed at random inside the walls.
 Particle volume V0 = box volume / N; mass = rho V0.
 """
 from __future__ import annotations
@@ -111,20 +114,23 @@ def rollout(n: int = 262144, seed: int = 3) -> Scene:
 
 
 def batch_scenes(nscenes: int = 64, seed: int = 4, min_n: int = 4096, max_n: int = 65536,
-                 sides=(32, 48, 64)) -> list:
+                 sides=(32, 48, 64), ppc: float = 8.0) -> list:
     """Independent scenes of a batch (see the module docstring)."""
     rng = np.random.default_rng(seed)
     out = []
     for s in range(nscenes):
         side = int(sides[rng.integers(len(sides))])
         n = int(np.exp(rng.uniform(np.log(min_n), np.log(max_n))))
-        size = 0.15 + 0.15 * rng.random(3)
+        dx = 1.0 / side
+        aspect = rng.uniform(0.7, 1.4, 3)
+        aspect /= np.prod(aspect) ** (1.0 / 3.0)
+        size = np.minimum(aspect * (n * dx ** 3 / ppc) ** (1.0 / 3.0), 0.7)
         lo = 0.15 + (0.7 - size) * rng.random(3)
         x = _uniform(rng, n, lo, lo + size)
         v = rng.normal(0.0, 0.5, (n, 3)).astype(np.float32)
         material = int(rng.integers(2))
         E = float(np.exp(rng.uniform(np.log(5e3), np.log(5e4))))
         nu = float(rng.uniform(0.2, 0.35))
-        out.append(Scene(f"batch_{s}_{side}", (side,) * 3, 1.0 / side, 1e-4, 500, material, E, nu, 1000.0,
+        out.append(Scene(f"batch_{s}_{side}", (side,) * 3, dx, 1e-4, 500, material, E, nu, 1000.0,
                          float(np.prod(size)) / n, x, v))
     return out

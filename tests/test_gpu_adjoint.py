@@ -72,6 +72,34 @@ def test_backward_vs_oracle_reverse(gpu, oracle_lib, material, nsteps, K, impl):
     ctx.close()
 
 
+def test_backward_dense_tile_chunks(gpu, oracle_lib):
+    """6,000 particles in one tile (several work items per tile in the forward
+    recompute and in every reverse kernel): 5 steps, checkpoint every 2, against
+    the FP64 oracle reverse."""
+    from physgrad_amd.solver import MPMContext
+    O = oracle_lib
+    rng = np.random.default_rng(91)
+    n, nsteps, K = 6000, 5, 2
+    sc = scene(rng, n, 0, span=0.2)
+    C = rng.normal(0, 0.3, (n, 9)).astype(np.float32)
+    F = (np.eye(3).reshape(1, 9) + rng.normal(0, 0.03, (n, 9))).astype(np.float32)
+    seeds = [rng.normal(size=(n, c)) for c in (3, 3, 9, 9)]
+    print("dense tile backward", n, nsteps, K, flush=True)
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
+    ctx.forward_checkpointed(nsteps, K)
+    got = ctx.backward(*[torch.as_tensor(s, dtype=torch.float32) for s in seeds])
+    torch.cuda.synchronize()
+    want, wE = oracle_reverse(O, sc, sc.x.astype(np.float64), sc.v.astype(np.float64), C.astype(np.float64),
+                              F.astype(np.float64), sc.E_particles.astype(np.float64), nsteps, seeds)
+    for key, w in zip(("x", "v", "C", "F"), want):
+        err = rel(got[key].cpu().numpy().reshape(-1), w.reshape(-1))
+        print(key, err)
+        assert err < 1e-3, (key, err)
+    assert rel(got["E"].cpu().numpy(), wE) < 1e-3
+    ctx.close()
+
+
 def test_backward_requires_forward(gpu):
     from physgrad_amd import PgError
     from physgrad_amd.solver import MPMContext

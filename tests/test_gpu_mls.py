@@ -78,6 +78,34 @@ def test_step_parity_vs_fp64_oracle(gpu, oracle_lib, impl, material):
         x, v, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
 
 
+def test_dense_tile_chunks_parity(gpu, oracle_lib):
+    """6,000 particles in one 4^3-cell tile: the tiled path splits the tile into
+    several work items (chunks of at most 1,280 particles), each with its own
+    P2G window; four steps against the FP64 oracle."""
+    from physgrad_amd.scenes import Scene
+    O = oracle_lib
+    rng = np.random.default_rng(77)
+    n = 6000
+    x = (0.3 + 0.2 * rng.random((n, 3))).astype(np.float32)
+    v = rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+    E = np.exp(rng.uniform(np.log(5e3), np.log(5e4), n)).astype(np.float32)
+    sc = Scene("dense", (16, 16, 16), 1 / 16, 2e-4, 4, 1, 2e4, 0.3, 1000.0, 0.2 ** 3 / n, x, v, E_particles=E)
+    prm = O.scene_params(sc)
+    x64, v64 = x.astype(np.float64), v.astype(np.float64)
+    C64 = np.zeros((n, 9))
+    F64 = np.tile(np.eye(3).reshape(1, 9), (n, 1))
+    for k in range(4):
+        print("dense tile step", k, flush=True)
+        want = O.mls_run(prm, x64, v64, C64, F64, sc.p_mass, E.astype(np.float64), 1, want_grid=True)
+        got = gpu_step(sc, 1, x64.astype(np.float32), v64.astype(np.float32), C64.astype(np.float32),
+                       F64.astype(np.float32), E)
+        for key in ("x", "v", "C", "F"):
+            err = rel(got[key].reshape(-1), want[key].reshape(-1))
+            assert err <= TOL_STEP, (key, k, err)
+        check_grid(got["grid_m"], got["grid_v"], want["grid_m"], want["grid_v"])
+        x64, v64, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
+
+
 def test_error_vs_fp64_after_n_steps(gpu, oracle_lib):
     """Single-box workload (64^3, 32,768 particles, fixed-corotated): the FP32
     tiled GPU trajectory against the FP64 oracle after 50 steps."""
