@@ -159,10 +159,11 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
     for (int c = 0; c < 3; ++c) axo[c] = ax[c] + af[c] * g.inv_dx;
 }
 
-// A1: tiles of binning n; the particles of a pass are counted by stencil-base
-// cell and processed in rounds (lane c of a wave takes the next particle of
-// cell c), exactly like the forward tile kernel, so the velocity adjoint of a
-// round scatters into the wave's LDS window with plain read-modify-writes.
+// A1: work items of binning n; the particles of a pass are counted by
+// stencil-base cell and processed in rounds over layer pairs, exactly like the
+// forward tile kernel (layer L = the L-th particle of every cell; a round takes
+// up to 64 consecutive particles of a pair of layers, and the two layers
+// scatter into the wave's two LDS windows with plain read-modify-writes).
 // The windows are summed into the tile's buffer; the reverse grid update
 // gathers them.  A particle whose stencil base lies outside its tile (a
 // clamped key) scatters with global atomics into agv.
@@ -170,15 +171,14 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
     constexpr int PT = APASS / TB;
     static_assert(APASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
     __shared__ float4 sv[WV];
-    __shared__ float4 pw[NW][WV];
-    __shared__ float lx[3][APASS];
+    __shared__ float4 pw[NW][2][WV];          // per-wave windows, one per layer parity
     __shared__ uint32_t lo[APASS];            // output position (the adjoint's index), cell order
+    __shared__ uint16_t rs[APASS];            // first offset of each layer
+    __shared__ uint16_t ent2j[APASS];         // layer-order offset -> cell-order index
     __shared__ uint32_t lcnt[65], lstart[65];
     __shared__ uint32_t s_max, s_tot;
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ccx = lane & 3, ccz = (lane >> 2) & 3, ccy = lane >> 4;
-    const int cnb = (ccz * WN + ccy) * WN + ccx;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
         const SceneInfo sc = a.scenes[te.w];
@@ -195,7 +195,10 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
             if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
             sv[tid] = u;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) pw[w][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int w = 0; w < NW; ++w) {
+                pw[w][0][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                pw[w][1][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
         for (uint32_t pb = 0; pb < cnt; pb += APASS) {
             const uint32_t pc = min((uint32_t)APASS, cnt - pb);
@@ -245,26 +248,43 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                 }
             }
             __syncthreads();
+            // layer offsets: layer L starts after sum_c min(count_c, L) particles
+            for (uint32_t r = tid; r < s_max; r += TB) {
+                uint32_t o = 0;
+#pragma unroll 16
+                for (int c = 0; c < 64; ++c) o += min(lcnt[c], r);
+                rs[r] = (uint16_t)o;
+            }
 #pragma unroll
             for (int k = 0; k < PT; ++k) {
                 const uint32_t r = tid + k * TB;
                 if (r < pc) {
                     const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
-                    lx[0][j] = k0[k].x;
-                    lx[1][j] = k0[k].y;
-                    lx[2][j] = k0[k].z;
                     lo[j] = beg + pb + r;
                 }
             }
             __syncthreads();
+            const uint32_t maxcnt = s_max, tot = s_tot;
+            {
+                const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
+                for (uint32_t L = wave; L < maxcnt; L += NW) {
+                    const bool act = L < mycnt;
+                    const uint64_t am = __ballot(act);
+                    if (act) ent2j[rs[L] + lanes_below(am)] = (uint16_t)(mystart + L);
+                }
+            }
+            __syncthreads();
 
-            auto particle = [&](uint32_t j, bool own) {
+            // own: a round particle (its base is a cell of the tile); second: it is in
+            // the second layer of its round's pair (the wave's second window)
+            auto particle = [&](uint32_t j, bool own, bool second) {
                 const uint32_t o = lo[j];
                 const uint32_t s = a.src[o];
-                float x[3] = {lx[0][j], lx[1][j], lx[2][j]};
+                float x[3];
                 float F[9], ax[3], aF[9], av[3], aC[9];
                 {
                     const float4 q0 = a.S[s], q1 = a.S[a.cap + s], q2 = a.S[2 * a.cap + s];
+                    x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
                     F[0] = q0.w; F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
                     F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
                 }
@@ -273,10 +293,11 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                 make_stencil(x, g.o, g.inv_dx, st);
                 float axo[3], aFo[9];
                 if (own) {
-                    float4* __restrict__ win = pw[wave];
-                    adj_g2p_particle<false>([&](int ix, int iy, int iz) { return sv[cnb + (iz * WN + iy) * WN + ix]; },
+                    float4* __restrict__ win = pw[wave][second ? 1 : 0];
+                    const int nb = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
+                    adj_g2p_particle<false>([&](int ix, int iy, int iz) { return sv[nb + (iz * WN + iy) * WN + ix]; },
                                      [&](int ix, int iy, int iz, float4 c) {
-                                         float4* t = win + cnb + (iz * WN + iy) * WN + ix;
+                                         float4* t = win + nb + (iz * WN + iy) * WN + ix;
                                          float4 u = *t;
                                          u.x += c.x;
                                          u.y += c.y;
@@ -306,25 +327,33 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                 a.part[a.cap + s] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
                 a.part[2 * a.cap + s] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
             };
-            const uint32_t maxcnt = s_max;
-            const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
-            for (uint32_t r = wave; r < maxcnt; r += NW)
-                if (r < mycnt) particle(mystart + r, true);
+            uint32_t grp = 0;
+            for (uint32_t L = 0; L < maxcnt; L += 2) {
+                const uint32_t e1 = L + 1 < maxcnt ? (uint32_t)rs[L + 1] : tot;
+                const uint32_t e2 = L + 2 < maxcnt ? (uint32_t)rs[L + 2] : tot;
+                for (uint32_t e0 = rs[L]; e0 < e2; e0 += 64, ++grp) {
+                    if ((int)(grp % NW) != wave) continue;
+                    const uint32_t e = e0 + lane;
+                    if (e < e2) particle(ent2j[e], true, e >= e1);
+                }
+            }
             const uint32_t nout = lcnt[64], sout = lstart[64];
-            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, false);
+            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, false, false);
             __syncthreads();
         }
         // the tile's velocity-adjoint window goes to its buffer (atomics once the buffers are used up)
         const bool buffered = (int)ti < a.wcap;
         float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
         if (tid < WV) {
-            float4 m = pw[0][tid];
+            float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-            for (int w = 1; w < NW; ++w) {
-                m.x += pw[w][tid].x;
-                m.y += pw[w][tid].y;
-                m.z += pw[w][tid].z;
-            }
+            for (int w = 0; w < NW; ++w)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    m.x += pw[w][h][tid].x;
+                    m.y += pw[w][h][tid].y;
+                    m.z += pw[w][h][tid].z;
+                }
             if (buffered) {
                 wb[tid] = m;
             } else if (m.x != 0.f || m.y != 0.f || m.z != 0.f) {
@@ -472,7 +501,7 @@ __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, const Mls
     dE = amu * emu + alam * elam;
 }
 
-__global__ void __launch_bounds__(TB) k_adj_p2g(AdjArgs a) {
+__global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per SIMD: latency-bound gather
     __shared__ float sp[4][WV];
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x;
