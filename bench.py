@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU port leg")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward and gradient legs")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient-accuracy leg")
     ap.add_argument("--bwd-steps", type=int, default=64)
     ap.add_argument("--bwd-ckpt", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
@@ -111,6 +112,13 @@ def batch_leg(steps):
     tuned = b.autotune(trial_steps=10, use_cache=False)
     b.set_particles(counts, x, v)
     t_eager, _ = time_steps(b, steps, 10)
+    # per-kernel times of the tuned launch (HIP events on the launch stream)
+    b.set_particles(counts, x, v)
+    b.set_profiling(True)
+    b.step(min(steps, 50))
+    torch.cuda.synchronize()
+    kt = b.kernel_times()
+    b.set_profiling(False)
     b.set_particles(counts, x, v)
     b.set_graph(True)
     t_graph, _ = time_steps(b, steps, 10)
@@ -119,6 +127,7 @@ def batch_leg(steps):
     return {"value": n * steps / t_graph, "unit": "particle-steps/s", "scenes": len(scenes), "particles": n,
             "steps": steps, "ms_per_step": 1e3 * t_graph / steps, "ms_per_step_eager": 1e3 * t_eager / steps,
             "tile_workgroups": tuned, "state_finite": ok,
+            "kernel_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
             "workload": "batch_64: 64 scenes, side 32/48/64, 4,096..65,536 particles each (seed 4), "
                         "fixed-corotated / neo-Hookean, dt 1e-4, hipGraph replay"}
 
@@ -255,7 +264,7 @@ def main():
     if not args.no_bwd and not args.no_plain:
         k = min(args.bwd_steps, args.bwd_ckpt)
         plain_fwd_bwd = fwd_bwd_leg(scene, k, k, _lib.IMPL_PLAIN)
-    grad = None if args.no_bwd else grad_leg()
+    grad = None if args.no_bwd or args.no_grad else grad_leg()
     batch = None if args.no_batch else batch_leg(args.batch_steps)
 
     cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline(scene, args.cpu_steps)
