@@ -209,6 +209,7 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
                    (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
                    (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, 2 * toff)) ||
+                   (rc = dalloc(c, &w.tscene, toff)) || (rc = dalloc(c, &w.bscene, boff)) ||
                    (rc = dalloc(c, &w.nflag, toff)) || (rc = dalloc(c, &w.wflag, toff)))))
         return rc;
     if (tiled) {
@@ -226,8 +227,15 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
         hipMemset(w.nactive, 0, 4);
         hipMemset(w.nflag, 0, toff * 4);
         hipMemset(w.wflag, 0, toff * 4);
-        hipMemset(w.tslot, 0xFF, toff * 4);
-        hipMemset(w.tslot + toff, 0, toff * 4);
+        hipMemset(w.tslot, 0, toff * 8);
+        // scene of every tile slot and grid block (the kernels look scenes up by these)
+        std::vector<uint32_t> ts(toff), bs(boff);
+        for (int s = 0; s < ns; ++s) {
+            std::fill(ts.begin() + S[s].toff, ts.begin() + S[s].toff + S[s].tslots, (uint32_t)s);
+            std::fill(bs.begin() + S[s].boff, bs.begin() + S[s].boff + S[s].nblk, (uint32_t)s);
+        }
+        hipMemcpy(w.tscene, ts.data(), toff * 4, hipMemcpyHostToDevice);
+        hipMemcpy(w.bscene, bs.data(), boff * 4, hipMemcpyHostToDevice);
     }
     hipError_t e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);
     return e == hipSuccess ? PG_OK : hip_fail(c, e, "scene table upload");

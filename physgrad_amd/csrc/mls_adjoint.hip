@@ -375,13 +375,14 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
 // tiles of binning n that cover it (the window of tile T spans nodes
 // [4T, 4T + 6)) plus the global accumulator.  Clears the accumulator, the
 // step's grid and flags.
-__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes, int nscenes, int slots,
+__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes,
+                                                  const uint32_t* __restrict__ bscene,
                                                   const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
                                                   int wcap, float4* __restrict__ agv_all, float4* __restrict__ apam_all,
                                                   float4* __restrict__ mp_all, uint32_t* __restrict__ flags) {
     const int b = blockIdx.x;
     if (flags[b] == 0u) return;
-    const SceneInfo sc = scenes[scene_of(scenes, nscenes, (uint32_t)b, [](const SceneInfo& s) { return s.boff; })];
+    const SceneInfo sc = scenes[bscene[b]];
     const MlsGeom g = sc.g;
     float4* __restrict__ agv = agv_all + sc.goff;
     float4* __restrict__ apam = apam_all + sc.goff;
@@ -406,9 +407,9 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
             }
             if (ok) {
                 // every work item (chunk) of tile T
-                const uint32_t t = sc.toff + morton_tile(T[0], T[1], T[2]);
-                const uint32_t sl = tslot[t];
-                const uint32_t nb = sl < (uint32_t)wcap ? min(tslot[slots + t], (uint32_t)wcap - sl) : 0u;
+                const uint2 ts = reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(T[0], T[1], T[2])];
+                const uint32_t sl = ts.x;
+                const uint32_t nb = sl < (uint32_t)wcap ? min(ts.y, (uint32_t)wcap - sl) : 0u;
                 for (uint32_t c = 0; c < nb; ++c) {
                     const float4 u = wbuf[(size_t)(sl + c) * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
                     av.x += u.x;
@@ -704,8 +705,7 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         }
         {
             ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, t.tslot, w.wbuf, w.wcap, agv, apam,
-                                             t.mp, t.flags);
+            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, w.wcap, agv, apam, t.mp, t.flags);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);

@@ -617,7 +617,8 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 // uses and clears.  A block without flags returns after two loads, before the
 // scene lookup (a search over the scene table).
 template <bool TAPE>
-__global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes, int nscenes, int slots,
+__global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes,
+                                                    const uint32_t* __restrict__ tscene, int slots,
                                                     const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
                                                     uint32_t* __restrict__ wflag,
                                                     const float4* __restrict__ wbuf, int wcap,
@@ -629,7 +630,7 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
     const bool fb = nflag[B] != 0u, wb = wflag[B] != 0u;
     if (!fb && !wb) return;
     const int lane = threadIdx.x & 63;
-    const SceneInfo sc = scenes[scene_of(scenes, nscenes, (uint32_t)B, [](const SceneInfo& t) { return t.toff; })];
+    const SceneInfo sc = scenes[tscene[B]];
     const uint32_t lb = (uint32_t)B - sc.toff;
     const int bc[3] = {(int)compact3(lb), (int)compact3(lb >> 1), (int)compact3(lb >> 2)};
     const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
@@ -638,10 +639,9 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
     auto slot_of = [&](int tx, int ty, int tz, uint32_t& ne) -> uint32_t {
         ne = 0u;
         if (tx < 0 || ty < 0 || tz < 0 || tx >= nt[0] || ty >= nt[1] || tz >= nt[2]) return 0xFFFFFFFFu;
-        const uint32_t t = sc.toff + morton_tile(tx, ty, tz);
-        const uint32_t f = tslot[t];
-        if (f != 0xFFFFFFFFu) ne = tslot[slots + t];
-        return f;
+        const uint2 f = reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(tx, ty, tz)];
+        ne = f.y;
+        return f.x;
     };
     const MlsGeom g = sc.g;
     const int lc[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
@@ -718,7 +718,10 @@ __global__ void k_init_keys(int n, const float4* __restrict__ S, const uint32_t*
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint64_t todo = __ballot(valid);
     uint32_t rk = 0;
-    while (todo) {
+    // up to four distinct keys per wave with one atomic each (tile-sorted
+    // storage); the lanes left after that (unsorted storage, e.g. the caller's
+    // order) take their ranks with one atomic each, all in flight at once
+    for (int it = 0; it < 4 && todo; ++it) {
         const int leader = __ffsll((unsigned long long)todo) - 1;
         const uint32_t kl = __builtin_amdgcn_readlane(k, leader);
         const uint64_t peers = __ballot(k == kl) & todo;
@@ -728,6 +731,7 @@ __global__ void k_init_keys(int n, const float4* __restrict__ S, const uint32_t*
         if (k == kl && valid) rk = b + (uint32_t)__popcll(peers & lt);
         todo &= ~peers;
     }
+    if ((todo >> lane) & 1ull) rk = atomicAdd(count + k, 1u);
     if (valid) {
         key[s] = k;
         rank[s] = rk;
@@ -781,8 +785,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ 
 __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
                                                    uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
-                                                   uint32_t* __restrict__ tslot, const SceneInfo* __restrict__ scenes,
-                                                   int nscenes) {
+                                                   uint2* __restrict__ tslot, const uint32_t* __restrict__ tscene) {
     __shared__ uint32_t sh[SCAN_T];
     // totals of the preceding blocks
     uint32_t ps = 0, pa = 0;
@@ -809,10 +812,9 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
         if (base + k < slots) {
             start[base + k] = run;
             const uint32_t ne = (v[k] + CHUNK - 1) / CHUNK;
-            tslot[base + k] = v[k] ? pos : 0xFFFFFFFFu;
-            tslot[slots + base + k] = ne;
+            tslot[base + k] = make_uint2(v[k] ? pos : 0xFFFFFFFFu, ne);
             if (v[k]) {
-                const int sc = scene_of(scenes, nscenes, (uint32_t)(base + k), [](const SceneInfo& t) { return t.toff; });
+                const uint32_t sc = tscene[base + k];
                 for (uint32_t c = 0; c < ne; ++c)
                     alist[pos++] = make_uint4((uint32_t)(base + k), run + c * CHUNK, min((uint32_t)CHUNK, v[k] - c * CHUNK),
                                               (uint32_t)sc);
@@ -837,7 +839,7 @@ hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, tslot, w.scenes, w.nscenes);
+                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
 }
@@ -864,7 +866,7 @@ template <bool TAPE>
 void grid_update(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* gv, uint32_t* flags8,
                  hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
-    k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.nscenes, w.tile_slots, tslot, w.nflag,
+    k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag,
                                                                 w.wflag, w.wbuf, w.wcap, mp, gv, flags8);
 }
 

@@ -133,7 +133,9 @@ struct TiledWork {
     uint32_t* perm;        // [cap] sorted position -> storage slot
     uint32_t* flags;       // [total_blocks] 8^3 block flags (unused by the forward step)
     uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
-    uint32_t* tslot;       // [2 tile_slots] first active-list entry of the tile (or ~0), then its entry count
+    uint32_t* tslot;       // [tile_slots] uint2 (first active-list entry of the tile or ~0, entry count)
+    uint32_t* tscene;      // [tile_slots] scene of each tile slot
+    uint32_t* bscene;      // [total_blocks] scene of each 8^3 grid block
     uint32_t* nflag;       // [tile_slots] 4^3 node blocks hit by out-of-window P2G (global atomics)
     uint32_t* wflag;       // [tile_slots] 4^3 node blocks covered by a buffered P2G window
     float4* wbuf;          // [wcap][512] P2G window sums by active-list entry
@@ -153,7 +155,7 @@ struct TapeStep {
     uint32_t* src;      // output position of this step -> storage slot of this step
     uint32_t* alist;    // uint4 (tile, start, count, scene) per chunk of an active tile
     uint32_t* nactive;
-    uint32_t* tslot;    // tile slot -> first entry in alist (or ~0), then the entry count
+    uint32_t* tslot;    // tile slot -> uint2 (first entry in alist or ~0, entry count)
     float4* mp;         // P2G result (m, p) of the step; cleared by the reverse pass
     float4* gv;         // (v, m) after the grid update
     uint32_t* flags;    // 8^3 blocks holding the step's grid; cleared by the reverse pass
