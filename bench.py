@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward and gradient legs")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient-accuracy leg")
     ap.add_argument("--bwd-steps", type=int, default=64)
-    ap.add_argument("--bwd-ckpt", type=int, default=16)
+    # all 64 steps on the segment tape (~11 GB at 1M particles): the backward
+    # reverses them without recomputing the forward
+    ap.add_argument("--bwd-ckpt", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--batch-steps", type=int, default=500)
     return ap.parse_args()

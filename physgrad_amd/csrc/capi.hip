@@ -47,6 +47,10 @@ struct pg_mpm_ctx {
     // differentiation: checkpoints every ck_every steps, a segment tape, adjoint buffers
     int ck_every = 0, ck_steps = 0, ck_cap = 0, tape_len = 0;
     long long version = 0, ck_version = -1;
+    // the tape holds the last segment of the last checkpointed forward (tiled)
+    int tape_seg = -1;
+    long long tape_version = -1;
+    int tape_filled = 0;   // steps of the tape holding unreversed grids
     std::vector<pg::TapeStep> ckpt;   // S, pid, E, mass used
     std::vector<pg::TapeStep> tape;
     std::vector<void*> ck_allocs, tape_allocs;
@@ -735,6 +739,8 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     free_all(c->tape_allocs);
     c->tape.assign(K + 1, pg::TapeStep{});
     c->tape_len = 0;
+    c->tape_filled = 0;   // the new tape's grids start cleared
+    c->tape_seg = -1;
     const size_t G = c->G;
     const pg::TiledWork& w = c->tw;
     const size_t nblk = (size_t)w.total_blocks;
@@ -789,6 +795,12 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
     hipError_t e = hipSuccess;
     const bool plain = c->p.impl == PG_IMPL_PLAIN;
     const pg::MlsGeom g = mls_geom(c);
+    // a tape filled by an earlier forward and never reversed: clear its grids
+    if (!plain && c->tape_filled > 0) {
+        for (int s = 0; s < c->tape_filled && e == hipSuccess; ++s) e = pg::tape_clear(c->tw, c->tape[s], st);
+        c->tape_filled = 0;
+    }
+    c->tape_seg = -1;
     for (int k = 0; k < ncp && e == hipSuccess; ++k) {
         pg::TapeStep& ck = c->ckpt[k];
         e = copy_planes(ck.S, c->S, c->cap, c->n, 6, st);
@@ -808,6 +820,30 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
         float* E2 = c->have_E ? c->E2 : nullptr;
         float* m = c->have_mass ? c->mass : nullptr;
         float* m2 = c->have_mass ? c->mass2 : nullptr;
+        if (k + 1 == ncp) {
+            // last segment: run it into the tape, so that the backward reverses it
+            // without recomputing (the tape holds K steps anyway)
+            pg::TapeStep* T = c->tape.data();
+            if (e == hipSuccess) e = copy_planes(T[0].S, c->S, c->cap, c->n, 6, st);
+            if (e == hipSuccess) e = copy_u32(T[0].pid, c->pid, c->n, st);
+            if (e == hipSuccess && c->have_E)
+                e = hipMemcpyAsync(T[0].E, c->E, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess && c->have_mass)
+                e = hipMemcpyAsync(T[0].mass, c->mass, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = pg::tiled_tape_forward(c->tw, c->n, c->cap, T, L, st);
+            c->tape_filled = L;
+            // the end state and the last grid become the context's
+            if (e == hipSuccess) e = copy_planes(c->S, T[L].S, c->cap, c->n, 6, st);
+            if (e == hipSuccess) e = copy_u32(c->pid, T[L].pid, c->n, st);
+            if (e == hipSuccess && c->have_E)
+                e = hipMemcpyAsync(c->E, T[L].E, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess && c->have_mass)
+                e = hipMemcpyAsync(c->mass, T[L].mass, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(c->gv, T[L - 1].gv, c->G * sizeof(float4), hipMemcpyDeviceToDevice, st);
+            c->tape_seg = k;
+            continue;
+        }
         if (e == hipSuccess)
             e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv, L, st);
         if (c->have_E) { c->E = E; c->E2 = E2; }
@@ -818,6 +854,7 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
     c->steps += nsteps;
     c->version++;
     c->ck_version = c->version;
+    c->tape_version = c->version;
     c->ck_steps = nsteps;
     c->ck_every = K;
     return PG_OK;
@@ -857,8 +894,8 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         const int L = std::min(K, c->ck_steps - k * K);
         pg::TapeStep* T = c->tape.data();
         const pg::TapeStep& ck = c->ckpt[k];
-        e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
         if (plain) {
+            e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
             // recompute the segment: tape[s].S is the state before step s, tape[s].mp/gv its grid
             for (int s = 0; s < L && e == hipSuccess; ++s) {
                 e = copy_planes(T[s + 1].S, T[s].S, c->cap, n, 6, st);
@@ -872,11 +909,18 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
             if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, nullptr, c->abar, c->ux, c->uv, c->uC, c->uF, st);
             continue;
         }
-        if (e == hipSuccess) e = copy_u32(T[0].pid, ck.pid, n, st);
-        if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(T[0].E, ck.E, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess && c->have_mass)
-            e = hipMemcpyAsync(T[0].mass, ck.mass, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = pg::tiled_tape_forward(c->tw, n, c->cap, T, L, st);
+        const bool taped = k == c->tape_seg && c->tape_version == c->version && c->tape_filled == L;
+        if (!taped) {   // recompute the segment into the tape
+            if (e == hipSuccess) e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
+            if (e == hipSuccess) e = copy_u32(T[0].pid, ck.pid, n, st);
+            if (e == hipSuccess && c->have_E)
+                e = hipMemcpyAsync(T[0].E, ck.E, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess && c->have_mass)
+                e = hipMemcpyAsync(T[0].mass, ck.mass, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = pg::tiled_tape_forward(c->tw, n, c->cap, T, L, st);
+        }
+        c->tape_seg = -1;
+        c->tape_filled = 0;   // the reverse pass clears the grids it reads
         if (e == hipSuccess) e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, st);
         if (e == hipSuccess)
             e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, st);

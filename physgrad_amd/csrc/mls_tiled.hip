@@ -925,6 +925,32 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
     return hipGetLastError();
 }
 
+namespace {
+
+// clears the grid (m, p) of the flagged 8^3 blocks of a tape step and the flags
+__global__ void __launch_bounds__(512) k_tape_clear(const SceneInfo* __restrict__ scenes,
+                                                    const uint32_t* __restrict__ bscene, uint32_t* __restrict__ flags,
+                                                    float4* __restrict__ mp_all) {
+    const int b = blockIdx.x;
+    if (flags[b] == 0u) return;
+    const SceneInfo sc = scenes[bscene[b]];
+    const int lb = b - (int)sc.boff;
+    const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
+    const int i = bx * GB + (int)(threadIdx.x & 7), j = by * GB + (int)((threadIdx.x >> 3) & 7),
+              k = bz * GB + (int)(threadIdx.x >> 6);
+    if (i < sc.g.nx && j < sc.g.ny && k < sc.g.nz)
+        mp_all[sc.goff + node_id(sc.g, i, j, k)] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    if (threadIdx.x == 0) flags[b] = 0u;
+}
+
+}  // namespace
+
+hipError_t tape_clear(const TiledWork& w, const TapeStep& t, hipStream_t st) {
+    if (w.total_blocks > 0) k_tape_clear<<<w.total_blocks, 512, 0, st>>>(w.scenes, w.bscene, t.flags, t.mp);
+    return hipGetLastError();
+}
+
 hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, int L, hipStream_t st) {
     if (L <= 0) return hipSuccess;
     TileArgs a;

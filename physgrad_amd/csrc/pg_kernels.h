@@ -162,7 +162,10 @@ struct TapeStep {
 };
 
 // Recompute L steps from tape[0].S (tape[0].pid/E/mass set), filling tape[0..L-1]
-// and tape[L].S/pid/E/mass (the end state in its storage order).
+// and tape[L].S/pid/E/mass (the end state in its storage order).  The steps'
+// grids (mp) and 8^3 flags must be clear; the reverse pass leaves them clear,
+// tape_clear clears a tape that was filled but not reversed.
+hipError_t tape_clear(const TiledWork& w, const TapeStep& t, hipStream_t st);
 hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
                               hipStream_t st);
 

@@ -100,6 +100,42 @@ def test_backward_dense_tile_chunks(gpu, oracle_lib):
     ctx.close()
 
 
+def test_checkpointed_forward_matches_steps_and_refills_tape(gpu):
+    """The last segment of a checkpointed forward runs into the segment tape
+    (so the backward does not recompute it): its end state and grid must equal
+    plain steps, and a second forward over a tape that was never reversed must
+    give the same gradients as a fresh context."""
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(5)
+    n = 2000
+    sc = scene(rng, n, 0)
+    seeds = [torch.as_tensor(rng.normal(size=(n, c)), dtype=torch.float32) for c in (3, 3)]
+    a = MPMContext(sc.params(impl=1))
+    b = MPMContext(sc.params(impl=1))
+    a.set_particles(sc.x, sc.v, E=sc.E_particles)
+    b.set_particles(sc.x, sc.v, E=sc.E_particles)
+    a.forward_checkpointed(7, 3)
+    b.step(7)
+    pa, pb = a.get_particles(), b.get_particles()
+    for key in ("x", "v", "C", "F"):
+        assert rel(pa[key].cpu().numpy(), pb[key].cpu().numpy()) < 1e-5, key
+    ga, gb = a.get_grid(), b.get_grid()
+    assert rel(ga["mass"].cpu().numpy(), gb["mass"].cpu().numpy()) < 1e-5
+    # a second forward without a backward in between, then the backward
+    a.set_particles(sc.x, sc.v, E=sc.E_particles)
+    a.forward_checkpointed(7, 3)
+    got = a.backward(*seeds)
+    c = MPMContext(sc.params(impl=1))
+    c.set_particles(sc.x, sc.v, E=sc.E_particles)
+    c.forward_checkpointed(7, 3)
+    want = c.backward(*seeds)
+    torch.cuda.synchronize()
+    for key in ("x", "v", "C", "F", "E"):
+        assert rel(got[key].cpu().numpy(), want[key].cpu().numpy()) < 1e-4, key
+    for ctx in (a, b, c):
+        ctx.close()
+
+
 def test_backward_requires_forward(gpu):
     from physgrad_amd import PgError
     from physgrad_amd.solver import MPMContext
