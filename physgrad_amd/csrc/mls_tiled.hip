@@ -617,14 +617,12 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 // uses and clears.  A block without flags returns after two loads, before the
 // scene lookup (a search over the scene table).
 template <bool TAPE>
-__global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes,
-                                                    const uint32_t* __restrict__ tscene, int slots,
-                                                    const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
-                                                    uint32_t* __restrict__ wflag,
-                                                    const float4* __restrict__ wbuf, int wcap,
-                                                    float4* __restrict__ mp_all, float4* __restrict__ gv_all,
-                                                    uint32_t* __restrict__ flags8) {
-    const int B = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void grid_block(int B, const SceneInfo* __restrict__ scenes,
+                                           const uint32_t* __restrict__ tscene, int slots,
+                                           const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
+                                           uint32_t* __restrict__ wflag, const float4* __restrict__ wbuf, int wcap,
+                                           float4* __restrict__ mp_all, float4* __restrict__ gv_all,
+                                           uint32_t* __restrict__ flags8) {
     if (B >= slots) return;
     // covered by a buffered window, or global-atomic P2G here?
     const bool fb = nflag[B] != 0u, wb = wflag[B] != 0u;
@@ -699,6 +697,18 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
     }
 }
 
+template <bool TAPE>
+__global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes,
+                                                    const uint32_t* __restrict__ tscene, int slots,
+                                                    const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
+                                                    uint32_t* __restrict__ wflag,
+                                                    const float4* __restrict__ wbuf, int wcap,
+                                                    float4* __restrict__ mp_all, float4* __restrict__ gv_all,
+                                                    uint32_t* __restrict__ flags8) {
+    grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf, wcap,
+                     mp_all, gv_all, flags8);
+}
+
 // First binning of a call: tile key per storage slot, ranks from one global
 // atomic per distinct key in each wave (storage is normally tile-sorted
 // already, so a wave holds one or two keys).
@@ -762,10 +772,9 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, ui
     return incl - v;
 }
 
-__global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ count, int slots,
-                                                   uint32_t* __restrict__ bsum) {
-    __shared__ uint32_t sh[SCAN_T];
-    const int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
+__device__ __forceinline__ void scan_a_block(int blk, const uint32_t* __restrict__ count, int slots,
+                                             uint32_t* __restrict__ bsum, uint32_t* sh) {
+    const int base = blk * SCAN_B + threadIdx.x * 4;
     uint32_t s = 0, act = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -777,9 +786,37 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ 
     block_excl_scan(s, sh, &ts);
     block_excl_scan(act, sh, &ta);
     if (threadIdx.x == 0) {
-        bsum[2 * blockIdx.x] = ts;
-        bsum[2 * blockIdx.x + 1] = ta;
+        bsum[2 * blk] = ts;
+        bsum[2 * blk + 1] = ta;
     }
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ count, int slots,
+                                                   uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t sh[SCAN_T];
+    scan_a_block(blockIdx.x, count, slots, bsum, sh);
+}
+
+// The grid update of step n and the first scan pass over the tile counts of
+// binning n + 1 are independent: one launch runs both (workgroups [0, ngrid)
+// update the grid, the rest sum blocks of tile counts).
+template <bool TAPE>
+__global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restrict__ scenes,
+                                                        const uint32_t* __restrict__ tscene, int slots,
+                                                        const uint32_t* __restrict__ tslot,
+                                                        uint32_t* __restrict__ nflag, uint32_t* __restrict__ wflag,
+                                                        const float4* __restrict__ wbuf, int wcap,
+                                                        float4* __restrict__ mp_all, float4* __restrict__ gv_all,
+                                                        uint32_t* __restrict__ flags8, int ngrid,
+                                                        const uint32_t* __restrict__ count,
+                                                        uint32_t* __restrict__ bsum) {
+    static_assert(SCAN_T == 256, "both parts run 256 threads");
+    __shared__ uint32_t sh[SCAN_T];
+    if ((int)blockIdx.x < ngrid)
+        grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf,
+                         wcap, mp_all, gv_all, flags8);
+    else
+        scan_a_block(blockIdx.x - ngrid, count, slots, bsum, sh);
 }
 
 __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
@@ -833,6 +870,17 @@ __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_
     perm[start[key[s]] + rank[s]] = (uint32_t)s;
 }
 
+// binning after the first scan pass (k_scan_a or k_grid_and_count)
+hipError_t bin_rest(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32_t* nactive, uint32_t* tslot,
+                    hipStream_t st) {
+    ProfScope ps(w.prof, PK_BIN, st);
+    const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
+    k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
+                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
+    return hipGetLastError();
+}
+
 hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32_t* nactive, uint32_t* tslot,
                     hipStream_t st) {
     ProfScope ps(w.prof, PK_BIN, st);
@@ -868,6 +916,18 @@ void grid_update(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* 
     ProfScope ps(w.prof, PK_GRID, st);
     k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag,
                                                                 w.wflag, w.wbuf, w.wcap, mp, gv, flags8);
+}
+
+// grid update of the binning whose tile kernel filled the window buffers
+// (tslot), fused with the first scan pass of the next binning
+template <bool TAPE>
+void grid_update_and_count(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* gv, uint32_t* flags8,
+                           hipStream_t st) {
+    ProfScope ps(w.prof, PK_GRID, st);
+    const int ngrid = (w.tile_slots + 3) / 4;
+    const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
+    k_grid_and_count<TAPE><<<ngrid + nb, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag, w.wflag,
+                                                        w.wbuf, w.wcap, mp, gv, flags8, ngrid, w.count, w.bsum);
 }
 
 }  // namespace
@@ -911,8 +971,8 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
             ProfScope ps(w.prof, PK_TILE_G2P2G, st);
             k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
         }
-        grid_update<false>(w, w.tslot, mp, gv, nullptr, st);
-        e = bin(w, n, st);
+        grid_update_and_count<false>(w, w.tslot, mp, gv, nullptr, st);
+        e = bin_rest(w, n, w.perm, w.active, w.nactive, w.tslot, st);
         if (e != hipSuccess) return e;
         std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
     }
@@ -998,8 +1058,8 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
                 ProfScope ps(w.prof, PK_TILE_G2P2G, st);
                 k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
             }
-            grid_update<true>(w, cur.tslot, nxt.mp, nxt.gv, nxt.flags, st);
-            e = bin_into(w, n, nxt.perm, nxt.alist, nxt.nactive, nxt.tslot, st);
+            grid_update_and_count<true>(w, cur.tslot, nxt.mp, nxt.gv, nxt.flags, st);
+            e = bin_rest(w, n, nxt.perm, nxt.alist, nxt.nactive, nxt.tslot, st);
             if (e != hipSuccess) return e;
         } else {
             // last step of the segment: G2P only, to fix the end state's storage order
