@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU port leg")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward and gradient legs")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient-accuracy leg")
     ap.add_argument("--bwd-steps", type=int, default=64)
@@ -177,17 +177,25 @@ def grad_leg():
 
 
 def cpu_baseline(scene, steps):
-    """FP32 port of the same step (oracle), one host core, `steps` steps of the full workload."""
+    """FP32 port of the same step (oracle) on all host cores (OpenMP) for `steps`
+    steps of the full workload, the same port on one core for steps // 2 steps,
+    and the reference's own CPU solver for one step."""
     import numpy as np
     from oracle import oracle as O
     prm = O.scene_params(scene)
-    x = scene.x.copy()
+    threads = O.num_threads()
     t0 = time.perf_counter()
-    O.mls_run(prm, x, scene.v, None, None, scene.p_mass, scene.E, steps, dtype=np.float32)
+    O.mls_run_mt(prm, scene.x, scene.v, None, None, scene.p_mass, scene.E, steps)
     dt = time.perf_counter() - t0
-    out = {"value": scene.n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-           "sample": f"{steps} steps of the full workload ({scene.n} particles, 128^3) with the FP32 oracle "
-                     f"(same MLS-MPM step), single thread, {dt:.1f} s"}
+    out = {"value": scene.n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} steps of the full workload ({scene.n} particles, 128^3) with the FP32 port of the "
+                     f"same MLS-MPM step on {threads} OpenMP threads, {dt:.1f} s"}
+    s1 = max(1, steps // 2)
+    t0 = time.perf_counter()
+    O.mls_run(prm, scene.x.copy(), scene.v, None, None, scene.p_mass, scene.E, s1, dtype=np.float32)
+    dt1 = time.perf_counter() - t0
+    out["single_core"] = {"value": scene.n * s1 / dt1, "cores": 1,
+                          "sample": f"{s1} steps, the sequential FP32 port, {dt1:.1f} s"}
     ref = None
     if O.ref_available():
         # the reference's own CPU solver (PIC semantics, no stress) on the same particles and grid

@@ -81,6 +81,12 @@ void orc_mls_adjoint_step_f64(const orc_mls_params* p, int n,
                               const double* mass, const double* E,
                               double* ax, double* av, double* aC, double* aF, double* aE);
 
+/* FP32 MLS-MPM on all host cores (OpenMP; nthreads <= 0: the OpenMP default);
+ * the CPU baseline timed beside the GPU path */
+void orc_mls_run_f32_mt(const orc_mls_params* p, int n, float* x, float* v, float* C, float* F,
+                        const float* mass, const float* E, int nsteps, int nthreads);
+int orc_num_threads(void);
+
 /* ---------------- reference launchers (float4 particles) ---------------- */
 void orc_launch_p2g(const float* pos, const float* vel, float* gm, float* gp, const float* o, const float* h,
                     const int* d, int n, int order);

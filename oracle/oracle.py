@@ -67,6 +67,9 @@ def lib():
                                       _f64p, _f64p, _f64p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_mls_run_f32.argtypes = [ctypes.POINTER(MlsParams), ctypes.c_int, _f32p, _f32p, _f32p,
                                       _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_mls_run_f32_mt.argtypes = [ctypes.POINTER(MlsParams), ctypes.c_int, _f32p, _f32p, _f32p,
+                                         _f32p, _f32p, _f32p, ctypes.c_int, ctypes.c_int]
+        L.orc_num_threads.restype = ctypes.c_int
         L.orc_kirchhoff_f64.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _f64p, _f64p]
         L.orc_polar_f64.argtypes = [_f64p, _f64p, _f64p]
         L.orc_polar_f64.restype = ctypes.c_int
@@ -186,6 +189,21 @@ def mls_run(prm: MlsParams, x, v, C, F, mass, E, nsteps, dtype=np.float64, want_
         out["grid_m"] = gm
         out["grid_v"] = gv.reshape(-1, 3)
     return out
+
+
+def mls_run_mt(prm: MlsParams, x, v, C, F, mass, E, nsteps, nthreads: int = 0):
+    """FP32 MLS-MPM forward x nsteps on all host cores (OpenMP; nthreads 0 =
+    the OpenMP default).  Returns dict(x, v, C, F)."""
+    n, x, v, C, F = _state(x, v, C, F, np.float32)
+    mass = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (n,)), np.float32)
+    E = np.ascontiguousarray(np.broadcast_to(np.asarray(E, np.float32), (n,)), np.float32)
+    lib().orc_mls_run_f32_mt(ctypes.byref(prm), n, x, v, C, F, mass, E, int(nsteps), int(nthreads))
+    return dict(x=x.reshape(n, 3), v=v.reshape(n, 3), C=C.reshape(n, 3, 3), F=F.reshape(n, 3, 3))
+
+
+def num_threads() -> int:
+    """Threads the multi-threaded CPU port uses by default."""
+    return int(lib().orc_num_threads())
 
 
 def mls_adjoint_step(prm: MlsParams, x, v, C, F, mass, E, ax, av, aC, aF):
