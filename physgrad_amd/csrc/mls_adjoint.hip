@@ -503,7 +503,7 @@ __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, const Mls
 }
 
 __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per SIMD: latency-bound gather
-    __shared__ float sp[4][WV];
+    __shared__ float4 sp[WV];
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             int gi = t0[0] + l % WN, gj = t0[1] + (l / WN) % WN, gk = t0[2] + l / (WN * WN);
             float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gi < g.nx && gj < g.ny && gk < g.nz) u = apams[node_id(g, gi, gj, gk)];
-            sp[0][l] = u.x; sp[1][l] = u.y; sp[2][l] = u.z; sp[3][l] = u.w;
+            sp[l] = u;
         }
         __syncthreads();
         for (uint32_t r = tid; r < cnt; r += TB) {
@@ -535,8 +535,8 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             adj_p2g_particle(
                 [&](int ii, int jj, int kk, float& am, float* ap) {
                     if (inwin) {
-                        const int l = ((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii);
-                        am = sp[0][l]; ap[0] = sp[1][l]; ap[1] = sp[2][l]; ap[2] = sp[3][l];
+                        const float4 q = sp[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
+                        am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
                         return true;
                     }
                     const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
