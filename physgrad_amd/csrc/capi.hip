@@ -200,9 +200,14 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
     w.tile_slots = (int)toff;
     w.total_blocks = (int)boff;
     w.nblocks_launch = 2048;
-    // active-list entries: one per chunk of at most CHUNK particles of a tile
-    const uint64_t max_entries = toff + c->cap / pg::CHUNK + 1;
-    w.max_entries = (int)std::min<uint64_t>(max_entries, 1ull << 31);
+    // active-list entries: one per chunk of at most CHUNK particles of a non-empty
+    // tile.  A tile with c particles has ceil(c / CHUNK) = 1 + floor((c - 1) / CHUNK)
+    // entries, so a binning has at most min(tiles, particles) + particles / CHUNK.
+    uint64_t tiles = 0;
+    for (int s = 0; s < ns; ++s) tiles += (uint64_t)S[s].ntx * S[s].nty * S[s].ntz;
+    const uint64_t max_entries = std::min<uint64_t>(tiles, c->cap) + c->cap / pg::CHUNK + 1;
+    if (max_entries >= (1ull << 31)) return fail(c, PG_ERR_INVALID, "too many tiles and particles");
+    w.max_entries = (int)max_entries;
     w.prof = &c->prof;
     int rc;
     if ((rc = dalloc(c, &c->scenes_d, (size_t)ns)) || (rc = dalloc(c, &c->S, 6 * c->cap)) ||
@@ -217,10 +222,8 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.nflag, toff)) || (rc = dalloc(c, &w.wflag, toff)))))
         return rc;
     if (tiled) {
-        // P2G window buffers (8^3 nodes, 8 KiB each) for up to min(active-list
-        // capacity, particle capacity, 2^16) entries; later entries add with atomics
-        w.wcap = (int)std::min<uint64_t>(std::min<uint64_t>(max_entries, c->cap), 1u << 16);
-        if ((rc = dalloc(c, &w.wbuf, (size_t)w.wcap * 512))) return rc;
+        // P2G window buffers: 8^3 nodes (8 KiB) for every possible active-list entry
+        if ((rc = dalloc(c, &w.wbuf, (size_t)max_entries * 512))) return rc;
     }
     w.scenes = c->scenes_d;
     hipMemset(c->mp, 0, goff * sizeof(float4));

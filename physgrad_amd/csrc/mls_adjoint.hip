@@ -48,7 +48,6 @@ struct AdjArgs {
     const float4* apam;     // (am, ap) per node
     float* aE_user;         // dL/dE in user order (or NULL)
     float4* wbuf;           // per-tile window sums of the grid adjoint (by index in alist)
-    int wcap;
     const SceneInfo* scenes;
     size_t cap;
 };
@@ -341,8 +340,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
             for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, false, false);
             __syncthreads();
         }
-        // the tile's velocity-adjoint window goes to its buffer (atomics once the buffers are used up)
-        const bool buffered = (int)ti < a.wcap;
+        // the tile's velocity-adjoint window goes to its buffer (one per active-list entry)
         float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
         if (tid < WV) {
             float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -354,17 +352,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                     m.y += pw[w][h][tid].y;
                     m.z += pw[w][h][tid].z;
                 }
-            if (buffered) {
-                wb[tid] = m;
-            } else if (m.x != 0.f || m.y != 0.f || m.z != 0.f) {
-                const int gi = t0[0] + tid % WN, gj = t0[1] + (tid / WN) % WN, gk = t0[2] + tid / (WN * WN);
-                if (gi < g.nx && gj < g.ny && gk < g.nz) {
-                    float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
-                    atomic_add_f32(cp + 0, m.x);
-                    atomic_add_f32(cp + 1, m.y);
-                    atomic_add_f32(cp + 2, m.z);
-                }
-            }
+            wb[tid] = m;
         }
         __syncthreads();
     }
@@ -378,7 +366,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
 __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes,
                                                   const uint32_t* __restrict__ bscene,
                                                   const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
-                                                  int wcap, float4* __restrict__ agv_all, float4* __restrict__ apam_all,
+                                                  float4* __restrict__ agv_all, float4* __restrict__ apam_all,
                                                   float4* __restrict__ mp_all, uint32_t* __restrict__ flags) {
     const int b = blockIdx.x;
     if (flags[b] == 0u) return;
@@ -408,10 +396,8 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
             if (ok) {
                 // every work item (chunk) of tile T
                 const uint2 ts = reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(T[0], T[1], T[2])];
-                const uint32_t sl = ts.x;
-                const uint32_t nb = sl < (uint32_t)wcap ? min(ts.y, (uint32_t)wcap - sl) : 0u;
-                for (uint32_t c = 0; c < nb; ++c) {
-                    const float4 u = wbuf[(size_t)(sl + c) * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
+                for (uint32_t c = 0; c < ts.y; ++c) {
+                    const float4 u = wbuf[(size_t)(ts.x + c) * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
                     av.x += u.x;
                     av.y += u.y;
                     av.z += u.z;
@@ -696,7 +682,6 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.abar = abar; a.part = part; a.agv = agv; a.apam = apam;
         a.aE_user = aE_user;
         a.wbuf = w.wbuf;
-        a.wcap = w.wcap;
         a.scenes = w.scenes;
         a.cap = cap;
         {
@@ -705,7 +690,7 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         }
         {
             ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, w.wcap, agv, apam, t.mp, t.flags);
+            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, agv, apam, t.mp, t.flags);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);

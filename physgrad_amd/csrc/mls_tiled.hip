@@ -70,7 +70,6 @@ struct TileArgs {
     uint32_t* nflag;         // 4^3 node blocks hit by global-atomic P2G (by tile slot)
     uint32_t* wflag;         // 4^3 node blocks covered by a buffered window (by tile slot)
     float4* wbuf;            // window sums by active-list index
-    int wcap;
     const SceneInfo* scenes;
     size_t cap;
 };
@@ -561,9 +560,8 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
         }
 
         if (P2G) {
-            // the tile's window sum goes to its buffer (atomics once the buffers are used up);
+            // the tile's window sum goes to its buffer (one per active-list entry);
             // node l of the wide window is node (l - (1,1,1)) of the per-wave windows
-            const bool buffered = (int)ti < a.wcap;
             float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
             for (int l = tid; l < PV; l += TB) {
                 const int px = l % PN, py = (l / PN) % PN, pz = l / (PN * PN);
@@ -581,21 +579,9 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                             m.w += u.w;
                         }
                 }
-                if (buffered) {
-                    wb[l] = m;
-                } else if (m.x != 0.f) {
-                    const int gi = t0[0] - 1 + px, gj = t0[1] - 1 + py, gk = t0[2] - 1 + pz;
-                    if (in_grid(g, gi, gj, gk)) {
-                        float* cp = reinterpret_cast<float*>(mps + node_id(g, gi, gj, gk));
-                        atomic_add_f32(cp + 0, m.x);
-                        atomic_add_f32(cp + 1, m.y);
-                        atomic_add_f32(cp + 2, m.z);
-                        atomic_add_f32(cp + 3, m.w);
-                        flag_node(a.nflag, sc, gi, gj, gk);
-                    }
-                }
+                wb[l] = m;
             }
-            if (buffered && tid < 27) {
+            if (tid < 27) {
                 // node blocks this window covers: the tile's and its 26 neighbours
                 const int bx = tc[0] + tid % 3 - 1, by = tc[1] + (tid / 3) % 3 - 1, bz = tc[2] + tid / 9 - 1;
                 if (bx >= 0 && by >= 0 && bz >= 0 && bx < sc.ntx && by < sc.nty && bz < sc.ntz)
@@ -620,7 +606,7 @@ template <bool TAPE>
 __device__ __forceinline__ void grid_block(int B, const SceneInfo* __restrict__ scenes,
                                            const uint32_t* __restrict__ tscene, int slots,
                                            const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
-                                           uint32_t* __restrict__ wflag, const float4* __restrict__ wbuf, int wcap,
+                                           uint32_t* __restrict__ wflag, const float4* __restrict__ wbuf,
                                            float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                            uint32_t* __restrict__ flags8) {
     if (B >= slots) return;
@@ -657,8 +643,7 @@ __device__ __forceinline__ void grid_block(int B, const SceneInfo* __restrict__ 
             }
             uint32_t ne;
             const uint32_t sl = slot_of(bc[0] + dT[0], bc[1] + dT[1], bc[2] + dT[2], ne);
-            const uint32_t nb = sl < (uint32_t)wcap ? min(ne, (uint32_t)wcap - sl) : 0u;
-            for (uint32_t c = 0; c < nb; ++c) {
+            for (uint32_t c = 0; c < ne; ++c) {
                 const float4 u = wbuf[(size_t)(sl + c) * PV + (w[2] * PN + w[1]) * PN + w[0]];
                 q.x += u.x;
                 q.y += u.y;
@@ -702,11 +687,11 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
                                                     const uint32_t* __restrict__ tscene, int slots,
                                                     const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
                                                     uint32_t* __restrict__ wflag,
-                                                    const float4* __restrict__ wbuf, int wcap,
+                                                    const float4* __restrict__ wbuf,
                                                     float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                     uint32_t* __restrict__ flags8) {
-    grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf, wcap,
-                     mp_all, gv_all, flags8);
+    grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf, mp_all,
+                     gv_all, flags8);
 }
 
 // First binning of a call: tile key per storage slot, ranks from one global
@@ -805,7 +790,7 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
                                                         const uint32_t* __restrict__ tscene, int slots,
                                                         const uint32_t* __restrict__ tslot,
                                                         uint32_t* __restrict__ nflag, uint32_t* __restrict__ wflag,
-                                                        const float4* __restrict__ wbuf, int wcap,
+                                                        const float4* __restrict__ wbuf,
                                                         float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                         uint32_t* __restrict__ flags8, int ngrid,
                                                         const uint32_t* __restrict__ count,
@@ -814,7 +799,7 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
     __shared__ uint32_t sh[SCAN_T];
     if ((int)blockIdx.x < ngrid)
         grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf,
-                         wcap, mp_all, gv_all, flags8);
+                         mp_all, gv_all, flags8);
     else
         scan_a_block(blockIdx.x - ngrid, count, slots, bsum, sh);
 }
@@ -904,7 +889,6 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.nflag = w.nflag;
     a.wflag = w.wflag;
     a.wbuf = w.wbuf;
-    a.wcap = w.wcap;
     a.scenes = w.scenes;
     a.cap = cap;
 }
@@ -915,7 +899,7 @@ void grid_update(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* 
                  hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
     k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag,
-                                                                w.wflag, w.wbuf, w.wcap, mp, gv, flags8);
+                                                                w.wflag, w.wbuf, mp, gv, flags8);
 }
 
 // grid update of the binning whose tile kernel filled the window buffers
@@ -927,7 +911,7 @@ void grid_update_and_count(const TiledWork& w, const uint32_t* tslot, float4* mp
     const int ngrid = (w.tile_slots + 3) / 4;
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_grid_and_count<TAPE><<<ngrid + nb, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag, w.wflag,
-                                                        w.wbuf, w.wcap, mp, gv, flags8, ngrid, w.count, w.bsum);
+                                                        w.wbuf, mp, gv, flags8, ngrid, w.count, w.bsum);
 }
 
 }  // namespace

@@ -138,8 +138,7 @@ struct TiledWork {
     uint32_t* bscene;      // [total_blocks] scene of each 8^3 grid block
     uint32_t* nflag;       // [tile_slots] 4^3 node blocks hit by out-of-window P2G (global atomics)
     uint32_t* wflag;       // [tile_slots] 4^3 node blocks covered by a buffered P2G window
-    float4* wbuf;          // [wcap][512] P2G window sums by active-list entry
-    int wcap;              // entries with a window buffer; later ones add with atomics
+    float4* wbuf;          // [max_entries][512] P2G window sums by active-list entry
     int max_entries;       // capacity of the active list
     int nblocks_launch;    // persistent grid size of the tile kernels
 };
