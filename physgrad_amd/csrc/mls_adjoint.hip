@@ -7,15 +7,17 @@
 // masks) and P2G with the elastic stress, differentiated by hand.  Works on a
 // segment tape written by tiled_tape_forward; for step n of the tape:
 //
-//   A1  k_adj_g2p   tiles of binning n, particles in the step's output (cell)
-//                   order: recompute v, C from the gv window in LDS, push the
-//                   adjoint of the new (v, C) onto the grid (wave transpose-
-//                   reduce into an LDS window, one global atomic per node and
-//                   channel), accumulate the position adjoint from the weights
-//                   and x_i - x_p; partial = (ax, aF) by storage slot.
-//   A2  k_adj_grid  flagged 8^3 blocks: ap = agv/m, am = -(agv.p)/m^2 (zero
-//                   on sticky or empty nodes); clears agv, the tape's
-//                   accumulator and flags.
+//   A1  k_adj_g2p   tiles of binning n, particles in the forward tile
+//                   kernel's groups (contiguous output offsets): recompute v, C
+//                   from the gv window in LDS, push the adjoint of the new
+//                   (v, C) into the wave's LDS windows by read-modify-write
+//                   (summed into the tile's window buffer), accumulate the
+//                   position adjoint from the weights and x_i - x_p;
+//                   partial = (ax, aF) by storage slot.
+//   A2  k_adj_grid  flagged 8^3 blocks: the node's velocity adjoint is the sum
+//                   of the window buffers that cover it plus agv; ap = agv/m,
+//                   am = -(agv.p)/m^2 (zero on sticky or empty nodes); clears
+//                   agv, the tape's accumulator and flags.
 //   A3  k_adj_p2g   tiles of binning n: gather (am, ap) from the LDS window,
 //                   adjoints of v, C, F (stress VJP incl. polar derivative),
 //                   x, and dL/dE (accumulated in user particle order).
@@ -158,23 +160,25 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
     for (int c = 0; c < 3; ++c) axo[c] = ax[c] + af[c] * g.inv_dx;
 }
 
-// A1: work items of binning n; the particles of a pass are counted by
-// stencil-base cell and processed in rounds over layer pairs, exactly like the
-// forward tile kernel (layer L = the L-th particle of every cell; a round takes
-// up to 64 consecutive particles of a pair of layers, and the two layers
-// scatter into the wave's two LDS windows with plain read-modify-writes).
-// The windows are summed into the tile's buffer; the reverse grid update
-// gathers them.  A particle whose stencil base lies outside its tile (a
-// clamped key) scatters with global atomics into agv.
+// A1: work items of binning n, in the forward tile kernel's groups.  The
+// forward wrote the in-tile particles of a pass group by group: group grp of
+// G = max(ceil(tot / 64), ceil(maxcount / 2)) holds ceil((tot - grp) / G)
+// particles from output offset grp (tot div G) + min(grp, tot mod G), and the
+// particles of a pass whose stencil base lies outside the tile (clamped keys)
+// after them.  Counting the pass by stencil-base cell gives tot and maxcount
+// again, so the reverse runs the same groups over contiguous output offsets.
+// A group holds at most two particles of a cell, in adjacent lanes; the upper
+// one scatters into the wave's second LDS window, so within a window the lanes
+// touch distinct nodes and add with plain read-modify-writes.  The windows are
+// summed into the tile's buffer; the reverse grid update gathers them.  An
+// outside particle scatters with global atomics into agv.
 __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
     constexpr int PT = APASS / TB;
     static_assert(APASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
     __shared__ float4 sv[WV];
-    __shared__ float4 pw[NW][2][WV];          // per-wave windows, one per layer parity
-    __shared__ uint32_t lo[APASS];            // output position (the adjoint's index), cell order
-    __shared__ uint16_t rs[APASS];            // first offset of each layer
-    __shared__ uint16_t ent2j[APASS];         // layer-order offset -> cell-order index
-    __shared__ uint32_t lcnt[65], lstart[65];
+    __shared__ float4 pw[NW][2][WV];          // per-wave windows: lower and upper particle of a cell
+    __shared__ uint8_t lcell[APASS];          // stencil-base cell by output offset (64: outside)
+    __shared__ uint32_t lcnt[65];
     __shared__ uint32_t s_max, s_tot;
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -203,7 +207,6 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
             const uint32_t pc = min((uint32_t)APASS, cnt - pb);
             if (tid < 65) lcnt[tid] = 0;
             __syncthreads();
-            uint32_t key[PT];
             float4 k0[PT];
 #pragma unroll
             for (int k = 0; k < PT; ++k) {
@@ -213,7 +216,6 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
 #pragma unroll
             for (int k = 0; k < PT; ++k) {
                 const uint32_t r = tid + k * TB;
-                key[k] = 0u;
                 if (r < pc) {
                     const float xx[3] = {k0[k].x, k0[k].y, k0[k].z};
                     int l3[3];
@@ -224,60 +226,31 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                         in = in && l3[d] >= 0 && l3[d] < TT;
                     }
                     const int cell = in ? l3[0] + TT * l3[2] + TT * TT * l3[1] : 64;
-                    key[k] = ((uint32_t)cell << 16) | atomicAdd(&lcnt[cell], 1u);
+                    atomicAdd(&lcnt[cell], 1u);
+                    lcell[r] = (uint8_t)cell;
                 }
             }
             __syncthreads();
             if (wave == 0) {
-                const uint32_t cl = lcnt[lane];
-                uint32_t inc = cl;
+                uint32_t sum = lcnt[lane], mx = lcnt[lane];
 #pragma unroll
-                for (int off = 1; off < 64; off <<= 1) {
-                    const uint32_t t = __shfl_up(inc, off);
-                    if (lane >= off) inc += t;
+                for (int off = 32; off > 0; off >>= 1) {
+                    sum += (uint32_t)__shfl_xor(sum, off);
+                    mx = max(mx, (uint32_t)__shfl_xor(mx, off));
                 }
-                lstart[lane] = inc - cl;
-                uint32_t mx = cl;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-                if (lane == 63) {
-                    lstart[64] = inc;
-                    s_tot = inc;
+                if (lane == 0) {
+                    s_tot = sum;
                     s_max = mx;
                 }
             }
             __syncthreads();
-            // layer offsets: layer L starts after sum_c min(count_c, L) particles
-            for (uint32_t r = tid; r < s_max; r += TB) {
-                uint32_t o = 0;
-#pragma unroll 16
-                for (int c = 0; c < 64; ++c) o += min(lcnt[c], r);
-                rs[r] = (uint16_t)o;
-            }
-#pragma unroll
-            for (int k = 0; k < PT; ++k) {
-                const uint32_t r = tid + k * TB;
-                if (r < pc) {
-                    const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
-                    lo[j] = beg + pb + r;
-                }
-            }
-            __syncthreads();
             const uint32_t maxcnt = s_max, tot = s_tot;
-            {
-                const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
-                for (uint32_t L = wave; L < maxcnt; L += NW) {
-                    const bool act = L < mycnt;
-                    const uint64_t am = __ballot(act);
-                    if (act) ent2j[rs[L] + lanes_below(am)] = (uint16_t)(mystart + L);
-                }
-            }
-            __syncthreads();
 
-            // own: a round particle (its base is a cell of the tile); second: it is in
-            // the second layer of its round's pair (the wave's second window)
-            auto particle = [&](uint32_t j, bool own, bool second) {
-                const uint32_t o = lo[j];
+            // r = output offset in the pass; own: a group particle (its base is a cell
+            // of the tile); second: the upper particle of its cell in the group (the
+            // wave's second window)
+            auto particle = [&](uint32_t r, bool own, bool second) {
+                const uint32_t o = beg + pb + r;
                 const uint32_t s = a.src[o];
                 float x[3];
                 float F[9], ax[3], aF[9], av[3], aC[9];
@@ -326,18 +299,16 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                 a.part[a.cap + s] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
                 a.part[2 * a.cap + s] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
             };
-            uint32_t grp = 0;
-            for (uint32_t L = 0; L < maxcnt; L += 2) {
-                const uint32_t e1 = L + 1 < maxcnt ? (uint32_t)rs[L + 1] : tot;
-                const uint32_t e2 = L + 2 < maxcnt ? (uint32_t)rs[L + 2] : tot;
-                for (uint32_t e0 = rs[L]; e0 < e2; e0 += 64, ++grp) {
-                    if ((int)(grp % NW) != wave) continue;
-                    const uint32_t e = e0 + lane;
-                    if (e < e2) particle(ent2j[e], true, e >= e1);
+            const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
+            if (G > 0) {
+                const uint32_t gq = tot / G, gr = tot % G;
+                for (uint32_t grp = wave; grp < G; grp += NW) {
+                    const uint32_t r = grp * gq + min(grp, gr) + (uint32_t)lane;
+                    if ((uint32_t)lane < gq + (grp < gr ? 1u : 0u))
+                        particle(r, true, lane > 0 && lcell[r - 1] == lcell[r]);
                 }
             }
-            const uint32_t nout = lcnt[64], sout = lstart[64];
-            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, false, false);
+            for (uint32_t t = tid; t < pc - tot; t += TB) particle(tot + t, false, false);
             __syncthreads();
         }
         // the tile's velocity-adjoint window goes to its buffer (one per active-list entry)

@@ -22,7 +22,7 @@
 //        one layer touch distinct nodes in each instruction, so they add with
 //        plain read-modify-writes (a wave's LDS instructions execute in
 //        order).  A particle that moved to another cell adds into a shared,
-//        one node wider window with LDS atomics.
+//        one node wider window with LDS atomics, after the rounds.
 // G2P of step n and P2G of step n+1 are fused (the new state never leaves
 // registers); the new state is written round by round at consecutive
 // positions of the tile's output range, so stores and the next step's loads
@@ -118,24 +118,35 @@ __device__ __forceinline__ void p2g_prep(const MlsGeom& g, float Eq, float mq, c
     }
 }
 
-// visits the 27 contributions: emit(x, y, z, float4(w m, w p))
+// visits the 27 contributions: emit(x, y, z, float4(w m, w p)).  The node
+// offsets a0 x + a1 y + a2 z are built by additions along each axis: products
+// by the constants 0 and 1 are not folded away in IEEE arithmetic.
 template <class Emit>
 __device__ __forceinline__ void p2g_visit(const Stencil& st, const P2GVal& p, Emit emit) {
+    float bz[3] = {p.q[0], p.q[1], p.q[2]};
 #pragma unroll
-    for (int z = 0; z < 3; ++z)
+    for (int z = 0; z < 3; ++z) {
+        float by[3] = {bz[0], bz[1], bz[2]};
 #pragma unroll
         for (int y = 0; y < 3; ++y) {
             const float wyz = st.w[1][y] * st.w[2][z];
-            float b[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) b[c] = p.q[c] + p.a1[c] * (float)y + p.a2[c] * (float)z;
+            float bx[3] = {by[0], by[1], by[2]};
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
                 const float w = wyz * st.w[0][x];
-                emit(x, y, z, make_float4(w * p.m, w * (b[0] + p.a0[0] * (float)x), w * (b[1] + p.a0[1] * (float)x),
-                                          w * (b[2] + p.a0[2] * (float)x)));
+                emit(x, y, z, make_float4(w * p.m, w * bx[0], w * bx[1], w * bx[2]));
+                if (x < 2)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bx[c] += p.a0[c];
             }
+            if (y < 2)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) by[c] += p.a1[c];
         }
+        if (z < 2)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bz[c] += p.a2[c];
+    }
 }
 
 // Rare paths, written as rolled loops so that they stay out of the register
@@ -215,17 +226,21 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* _
 // Per pass of up to PASS particles of the work item: the particles' (x, F00)
 // and storage slots are loaded in bulk, counted by stencil-base cell (64
 // cells, plus "outside" for clamped keys) and staged in LDS in cell order.
-// Layer L = the L-th particle of every cell with more than L; output offsets
-// run layer by layer.  Rounds: each pair of layers is cut into groups of up to
-// 64 consecutive offsets, one particle per lane, and waves take groups in
-// turn.  The particles of one layer in a group sit in distinct cells, so their
-// 27 node updates never meet in one instruction; the two layers of the pair
-// accumulate into the wave's two LDS windows with plain read-modify-writes in
-// program order.  A particle that left its cell during
-// G2P adds into a shared, one node wider window with LDS atomics; one that
-// left even that (more than a cell per step) with global atomics.  Outputs are
-// written round by round, so stores and the next step's loads coalesce.  At
-// the end of the tile the windows are summed into the tile's buffer.
+// Rounds: the N in-tile particles are dealt, in cell order, over
+// G = max(ceil(N / 64), ceil(maxcount / 2)) groups (cell-order index j goes to
+// group j mod G, lane j div G), one particle per lane, and waves take groups
+// in turn.  Consecutive particles of a cell go to consecutive groups, so a
+// group holds at most two particles of a cell: the one of rank < G in the
+// cell adds into the wave's first LDS window, the one of rank >= G into the
+// second.  Within a window the lanes of a group sit in distinct cells, so
+// their 27 node updates never meet in one instruction and add with plain
+// read-modify-writes in program order; groups are nearly full whatever the
+// spread of the cell counts.  A particle that left its cell during G2P is
+// queued; after the rounds the queue is compacted and its particles add into
+// a shared, one node wider window with LDS atomics, or, having left even that
+// (more than a cell per step), with global atomics.  Outputs are written
+// group by group, so stores coalesce.  At the end of the tile the windows are
+// summed into the tile's buffer.
 template <int MODE>
 __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     constexpr bool G2P = (MODE & 1) != 0;
@@ -233,17 +248,17 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     constexpr int PT = PASS / TB;                 // particles per thread in the bulk phases
     static_assert(PASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
     __shared__ float4 sv[G2P ? WV : 1];          // G2P window: grid (v, m) on nodes [t0, t0 + WN)
-    __shared__ float4 pw[P2G ? NW : 1][P2G ? 2 : 1][WV];  // per-wave P2G windows (m, p) on nodes [t0, t0 + WN),
-                                                        // one per layer parity
+    __shared__ float4 pw[P2G ? NW : 1][P2G ? 2 : 1][WV];  // per-wave P2G windows (m, p) on nodes [t0, t0 + WN):
+                                                        // rank in cell < G, >= G
     __shared__ float4 mv[P2G ? PV : 1];          // P2G of moved particles on nodes [t0 - 1, t0 + PN - 1)
     __shared__ uint32_t lslot[PASS];             // storage slot, cell order
-    __shared__ uint16_t ent2j[PASS];             // output offset -> cell-order index
-    __shared__ uint32_t lqueue[(G2P && P2G) ? PASS / 32 : 1];  // bit oo: P2G outside both windows
-    __shared__ uint16_t rs[PASS];                // first output offset of each round
+    __shared__ uint16_t lrank[PASS];             // rank in its cell, cell order (then the P2G queue)
+    __shared__ uint32_t lqueue[(G2P && P2G) ? PASS / 32 : 1];  // bit oo: P2G after the rounds (changed cell)
     __shared__ uint16_t ldir[(G2P && P2G) ? PASS : 1];  // (direction << 11 | rank) of the new tile, or FAR
     __shared__ uint32_t lcnt[65], lstart[65];
     __shared__ uint32_t dcnt[27], dbase[27];
-    __shared__ uint32_t s_max, s_tot;
+    __shared__ uint32_t s_max, s_tot, s_nq;
+    static_assert(PASS / 32 <= 64, "one wave compacts the P2G queue");
 
     const XcdChunk ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -336,13 +351,6 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 }
             }
             __syncthreads();
-            // ---- round offsets: round r starts after sum_c min(count_c, r) particles
-            for (uint32_t r = tid; r < s_max; r += TB) {
-                uint32_t o = 0;
-#pragma unroll 16
-                for (int c = 0; c < 64; ++c) o += min(lcnt[c], r);
-                rs[r] = (uint16_t)o;
-            }
             // ---- stage in cell order
 #pragma unroll
             for (int k = 0; k < PT; ++k) {
@@ -350,6 +358,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 if (r < pc) {
                     const uint32_t j = lstart[key[k] >> 16] + (key[k] & 0xFFFFu);
                     lslot[j] = ks[k];
+                    lrank[j] = (uint16_t)(key[k] & 0xFFFFu);
                 }
             }
             __syncthreads();
@@ -388,32 +397,31 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             };
 
             // P2G of a particle with its (new) state: into the wave's window by
-            // read-modify-write when its base is this lane's cell (own), into the
-            // shared wide window with LDS atomics when it moved to another cell
-            // nearby, else queued for the global scatter (MODE 3) or added with
-            // global atomics.
+            // read-modify-write when its base is this lane's cell (own).  Otherwise
+            // the fused kernel queues it for the pass after the rounds (so that the
+            // rounds never branch off for the few particles that changed cell), and
+            // the P2G-only kernel adds it into the shared wide window with LDS
+            // atomics when it is nearby, else with global atomics.
             // ob = window index of the particle's base before its G2P (own_ok only);
-            // the second layer of a round's pair (offsets from e1 on) uses the wave's
-            // second window
+            // win = which of the wave's two windows
             auto scatter = [&](const float* x, const float* F, const float* C, const float* v, float Eq, float mq,
-                               bool own_ok, uint32_t oo, int ob, uint32_t e1) {
+                               bool own_ok, uint32_t oo, int ob, int win) {
                 Stencil st;
                 make_stencil(x, g.o, g.inv_dx, st);
                 const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
                 const int nb = (l2 * WN + l1) * WN + l0;
                 const bool inside = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
                 const bool own = own_ok && inside && nb == ob;
-                const bool near = l0 >= -1 && l0 < TT + 1 && l1 >= -1 && l1 < TT + 1 && l2 >= -1 && l2 < TT + 1;
-                if (!near && G2P) {
-                    atomicOr(&lqueue[oo >> 5], 1u << (oo & 31));   // scattered after the rounds
+                if (G2P && !own) {
+                    atomicOr(&lqueue[oo >> 5], 1u << (oo & 31));
                     return;
                 }
                 P2GVal p;
                 p2g_prep(g, Eq, mq, F, C, v, st, p);
-                if (own) {
-                    float4* __restrict__ win = pw[wave][oo >= e1 ? 1 : 0];
+                if (G2P || own) {
+                    float4* __restrict__ pwin = pw[wave][win];
                     p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
-                        float4* t = win + nb + (iz * WN + iy) * WN + ix;
+                        float4* t = pwin + nb + (iz * WN + iy) * WN + ix;
                         float4 u = *t;
                         u.x += c.x;
                         u.y += c.y;
@@ -422,7 +430,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                         *t = u;
                         lds_order();
                     });
-                } else if (near) {
+                } else if (l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT) {
                     p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st, p);
                 } else {
                     p2g_global(mps, a.nflag, sc, g, st, p);
@@ -432,8 +440,8 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 
             // one particle: j = its cell-order index, oo = its output offset, own_ok = it
             // is in a round (its base is a cell of the tile: G2P from the window),
-            // e1 = first offset of the second layer of its round's pair
-            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok, uint32_t e1) {
+            // win = which of the wave's two P2G windows it adds into
+            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok, int win) {
                 float x[3], F[9], v[3], C[9];
                 const uint32_t s = lslot[j];
                 const uint32_t o = beg + pb + oo;
@@ -485,49 +493,47 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     make_stencil(x, g.o, g.inv_dx, st);
                     ob = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
                 }
-                if (P2G) scatter(x, F, C, v, Eq, mq, own_ok, oo, ob, e1);
+                if (P2G) scatter(x, F, C, v, Eq, mq, own_ok, oo, ob, win);
             };
 
-            // ---- layers: layer L holds the L-th particle of every cell with more than L,
-            // in cell order, at output offsets [rs[L], rs[L + 1]); ent2j inverts the
-            // (layer, cell) -> offset map
+            // ---- rounds: cell-order index j = grp + lane G of group grp (grp < G); the
+            // groups' output ranges follow each other (group grp holds ceil((tot - grp) / G)
+            // particles and starts at grp (tot div G) + min(grp, tot mod G))
             const uint32_t maxcnt = s_max, tot = s_tot;
-            {
-                const uint32_t mycnt = lcnt[lane], mystart = lstart[lane];
-                for (uint32_t L = wave; L < maxcnt; L += NW) {
-                    const bool act = L < mycnt;
-                    const uint64_t am = __ballot(act);
-                    if (act) ent2j[rs[L] + lanes_below(am)] = (uint16_t)(mystart + L);
-                }
-            }
-            __syncthreads();
-            // ---- rounds: layers are taken in pairs and each pair is cut into groups of
-            // up to 64 consecutive offsets, one particle per lane; waves take groups in
-            // turn.  In a group, the particles of one layer sit in distinct cells (so
-            // their 27 node updates never meet in one instruction) and the two layers
-            // of the pair accumulate into the wave's two windows.  A cell's particles
-            // in the same group are thus in different windows, and a group is nearly
-            // full even where few cells are still non-empty.
-            auto rsx = [&](uint32_t L) -> uint32_t { return L < maxcnt ? (uint32_t)rs[L] : tot; };
-            uint32_t grp = 0;
-            for (uint32_t L = 0; L < maxcnt; L += 2) {
-                const uint32_t e1 = rsx(L + 1), e2 = rsx(L + 2);
-                for (uint32_t e0 = rs[L]; e0 < e2; e0 += 64, ++grp) {
-                    if ((int)(grp % NW) != wave) continue;
-                    const uint32_t e = e0 + lane;
-                    if (e < e2) particle(ent2j[e], e, true, e1);
+            const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
+            if (G > 0) {
+                const uint32_t gq = tot / G, gr = tot % G;
+                for (uint32_t grp = wave; grp < G; grp += NW) {
+                    const uint32_t j = grp + lane * G;
+                    if (j < tot) particle(j, grp * gq + min(grp, gr) + lane, true, lrank[j] >= G ? 1 : 0);
                 }
             }
             // ---- particles whose stencil base is outside the tile (clamped keys)
             const uint32_t nout = lcnt[64], sout = lstart[64];
-            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, tot + t, false, 0u);
+            for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, tot + t, false, 0);
             __syncthreads();
-            // ---- queued P2G of particles that left both windows: global atomics,
-            // from the state just written
+            // ---- queued P2G of the particles that changed cell, from the state just
+            // written: the queue is compacted (into lrank, free after the rounds) so
+            // that each wave takes up to 64 of them at once; into the shared wide
+            // window with LDS atomics when nearby, else with global atomics
             if (P2G && G2P) {
-                for (uint32_t t = tid; t < pc; t += TB) {
-                    if (!(lqueue[t >> 5] & (1u << (t & 31)))) continue;
-                    const uint32_t o = beg + pb + t;
+                if (wave == 0) {
+                    const uint32_t bits = lane < (pc + 31) / 32 ? lqueue[lane] : 0u;
+                    const uint32_t nb = (uint32_t)__popc(bits);
+                    uint32_t inc = nb;
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const uint32_t t = __shfl_up(inc, off);
+                        if (lane >= off) inc += t;
+                    }
+                    uint32_t q = inc - nb;
+                    for (uint32_t r = bits; r; r &= r - 1u) lrank[q++] = (uint16_t)(lane * 32 + __ffs(r) - 1);
+                    if (lane == 63) s_nq = inc;
+                }
+                __syncthreads();
+                const uint32_t nq = s_nq;
+                for (uint32_t q = tid; q < nq; q += TB) {
+                    const uint32_t o = beg + pb + lrank[q];
                     float x[3], F[9], v[3], C[9];
                     load_state(a.Sout, a.cap, o, x, F, v, C);
                     float Eq = g.E, mq = g.p_mass;
@@ -535,9 +541,14 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     if (a.m_out && a.m_in) mq = a.m_out[o];
                     Stencil st;
                     make_stencil(x, g.o, g.inv_dx, st);
+                    const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
                     P2GVal p;
                     p2g_prep(g, Eq, mq, F, C, v, st, p);
-                    p2g_global(mps, a.nflag, sc, g, st, p);
+                    if (l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT)
+                        p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st,
+                                          p);
+                    else
+                        p2g_global(mps, a.nflag, sc, g, st, p);
                 }
             }
             // ---- ranks in the new tiles: one global atomic per (tile, direction)
