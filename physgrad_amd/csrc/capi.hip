@@ -62,6 +62,7 @@ struct pg_mpm_ctx {
     hipEvent_t gev[2] = {nullptr, nullptr};
     float4 *abar = nullptr, *part = nullptr, *agv = nullptr, *apam = nullptr;
     float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
+    float *aEs = nullptr, *aEp = nullptr;   // dL/dE by storage slot through a segment, and its carry
 };
 
 namespace {
@@ -768,7 +769,8 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     }
     if ((rc = valloc(c, A, &c->abar, 6 * c->cap, false)) || (rc = valloc(c, A, &c->part, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->agv, G, true)) || (rc = valloc(c, A, &c->apam, G, true)) ||
-        (rc = valloc(c, A, &c->aE, c->cap, false)) || (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
+        (rc = valloc(c, A, &c->aE, c->cap, false)) || (rc = valloc(c, A, &c->aEs, c->cap, false)) ||
+        (rc = valloc(c, A, &c->aEp, c->cap, false)) || (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uv, 3 * c->cap, false)) || (rc = valloc(c, A, &c->uC, 9 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uF, 9 * c->cap, false)))
         goto fail;
@@ -778,7 +780,7 @@ fail:
     c->tape.clear();
     free_all(c->tape_allocs);
     c->abar = c->part = c->agv = c->apam = nullptr;
-    c->aE = c->ux = c->uv = c->uC = c->uF = nullptr;
+    c->aE = c->aEs = c->aEp = c->ux = c->uv = c->uC = c->uF = nullptr;
     return rc;
 }
 
@@ -891,6 +893,8 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     seed(c->aE, nullptr, (size_t)n);
     const bool plain = c->p.impl == PG_IMPL_PLAIN;
     const pg::MlsGeom g = mls_geom(c);
+    float* aE = gE ? c->aE : nullptr;      // dL/dE only when asked for
+    float* aEs = gE ? c->aEs : nullptr;
     const float* pE = c->have_E ? c->E : nullptr;
     const float* pm = c->have_mass ? c->mass : nullptr;
     for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
@@ -905,11 +909,14 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
                 if (e == hipSuccess)
                     e = pg::mls_plain_step(g, n, c->cap, T[s + 1].S, pm, pE, T[s].mp, T[s].gv, st, &c->prof);
             }
-            if (e == hipSuccess) e = pg::adjoint_from_user(n, c->cap, nullptr, c->ux, c->uv, c->uC, c->uF, c->abar, st);
             if (e == hipSuccess)
-                e = pg::adjoint_segment_plain(g, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, pE, pm,
-                                              st, &c->prof);
-            if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, nullptr, c->abar, c->ux, c->uv, c->uC, c->uF, st);
+                e = pg::adjoint_from_user(n, c->cap, nullptr, c->ux, c->uv, c->uC, c->uF, c->abar, nullptr, nullptr,
+                                          st);
+            if (e == hipSuccess)
+                e = pg::adjoint_segment_plain(g, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aE, pE, pm, st,
+                                              &c->prof);
+            if (e == hipSuccess)
+                e = pg::adjoint_to_user(n, c->cap, nullptr, c->abar, c->ux, c->uv, c->uC, c->uF, nullptr, nullptr, st);
             continue;
         }
         const bool taped = k == c->tape_seg && c->tape_version == c->version && c->tape_filled == L;
@@ -924,10 +931,12 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         }
         c->tape_seg = -1;
         c->tape_filled = 0;   // the reverse pass clears the grids it reads
-        if (e == hipSuccess) e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, st);
         if (e == hipSuccess)
-            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, c->aE, st);
-        if (e == hipSuccess) e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, st);
+            e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, aE, aEs, st);
+        if (e == hipSuccess)
+            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aEs, c->aEp, st);
+        if (e == hipSuccess)
+            e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);
     }
     auto out = [&](float* dst, const float* src, size_t count) {
         if (e == hipSuccess && dst) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDefault, st);

@@ -48,7 +48,8 @@ struct AdjArgs {
     float4* part;           // partial adjoint planes (3*cap) by slot of step n: (ax, aF)
     float4* agv;            // grid accumulator of the velocity adjoint (x,y,z used)
     const float4* apam;     // (am, ap) per node
-    float* aE_user;         // dL/dE in user order (or NULL)
+    float* aEs;             // dL/dE accumulated over the later steps, like abar (or NULL: not wanted)
+    float* aEp;             // aEs carried by A1 to the particle's slot of step n
     float4* wbuf;           // per-tile window sums of the grid adjoint (by index in alist)
     const SceneInfo* scenes;
     size_t cap;
@@ -56,6 +57,20 @@ struct AdjArgs {
 
 PG_DEV void load_adj(const float4* __restrict__ A, size_t cap, size_t i, float* ax, float* aF, float* av, float* aC) {
     load_state(A, cap, i, ax, aF, av, aC);
+}
+
+// x, v, C of a particle from the state planes, and F
+PG_DEV void load_xvC(const float4* __restrict__ S, size_t cap, size_t i, float* x, float* v, float* C) {
+    const float4 q0 = S[i], q3 = S[3 * cap + i], q4 = S[4 * cap + i], q5 = S[5 * cap + i];
+    x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
+    v[0] = q3.x; v[1] = q3.y; v[2] = q3.z; C[0] = q3.w;
+    C[1] = q4.x; C[2] = q4.y; C[3] = q4.z; C[4] = q4.w;
+    C[5] = q5.x; C[6] = q5.y; C[7] = q5.z; C[8] = q5.w;
+}
+PG_DEV void load_F(const float4* __restrict__ S, size_t cap, size_t i, float* F) {
+    const float4 q0 = S[i], q1 = S[cap + i], q2 = S[2 * cap + i];
+    F[0] = q0.w; F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
+    F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
 }
 
 PG_DEV void weight_grads(const Stencil& st, float dw[3][3]) {
@@ -298,6 +313,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                 a.part[s] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
                 a.part[a.cap + s] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
                 a.part[2 * a.cap + s] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
+                if (a.aEs) a.aEp[s] = a.aEs[o];
             };
             const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
             if (G > 0) {
@@ -393,54 +409,134 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
 }
 
 // A3 for one particle: the reverse of its P2G (stress included) given the
-// grid adjoints of its stencil nodes (load(x, y, z, am, ap) -> false where the
-// node is outside the grid) and its partial adjoints (ax, aF) from A1
-// (read by part() after the node loop, so they are not live across it).
+// grid adjoints (am, ap) of its stencil nodes (load(x, y, z) -> float4, zero
+// outside the grid) and its partial adjoints (ax, aF) from A1 (read by part()
+// after the node loop, so they are not live across it).  The forward put
+// w_n (m, b_n) on node n with b_n = m v + A d_n, d_n = (n - f) dx; with
+// aw_n = m am_n + ap_n . b_n the adjoints are
+//   v: m sum w ap,  A: dx sum w ap (n - f)^T,
+//   x (times dx): sum aw_n dw_n/df - dx A^T sum w ap.
+// The unrolled form factors the sums along the axes (x innermost) like the
+// forward G2P; ROLLED is the small form for the rare particle whose stencil
+// is not in the tile's window.  getF(F) reads the particle's F; it is read
+// again after the node loop rather than kept live across it.
 // Returns the adjoints of x, F, v, C before the step and dL/dE.
-template <class Load, class Part>
-__device__ __forceinline__ void adj_p2g_particle(Load load, Part part, const MlsGeom& g, const Stencil& st,
-                                                 const float* F, const float* v, const float* C, float Eq,
+template <bool ROLLED, class Load, class Part, class GetF>
+__device__ __forceinline__ void adj_p2g_particle(Load load, Part part, GetF getF, const MlsGeom& g,
+                                                 const Stencil& st, const float* v, const float* C, float Eq,
                                                  float mq, float* ax, float* aF, float* avq, float* aC, float& dE) {
     const float s4 = 4.f * g.inv_dx * g.inv_dx;
     const float kscale = -g.dt * g.p_vol * s4;
     const float emu = 1.f / (2.f * (1.f + g.nu));
     const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
     const float mu = Eq * emu, lam = Eq * elam;
-    float tau[9], A[9];
-    kirchhoff(g.material, mu, lam, F, tau);
+    float A[9];
+    {
+        float F[9], tau[9];
+        getF(F);
+        kirchhoff(g.material, mu, lam, F, tau);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
+        for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
+    }
     float dw[3][3];
     weight_grads(st, dw);
-    float aA[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    float af[3] = {0.f, 0.f, 0.f};
+    // b_n = q + dx (A[:, 0] x + A[:, 1] y + A[:, 2] z)
+    float q[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) avq[c] = 0.f;
-    for (int ii = 0; ii < 3; ++ii)
-        for (int jj = 0; jj < 3; ++jj)
+    for (int c = 0; c < 3; ++c)
+        q[c] = mq * v[c] - g.dx * (A[3 * c] * st.f[0] + A[3 * c + 1] * st.f[1] + A[3 * c + 2] * st.f[2]);
+    // s0 = sum w ap, s1[c][e] = sum w ap[c] (n - f)[e], ga[a] = sum aw dw/df_a
+    float s0[3] = {0.f, 0.f, 0.f}, ga[3] = {0.f, 0.f, 0.f};
+    float s1[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (ROLLED) {
+#pragma unroll 1
+        for (int n = 0; n < 27; ++n) {
+            const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
+            const float4 u = load(ix, iy, iz);
+            const float wx = pick3(st.w[0], ix), wy = pick3(st.w[1], iy), wz = pick3(st.w[2], iz);
+            const float w = wx * wy * wz;
+            const float d[3] = {(float)ix - st.f[0], (float)iy - st.f[1], (float)iz - st.f[2]};
+            const float ap[3] = {u.y, u.z, u.w};
+            float aw = u.x * mq;
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) {
-                const float wx = st.w[0][ii], wy = st.w[1][jj], wz = st.w[2][kk];
-                const float w = wx * wy * wz;
-                const float d[3] = {((float)ii - st.f[0]) * g.dx, ((float)jj - st.f[1]) * g.dx,
-                                    ((float)kk - st.f[2]) * g.dx};
-                float am, ap[3];
-                if (!load(ii, jj, kk, am, ap)) continue;
-                float aw = am * mq;
+            for (int c = 0; c < 3; ++c) {
+                aw += ap[c] * (q[c] + g.dx * (A[3 * c] * (float)ix + A[3 * c + 1] * (float)iy +
+                                              A[3 * c + 2] * (float)iz));
+                s0[c] += w * ap[c];
+#pragma unroll
+                for (int e = 0; e < 3; ++e) s1[c][e] += w * ap[c] * d[e];
+            }
+            ga[0] += aw * pick3(dw[0], ix) * wy * wz;
+            ga[1] += aw * wx * pick3(dw[1], iy) * wz;
+            ga[2] += aw * wx * wy * pick3(dw[2], iz);
+        }
+    } else {
+        // rows (y, z) as a loop, nodes along x unrolled: bounds the live registers
+        auto wd = [&](int a, int n) { return pick3(st.w[a], n) * ((float)n - st.f[a]); };
+        auto dwf = [&](int a, int n) { return pick3(dw[a], n); };
+        float bz[3] = {q[0], q[1], q[2]};
+#pragma unroll 1
+        for (int z = 0; z < 3; ++z) {
+            float R[3] = {0.f, 0.f, 0.f}, Ry[3] = {0.f, 0.f, 0.f}, Rx[3] = {0.f, 0.f, 0.f};
+            float Rw = 0.f, R1 = 0.f, R0 = 0.f;
+            float by[3] = {bz[0], bz[1], bz[2]};
+#pragma unroll 1
+            for (int y = 0; y < 3; ++y) {
+                float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+                float Pw = 0.f, Pd = 0.f;
+                float bx[3] = {by[0], by[1], by[2]};
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const float4 u = load(x, y, z);
+                    const float ap[3] = {u.y, u.z, u.w};
+                    const float aw = u.x * mq + ap[0] * bx[0] + ap[1] * bx[1] + ap[2] * bx[2];
+                    const float wx = st.w[0][x], wdx = wx * ((float)x - st.f[0]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        P[c] += wx * ap[c];
+                        Q[c] += wdx * ap[c];
+                    }
+                    Pw += wx * aw;
+                    Pd += dw[0][x] * aw;
+                    if (x < 2)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) bx[c] += g.dx * A[3 * c];
+                }
+                const float wy = pick3(st.w[1], y), wdy = wd(1, y), dwy = dwf(1, y);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float contrib = mq * v[c] + A[3 * c] * d[0] + A[3 * c + 1] * d[1] + A[3 * c + 2] * d[2];
-                    aw += ap[c] * contrib;
-                    avq[c] += w * mq * ap[c];
-#pragma unroll
-                    for (int e = 0; e < 3; ++e) aA[3 * c + e] += w * ap[c] * d[e];
+                    R[c] += wy * P[c];
+                    Ry[c] += wdy * P[c];
+                    Rx[c] += wy * Q[c];
+                    by[c] += g.dx * A[3 * c + 1];
                 }
-#pragma unroll
-                for (int e = 0; e < 3; ++e) af[e] -= g.dx * w * (A[e] * ap[0] + A[3 + e] * ap[1] + A[6 + e] * ap[2]);
-                af[0] += aw * dw[0][ii] * wy * wz;
-                af[1] += aw * wx * dw[1][jj] * wz;
-                af[2] += aw * wx * wy * dw[2][kk];
+                Rw += wy * Pw;
+                R1 += dwy * Pw;
+                R0 += wy * Pd;
             }
+            const float wz = pick3(st.w[2], z), wdz = wd(2, z), dwz = dwf(2, z);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                s0[c] += wz * R[c];
+                s1[c][2] += wdz * R[c];
+                s1[c][1] += wz * Ry[c];
+                s1[c][0] += wz * Rx[c];
+                bz[c] += g.dx * A[3 * c + 2];
+            }
+            ga[2] += dwz * Rw;
+            ga[1] += wz * R1;
+            ga[0] += wz * R0;
+        }
+    }
+    float af[3], aA[9];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) af[e] = ga[e] - g.dx * (A[e] * s0[0] + A[3 + e] * s0[1] + A[6 + e] * s0[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        avq[c] = mq * s0[c];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) aA[3 * c + e] = g.dx * s1[c][e];
+    }
     float4 p0, p1, p2;
     part(p0, p1, p2);
     ax[0] = p0.x + af[0] * g.inv_dx;
@@ -455,6 +551,8 @@ __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, const Mls
         aC[c] = mq * aA[c];
     }
     float amu = 0.f, alam = 0.f;
+    float F[9];
+    getF(F);
     kirchhoff_vjp(g.material, mu, lam, F, at, aF, amu, alam);
     dE = amu * emu + alam * elam;
 }
@@ -480,8 +578,9 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
         __syncthreads();
         for (uint32_t r = tid; r < cnt; r += TB) {
             const uint32_t s = a.perm[beg + r];
-            float x[3], F[9], v[3], C[9];
-            load_state(a.S, a.cap, s, x, F, v, C);
+            float x[3], v[3], C[9];
+            load_xvC(a.S, a.cap, s, x, v, C);
+            auto getF = [&](float* F) { load_F(a.S, a.cap, s, F); };
             const float Eq = a.E ? a.E[s] : g.E;
             const float mq = a.mass ? a.mass[s] : g.p_mass;
             Stencil st;
@@ -489,25 +588,23 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
             const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
             float ax[3], aF[9], avq[3], aC[9], dE;
-            adj_p2g_particle(
-                [&](int ii, int jj, int kk, float& am, float* ap) {
-                    if (inwin) {
-                        const float4 q = sp[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)];
-                        am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
-                        return true;
-                    }
-                    const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-                    if (!in_grid(g, gi, gj, gk)) return false;
-                    const float4 q = apams[node_id(g, gi, gj, gk)];
-                    am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
-                    return true;
-                },
-                [&](float4& p0, float4& p1, float4& p2) {
-                    p0 = a.part[s]; p1 = a.part[a.cap + s]; p2 = a.part[2 * a.cap + s];
-                },
-                g, st, F, v, C, Eq, mq, ax, aF, avq, aC, dE);
+            auto part = [&](float4& p0, float4& p1, float4& p2) {
+                p0 = a.part[s]; p1 = a.part[a.cap + s]; p2 = a.part[2 * a.cap + s];
+            };
+            if (inwin)
+                adj_p2g_particle<false>(
+                    [&](int ii, int jj, int kk) { return sp[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)]; }, part,
+                    getF, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
+            else
+                adj_p2g_particle<true>(
+                    [&](int ii, int jj, int kk) {
+                        const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
+                        return in_grid(g, gi, gj, gk) ? apams[node_id(g, gi, gj, gk)]
+                                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+                    },
+                    part, getF, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
             store_state(a.abar, a.cap, s, ax, aF, avq, aC);
-            if (a.aE_user) a.aE_user[a.pid[s]] += dE;
+            if (a.aEs) a.aEs[s] = a.aEp[s] + dE;
         }
         __syncthreads();
     }
@@ -571,25 +668,22 @@ __global__ void __launch_bounds__(256) k_adj_p2g_plain(int n, size_t cap, MlsGeo
                                                        float4* __restrict__ abar, float* __restrict__ aE) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float x[3], F[9], v[3], C[9];
-    load_state(S, cap, i, x, F, v, C);
+    float x[3], v[3], C[9];
+    load_xvC(S, cap, i, x, v, C);
     const float Eq = E ? E[i] : g.E;
     const float mq = mass ? mass[i] : g.p_mass;
     Stencil st;
     make_stencil(x, g.o, g.inv_dx, st);
     float ax[3], aF[9], avq[3], aC[9], dE;
-    adj_p2g_particle(
-        [&](int ii, int jj, int kk, float& am, float* ap) {
+    adj_p2g_particle<false>(
+        [&](int ii, int jj, int kk) {
             const int gi = st.b[0] + ii, gj = st.b[1] + jj, gk = st.b[2] + kk;
-            if (!in_grid(g, gi, gj, gk)) return false;
-            const float4 q = apam[node_id(g, gi, gj, gk)];
-            am = q.x; ap[0] = q.y; ap[1] = q.z; ap[2] = q.w;
-            return true;
+            return in_grid(g, gi, gj, gk) ? apam[node_id(g, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
         },
         [&](float4& p0, float4& p1, float4& p2) {
             p0 = part[i]; p1 = part[cap + i]; p2 = part[2 * cap + i];
         },
-        g, st, F, v, C, Eq, mq, ax, aF, avq, aC, dE);
+        [&](float* F) { load_F(S, cap, i, F); }, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
     store_state(abar, cap, i, ax, aF, avq, aC);
     if (aE) aE[i] += dE;
 }
@@ -597,10 +691,11 @@ __global__ void __launch_bounds__(256) k_adj_p2g_plain(int n, size_t cap, MlsGeo
 // user order <-> storage order conversions of the adjoint planes
 __global__ void k_gather_user(int n, size_t cap, const uint32_t* __restrict__ pid, const float* __restrict__ ax,
                               const float* __restrict__ av, const float* __restrict__ aC, const float* __restrict__ aF,
-                              float4* __restrict__ abar) {
+                              float4* __restrict__ abar, const float* __restrict__ aE, float* __restrict__ aEs) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const size_t u = pid ? pid[s] : (uint32_t)s;
+    if (aEs) aEs[s] = aE[u];
     float x[3], v[3], C[9], F[9];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -617,10 +712,11 @@ __global__ void k_gather_user(int n, size_t cap, const uint32_t* __restrict__ pi
 
 __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ pid, const float4* __restrict__ abar,
                                float* __restrict__ ax, float* __restrict__ av, float* __restrict__ aC,
-                               float* __restrict__ aF) {
+                               float* __restrict__ aF, const float* __restrict__ aEs, float* __restrict__ aE) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const size_t u = pid ? pid[s] : (uint32_t)s;
+    if (aEs) aE[u] = aEs[s];
     float x[3], v[3], C[9], F[9];
     load_state(abar, cap, s, x, F, v, C);
 #pragma unroll
@@ -638,7 +734,7 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
 }  // namespace
 
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L, float4* abar,
-                           float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st) {
+                           float4* part, float4* agv, float4* apam, float* aEs, float* aEp, hipStream_t st) {
     (void)n;
     const int nblk = w.total_blocks;
     for (int s = L - 1; s >= 0; --s) {
@@ -651,7 +747,8 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.nactive = t.nactive;
         a.gv = t.gv;
         a.abar = abar; a.part = part; a.agv = agv; a.apam = apam;
-        a.aE_user = aE_user;
+        a.aEs = aEs;
+        a.aEp = aEp;
         a.wbuf = w.wbuf;
         a.scenes = w.scenes;
         a.cap = cap;
@@ -696,14 +793,15 @@ hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* 
 }
 
 hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float* ax, const float* av,
-                             const float* aC, const float* aF, float4* abar, hipStream_t st) {
-    if (n > 0) k_gather_user<<<(n + 255) / 256, 256, 0, st>>>(n, cap, pid, ax, av, aC, aF, abar);
+                             const float* aC, const float* aF, float4* abar, const float* aE, float* aEs,
+                             hipStream_t st) {
+    if (n > 0) k_gather_user<<<(n + 255) / 256, 256, 0, st>>>(n, cap, pid, ax, av, aC, aF, abar, aE, aEs);
     return hipGetLastError();
 }
 
 hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,
-                           float* aC, float* aF, hipStream_t st) {
-    if (n > 0) k_scatter_user<<<(n + 255) / 256, 256, 0, st>>>(n, cap, pid, abar, ax, av, aC, aF);
+                           float* aC, float* aF, const float* aEs, float* aE, hipStream_t st) {
+    if (n > 0) k_scatter_user<<<(n + 255) / 256, 256, 0, st>>>(n, cap, pid, abar, ax, av, aC, aF, aEs, aE);
     return hipGetLastError();
 }
 

@@ -169,8 +169,11 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
                               hipStream_t st);
 
 // Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
+// aEs: dL/dE by storage slot, carried through the steps like abar (NULL: not
+// wanted); aEp: its per-step carry buffer.
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
-                           float4* abar, float4* part, float4* agv, float4* apam, float* aE_user, hipStream_t st);
+                           float4* abar, float4* part, float4* agv, float4* apam, float* aEs, float* aEp,
+                           hipStream_t st);
 // The plain port of the reverse pass over a segment recomputed with
 // mls_plain_step: tape[s].S is the state before step s (user order),
 // tape[s].mp / .gv the dense grid of step s.
@@ -178,11 +181,12 @@ hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* 
                                  float4* part, float4* agv, float4* apam, float* aE_user, const float* E,
                                  const float* mass, hipStream_t st, KernelProf* prof);
 // user-order adjoint arrays (x,v [3n], C,F [9n]; NULL = 0) -> adjoint planes in storage order (pid;
-// NULL = identity)
+// NULL = identity), and dL/dE aE (user order) -> aEs (storage order) when aEs is not NULL
 hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float* ax, const float* av,
-                             const float* aC, const float* aF, float4* abar, hipStream_t st);
+                             const float* aC, const float* aF, float4* abar, const float* aE, float* aEs,
+                             hipStream_t st);
 hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,
-                           float* aC, float* aF, hipStream_t st);
+                           float* aC, float* aF, const float* aEs, float* aE, hipStream_t st);
 
 hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t* pid, hipStream_t st);
 // Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on
