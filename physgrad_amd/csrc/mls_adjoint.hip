@@ -98,10 +98,22 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
                                                  const float* F, const float* ax, const float* aF,
                                                  const float* av, const float* aC, float* axo, float* aFo) {
     const float s4 = 4.f * g.inv_dx * g.inv_dx;
-    float v[3], nc[9];
-    if (ROLLED) {
+    // adjoints of v' and C' (through x' = x + dt v' and F' = (I + dt C') F); they do
+    // not depend on C' itself
+    float anv[3], anc[9];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = 0.f;
+    for (int c = 0; c < 3; ++c) anv[c] = av[c] + g.dt * ax[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            anc[3 * i + j] = aC[3 * i + j] + g.dt * (aF[3 * i] * F[3 * j] + aF[3 * i + 1] * F[3 * j + 1] +
+                                                     aF[3 * i + 2] * F[3 * j + 2]);
+    float dw[3][3];
+    weight_grads(st, dw);
+    float nc[9], af[3] = {0.f, 0.f, 0.f};
+    if (ROLLED) {
+        // rare path: two rolled passes, stencil values picked by selects
 #pragma unroll
         for (int c = 0; c < 9; ++c) nc[c] = 0.f;
 #pragma unroll 1
@@ -116,61 +128,122 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
 #pragma unroll
                 for (int e = 0; e < 3; ++e) nc[3 * c + e] += 4.f * g.inv_dx * w * uu[c] * d[e];
         }
-    } else {
-        g2p_sum(load, st, g.inv_dx, v, nc);
-    }
-    float dw[3][3];
-    weight_grads(st, dw);
-    float anv[3], anc[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) anv[c] = av[c] + g.dt * ax[c];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            anc[3 * i + j] = aC[3 * i + j] + g.dt * (aF[3 * i] * F[3 * j] + aF[3 * i + 1] * F[3 * j + 1] +
-                                                     aF[3 * i + 2] * F[3 * j + 2]);
-            aFo[3 * i + j] = aF[3 * i + j] + g.dt * (nc[i] * aF[j] + nc[3 + i] * aF[3 + j] + nc[6 + i] * aF[6 + j]);
-        }
-    float af[3] = {0.f, 0.f, 0.f};
-    auto node = [&](int ix, int iy, int iz, float wx, float wy, float wz, float dwx, float dwy, float dwz) {
-                const float w = wx * wy * wz;
-                const float d[3] = {((float)ix - st.f[0]) * g.dx, ((float)iy - st.f[1]) * g.dx,
-                                    ((float)iz - st.f[2]) * g.dx};
-                const float4 u4 = load(ix, iy, iz);
-                const float u[3] = {u4.x, u4.y, u4.z};
-                float t[3], aw = 0.f;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    t[c] = anv[c] + s4 * (anc[3 * c] * d[0] + anc[3 * c + 1] * d[1] + anc[3 * c + 2] * d[2]);
-                    aw += u[c] * t[c];
-                }
-#pragma unroll
-                for (int e = 0; e < 3; ++e) af[e] -= g.dx * s4 * w * (anc[e] * u[0] + anc[3 + e] * u[1] + anc[6 + e] * u[2]);
-                af[0] += aw * dwx * wy * wz;
-                af[1] += aw * wx * dwy * wz;
-                af[2] += aw * wx * wy * dwz;
-                emit(ix, iy, iz, make_float4(w * t[0], w * t[1], w * t[2], 0.f));
-    };
-    if (ROLLED) {
-        // rare path: a rolled loop, stencil values picked by selects
 #pragma unroll 1
         for (int n = 0; n < 27; ++n) {
             const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
-            node(ix, iy, iz, pick3(st.w[0], ix), pick3(st.w[1], iy), pick3(st.w[2], iz), pick3(dw[0], ix),
-                 pick3(dw[1], iy), pick3(dw[2], iz));
+            const float wx = pick3(st.w[0], ix), wy = pick3(st.w[1], iy), wz = pick3(st.w[2], iz);
+            const float w = wx * wy * wz;
+            const float d[3] = {((float)ix - st.f[0]) * g.dx, ((float)iy - st.f[1]) * g.dx,
+                                ((float)iz - st.f[2]) * g.dx};
+            const float4 u4 = load(ix, iy, iz);
+            const float u[3] = {u4.x, u4.y, u4.z};
+            float t[3], aw = 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                t[c] = anv[c] + s4 * (anc[3 * c] * d[0] + anc[3 * c + 1] * d[1] + anc[3 * c + 2] * d[2]);
+                aw += u[c] * t[c];
+            }
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+                af[e] -= g.dx * s4 * w * (anc[e] * u[0] + anc[3 + e] * u[1] + anc[6 + e] * u[2]);
+            af[0] += aw * pick3(dw[0], ix) * wy * wz;
+            af[1] += aw * wx * pick3(dw[1], iy) * wz;
+            af[2] += aw * wx * wy * pick3(dw[2], iz);
+            emit(ix, iy, iz, make_float4(w * t[0], w * t[1], w * t[2], 0.f));
         }
     } else {
+        // one pass over the nodes (rows as a loop, x unrolled): the recomputed
+        // v' = sum w u and sum w u (n - f)^T, the emitted w_n t_n with
+        // t_n = anv + s4 anc d_n built by additions along the axes, and
+        // ga[a] = sum (u_n . t_n) dw_n/df_a
+        float t0[3], tdx[3], tdy[3], tdz[3];
 #pragma unroll
-        for (int iz = 0; iz < 3; ++iz) {
-#pragma unroll
-            for (int iy = 0; iy < 3; ++iy)
-#pragma unroll
-                for (int ix = 0; ix < 3; ++ix)
-                    node(ix, iy, iz, st.w[0][ix], st.w[1][iy], st.w[2][iz], dw[0][ix], dw[1][iy], dw[2][iz]);
-            __builtin_amdgcn_sched_barrier(0);   // one 9-node slice in flight: bounds the registers
+        for (int c = 0; c < 3; ++c) {
+            t0[c] = anv[c] - s4 * g.dx * (anc[3 * c] * st.f[0] + anc[3 * c + 1] * st.f[1] + anc[3 * c + 2] * st.f[2]);
+            tdx[c] = s4 * g.dx * anc[3 * c];
+            tdy[c] = s4 * g.dx * anc[3 * c + 1];
+            tdz[c] = s4 * g.dx * anc[3 * c + 2];
         }
+        float v[3] = {0.f, 0.f, 0.f}, cx[3] = {0.f, 0.f, 0.f}, cy[3] = {0.f, 0.f, 0.f}, cz[3] = {0.f, 0.f, 0.f};
+        float ga[3] = {0.f, 0.f, 0.f};
+        float tz[3] = {t0[0], t0[1], t0[2]};
+#pragma unroll 1
+        for (int z = 0; z < 3; ++z) {
+            float R[3] = {0.f, 0.f, 0.f}, Ry[3] = {0.f, 0.f, 0.f}, Rx[3] = {0.f, 0.f, 0.f};
+            float Rw = 0.f, R1 = 0.f, R0 = 0.f;
+            float ty[3] = {tz[0], tz[1], tz[2]};
+            const float wz = pick3(st.w[2], z);
+#pragma unroll 1
+            for (int y = 0; y < 3; ++y) {
+                float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f};
+                float Pw = 0.f, Pd = 0.f;
+                float tx[3] = {ty[0], ty[1], ty[2]};
+                const float wy = pick3(st.w[1], y), wyz = wy * wz;
+                // the row's three velocities are read before its three emissions, whose
+                // read-modify-writes are ordered after one another
+                const float4 ur[3] = {load(0, y, z), load(1, y, z), load(2, y, z)};
+                float4 er[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const float u[3] = {ur[x].x, ur[x].y, ur[x].z};
+                    const float wx = st.w[0][x], wdx = wx * ((float)x - st.f[0]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        P[c] += wx * u[c];
+                        Q[c] += wdx * u[c];
+                    }
+                    const float aw = u[0] * tx[0] + u[1] * tx[1] + u[2] * tx[2];
+                    Pw += wx * aw;
+                    Pd += dw[0][x] * aw;
+                    const float w = wx * wyz;
+                    er[x] = make_float4(w * tx[0], w * tx[1], w * tx[2], 0.f);
+                    if (x < 2)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) tx[c] += tdx[c];
+                }
+#pragma unroll
+                for (int x = 0; x < 3; ++x) emit(x, y, z, er[x]);
+                const float wdy = wy * ((float)y - st.f[1]), dwy = pick3(dw[1], y);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    R[c] += wy * P[c];
+                    Ry[c] += wdy * P[c];
+                    Rx[c] += wy * Q[c];
+                    ty[c] += tdy[c];
+                }
+                Rw += wy * Pw;
+                R1 += dwy * Pw;
+                R0 += wy * Pd;
+            }
+            const float wdz = wz * ((float)z - st.f[2]), dwz = pick3(dw[2], z);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[c] += wz * R[c];
+                cz[c] += wdz * R[c];
+                cy[c] += wz * Ry[c];
+                cx[c] += wz * Rx[c];
+                tz[c] += tdz[c];
+            }
+            ga[2] += dwz * Rw;
+            ga[1] += wz * R1;
+            ga[0] += wz * R0;
+        }
+        const float s = 4.f * g.inv_dx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nc[3 * c + 0] = s * cx[c];
+            nc[3 * c + 1] = s * cy[c];
+            nc[3 * c + 2] = s * cz[c];
+        }
+        // d_n = x_n - x_p also moves with x: - s4 dx anc^T sum w u
+#pragma unroll
+        for (int e = 0; e < 3; ++e) af[e] = ga[e] - g.dx * s4 * (anc[e] * v[0] + anc[3 + e] * v[1] + anc[6 + e] * v[2]);
     }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            aFo[3 * i + j] = aF[3 * i + j] + g.dt * (nc[i] * aF[j] + nc[3 + i] * aF[3 + j] + nc[6 + i] * aF[6 + j]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) axo[c] = ax[c] + af[c] * g.inv_dx;
 }
