@@ -100,6 +100,64 @@ def test_backward_dense_tile_chunks(gpu, oracle_lib):
     ctx.close()
 
 
+def test_backward_clustered_cells(gpu, oracle_lib):
+    """Three cells holding 300 particles each (groups formed from the largest
+    cell count in the forward tile kernel and in the reverse G2P): 3 steps,
+    checkpoint every 2, against the FP64 oracle reverse."""
+    from physgrad_amd.solver import MPMContext
+    from test_gpu_mls import clustered_positions
+    O = oracle_lib
+    rng = np.random.default_rng(17)
+    x = clustered_positions(rng)
+    n, nsteps, K = len(x), 3, 2
+    sc = scene(rng, n, 1)
+    sc.x = x
+    C = rng.normal(0, 0.3, (n, 9)).astype(np.float32)
+    F = (np.eye(3).reshape(1, 9) + rng.normal(0, 0.03, (n, 9))).astype(np.float32)
+    seeds = [rng.normal(size=(n, c)) for c in (3, 3, 9, 9)]
+    print("clustered backward", n, nsteps, K, flush=True)
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
+    ctx.forward_checkpointed(nsteps, K)
+    got = ctx.backward(*[torch.as_tensor(s, dtype=torch.float32) for s in seeds])
+    torch.cuda.synchronize()
+    want, wE = oracle_reverse(O, sc, sc.x.astype(np.float64), sc.v.astype(np.float64), C.astype(np.float64),
+                              F.astype(np.float64), sc.E_particles.astype(np.float64), nsteps, seeds)
+    for key, w in zip(("x", "v", "C", "F"), want):
+        err = rel(got[key].cpu().numpy().reshape(-1), w.reshape(-1))
+        print(key, err)
+        assert err < 1e-3, (key, err)
+    assert rel(got["E"].cpu().numpy(), wE) < 1e-3
+    ctx.close()
+
+
+def test_backward_without_E_gradient(gpu):
+    """pg_mpm_backward with no dL/dE output skips the E adjoint: the other
+    gradients are those of a call that asks for it."""
+    import ctypes
+    from physgrad_amd import _lib
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(23)
+    n = 1500
+    sc = scene(rng, n, 0)
+    ax = torch.as_tensor(rng.normal(size=(n, 3)), dtype=torch.float32, device="cuda")
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, E=sc.E_particles)
+    ctx.forward_checkpointed(5, 2)
+    want = ctx.backward(ax)
+    gx = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    gv = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    L = _lib.load()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = L.pg_mpm_backward(ctx._h, ctypes.c_void_p(ax.data_ptr()), None, None, None,
+                           ctypes.c_void_p(gx.data_ptr()), ctypes.c_void_p(gv.data_ptr()), None, None, None, stream)
+    assert rc == 0, L.pg_mpm_last_error(ctx._h)
+    torch.cuda.synchronize()
+    assert rel(gx.cpu().numpy(), want["x"].cpu().numpy()) < 1e-6
+    assert rel(gv.cpu().numpy(), want["v"].cpu().numpy()) < 1e-6
+    ctx.close()
+
+
 def test_checkpointed_forward_matches_steps_and_refills_tape(gpu):
     """The last segment of a checkpointed forward runs into the segment tape
     (so the backward does not recompute it): its end state and grid must equal

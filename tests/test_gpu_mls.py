@@ -106,6 +106,44 @@ def test_dense_tile_chunks_parity(gpu, oracle_lib):
         x64, v64, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
 
 
+def clustered_positions(rng, n_cluster=300, n_spread=1000, grid=16):
+    """Three cells of one tile holding n_cluster particles each, plus n_spread
+    particles spread over the middle of the domain: the tile kernels then form
+    their groups from the largest cell count (two particles of a cell per
+    group), not from the particle count."""
+    cells = np.array([[5, 5, 5], [6, 5, 5], [5, 6, 6]])
+    xc = np.concatenate([(c + 0.5 + 0.9 * (rng.random((n_cluster, 3)) - 0.5)) / grid for c in cells])
+    xs = 0.3 + 0.4 * rng.random((n_spread, 3))
+    return np.concatenate([xc, xs]).astype(np.float32)
+
+
+def test_clustered_cells_parity(gpu, oracle_lib):
+    """Three cells with 300 particles each among 1,000 spread ones: two steps
+    of the tiled path against the FP64 oracle, both materials."""
+    from physgrad_amd.scenes import Scene
+    O = oracle_lib
+    for material in (0, 1):
+        rng = np.random.default_rng(40 + material)
+        x = clustered_positions(rng)
+        n = len(x)
+        v = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+        sc = Scene("clustered", (16, 16, 16), 1 / 16, 1e-4, 2, material, 2e4, 0.3, 1000.0, 0.4 ** 3 / n, x, v)
+        prm = O.scene_params(sc)
+        x64, v64 = x.astype(np.float64), v.astype(np.float64)
+        C64 = rng.normal(0, 0.3, (n, 9))
+        F64 = np.eye(3).reshape(1, 9) + rng.normal(0, 0.02, (n, 9))
+        for k in range(2):
+            print("clustered material", material, "step", k, flush=True)
+            want = O.mls_run(prm, x64, v64, C64, F64, sc.p_mass, sc.E, 1, want_grid=True)
+            got = gpu_step(sc, 1, x64.astype(np.float32), v64.astype(np.float32), C64.astype(np.float32),
+                           F64.astype(np.float32), None)
+            for key in ("x", "v", "C", "F"):
+                err = rel(got[key].reshape(-1), want[key].reshape(-1))
+                assert err <= TOL_STEP, (key, material, k, err)
+            check_grid(got["grid_m"], got["grid_v"], want["grid_m"], want["grid_v"])
+            x64, v64, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
+
+
 def test_error_vs_fp64_after_n_steps(gpu, oracle_lib):
     """Single-box workload (64^3, 32,768 particles, fixed-corotated): the FP32
     tiled GPU trajectory against the FP64 oracle after 50 steps."""
