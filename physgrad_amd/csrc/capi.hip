@@ -63,6 +63,7 @@ struct pg_mpm_ctx {
     float4 *abar = nullptr, *part = nullptr, *agv = nullptr, *apam = nullptr;
     float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
     float *aEs = nullptr, *aEp = nullptr;   // dL/dE by storage slot through a segment, and its carry
+    uint32_t* adst = nullptr;               // storage slot -> output position of a reverse step
 };
 
 namespace {
@@ -770,7 +771,8 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     if ((rc = valloc(c, A, &c->abar, 6 * c->cap, false)) || (rc = valloc(c, A, &c->part, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->agv, G, true)) || (rc = valloc(c, A, &c->apam, G, true)) ||
         (rc = valloc(c, A, &c->aE, c->cap, false)) || (rc = valloc(c, A, &c->aEs, c->cap, false)) ||
-        (rc = valloc(c, A, &c->aEp, c->cap, false)) || (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
+        (rc = valloc(c, A, &c->aEp, c->cap, false)) || (rc = valloc(c, A, &c->adst, c->cap, false)) ||
+        (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uv, 3 * c->cap, false)) || (rc = valloc(c, A, &c->uC, 9 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uF, 9 * c->cap, false)))
         goto fail;
@@ -781,6 +783,7 @@ fail:
     free_all(c->tape_allocs);
     c->abar = c->part = c->agv = c->apam = nullptr;
     c->aE = c->aEs = c->aEp = c->ux = c->uv = c->uC = c->uF = nullptr;
+    c->adst = nullptr;
     return rc;
 }
 
@@ -934,7 +937,8 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         if (e == hipSuccess)
             e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, aE, aEs, st);
         if (e == hipSuccess)
-            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aEs, c->aEp, st);
+            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aEs, c->aEp, c->adst,
+                                    st);
         if (e == hipSuccess)
             e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);
     }

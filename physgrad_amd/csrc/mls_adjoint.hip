@@ -13,20 +13,22 @@
 //                   (v, C) into the wave's LDS windows by read-modify-write
 //                   (summed into the tile's window buffer), accumulate the
 //                   position adjoint from the weights and x_i - x_p;
-//                   partial = (ax, aF) by storage slot.
+//                   partial = (ax, aF) by output position (coalesced), and
+//                   dst = the storage slot -> output position map of the step.
 //   A2  k_adj_grid  flagged 8^3 blocks: the node's velocity adjoint is the sum
 //                   of the window buffers that cover it plus agv; ap = agv/m,
 //                   am = -(agv.p)/m^2 (zero on sticky or empty nodes); clears
 //                   agv, the tape's accumulator and flags.
 //   A3  k_adj_p2g   tiles of binning n: gather (am, ap) from the LDS window,
 //                   adjoints of v, C, F (stress VJP incl. polar derivative),
-//                   x, and dL/dE (accumulated in user particle order).
+//                   x, and dL/dE (carried by storage slot like the state
+//                   adjoint when the caller asks for it).
 //
 // Indexing: the adjoint of the state after step n is stored by storage slot
 // of step n+1, which is the output position o of step n (src_n[o] is the
 // particle's slot in step n); A3 writes the adjoint of the state before step n
 // at its slot of step n, which is what A1 of step n-1 reads by its output
-// position.
+// position.  A3 reads A1's partials at dst[slot].
 #include <algorithm>
 
 #include "pg_tile.h"
@@ -49,7 +51,8 @@ struct AdjArgs {
     float4* agv;            // grid accumulator of the velocity adjoint (x,y,z used)
     const float4* apam;     // (am, ap) per node
     float* aEs;             // dL/dE accumulated over the later steps, like abar (or NULL: not wanted)
-    float* aEp;             // aEs carried by A1 to the particle's slot of step n
+    float* aEp;             // aEs carried from A1 to A3, by output position of step n
+    uint32_t* dst;          // storage slot of step n -> output position of step n (A1 -> A3)
     float4* wbuf;           // per-tile window sums of the grid adjoint (by index in alist)
     const SceneInfo* scenes;
     size_t cap;
@@ -383,10 +386,12 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
                         },
                         g, st, F, ax, aF, av, aC, axo, aFo);
                 }
-                a.part[s] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
-                a.part[a.cap + s] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
-                a.part[2 * a.cap + s] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
-                if (a.aEs) a.aEp[s] = a.aEs[o];
+                // partials by output position (coalesced); A3 finds them through dst
+                a.part[o] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
+                a.part[a.cap + o] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
+                a.part[2 * a.cap + o] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
+                a.dst[s] = o;
+                if (a.aEs) a.aEp[o] = a.aEs[o];
             };
             const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
             if (G > 0) {
@@ -661,8 +666,9 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
             const bool inwin = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
             float ax[3], aF[9], avq[3], aC[9], dE;
+            const uint32_t q = a.dst[s];
             auto part = [&](float4& p0, float4& p1, float4& p2) {
-                p0 = a.part[s]; p1 = a.part[a.cap + s]; p2 = a.part[2 * a.cap + s];
+                p0 = a.part[q]; p1 = a.part[a.cap + q]; p2 = a.part[2 * a.cap + q];
             };
             if (inwin)
                 adj_p2g_particle<false>(
@@ -677,7 +683,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
                     },
                     part, getF, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
             store_state(a.abar, a.cap, s, ax, aF, avq, aC);
-            if (a.aEs) a.aEs[s] = a.aEp[s] + dE;
+            if (a.aEs) a.aEs[s] = a.aEp[q] + dE;
         }
         __syncthreads();
     }
@@ -807,7 +813,8 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
 }  // namespace
 
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L, float4* abar,
-                           float4* part, float4* agv, float4* apam, float* aEs, float* aEp, hipStream_t st) {
+                           float4* part, float4* agv, float4* apam, float* aEs, float* aEp, uint32_t* dst,
+                           hipStream_t st) {
     (void)n;
     const int nblk = w.total_blocks;
     for (int s = L - 1; s >= 0; --s) {
@@ -822,6 +829,7 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.abar = abar; a.part = part; a.agv = agv; a.apam = apam;
         a.aEs = aEs;
         a.aEp = aEp;
+        a.dst = dst;
         a.wbuf = w.wbuf;
         a.scenes = w.scenes;
         a.cap = cap;

@@ -170,10 +170,11 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
 
 // Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
 // aEs: dL/dE by storage slot, carried through the steps like abar (NULL: not
-// wanted); aEp: its per-step carry buffer.
+// wanted); aEp: its per-step carry buffer; part/aEp are indexed by output
+// position and dst (cap) maps a step's storage slots to its output positions.
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
                            float4* abar, float4* part, float4* agv, float4* apam, float* aEs, float* aEp,
-                           hipStream_t st);
+                           uint32_t* dst, hipStream_t st);
 // The plain port of the reverse pass over a segment recomputed with
 // mls_plain_step: tape[s].S is the state before step s (user order),
 // tape[s].mp / .gv the dense grid of step s.
