@@ -111,7 +111,8 @@ int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);
  * then reversed (G2P, grid update with sticky walls, P2G with elasticity).
  * Inputs: adjoints dL/d(final x, v, C, F) in the caller's particle order
  * ([3n],[3n],[9n],[9n]; NULL = zero).  Outputs: dL/d(initial x, v, C, F) and
- * dL/dE per particle ([n]); NULL outputs are skipped.  Arrays may be in host
+ * dL/dE per particle ([n]); NULL outputs are skipped (with gE NULL the E
+ * adjoint is not computed at all).  Arrays may be in host
  * or device memory; when any is host memory the call returns after the
  * outputs are written, otherwise it is asynchronous on the stream.  The
  * reference's computeGradients (src/differentiable_mpm.h:772-783) reverses an
@@ -125,7 +126,8 @@ int pg_mpm_backward(pg_mpm_ctx* ctx, const float* ax, const float* av, const flo
  * src/mpm_solver.h:236-284, and its MPMPerformanceMetrics :72-101).
  * pg_mpm_set_profiling resets the totals.  Kernel ids: 0 tiled P2G, 1 tiled
  * fused G2P+P2G, 2 tiled G2P, 3 grid update, 4 binning, 5 plain P2G, 6 plain
- * G2P, 7 grid clear, 8 ref-compat step. */
+ * G2P, 7 grid clear, 8 ref-compat step, 9 reverse G2P, 10 reverse grid
+ * update, 11 reverse P2G. */
 int pg_mpm_set_profiling(pg_mpm_ctx* ctx, int on);
 int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long* launches);
 
