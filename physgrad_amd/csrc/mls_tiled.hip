@@ -12,17 +12,16 @@
 // Transfers.  One workgroup processes one work item at a time: a chunk of at
 // most CHUNK particles of one tile (dense tiles have several).  Per pass of up
 // to PASS particles, x and the storage slots are loaded in bulk and ordered by
-// stencil-base cell with an LDS counting sort (64 cells).  Layer L holds the
-// L-th particle of every cell; layers are processed in pairs, cut into rounds
-// of up to 64 particles (one per lane), so the particles of one layer in a
-// round sit in distinct cells.
+// stencil-base cell with an LDS counting sort (64 cells) and dealt over
+// groups of up to 64 (one per lane) that hold at most two particles of a cell.
 //   G2P  gathers the 27 node velocities from the tile's (TT+2)^3 window of
 //        grid velocity held in LDS;
-//   P2G  each wave owns two LDS windows, one per layer of the pair; lanes of
-//        one layer touch distinct nodes in each instruction, so they add with
-//        plain read-modify-writes (a wave's LDS instructions execute in
-//        order).  A particle that moved to another cell adds into a shared,
-//        one node wider window with LDS atomics, after the rounds.
+//   P2G  each wave owns two LDS windows, one for each of the two particles a
+//        cell may have in a group; the lanes of one window touch distinct
+//        nodes in each instruction, so they add with plain read-modify-writes
+//        (a wave's LDS instructions execute in order).  A particle that moved
+//        to another cell adds into a shared, one node wider window with LDS
+//        atomics, after the rounds.
 // G2P of step n and P2G of step n+1 are fused (the new state never leaves
 // registers); the new state is written round by round at consecutive
 // positions of the tile's output range, so stores and the next step's loads
