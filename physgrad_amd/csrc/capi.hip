@@ -221,7 +221,8 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
                    (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, 2 * toff)) ||
                    (rc = dalloc(c, &w.tscene, toff)) || (rc = dalloc(c, &w.bscene, boff)) ||
-                   (rc = dalloc(c, &w.nflag, toff)) || (rc = dalloc(c, &w.wflag, toff)))))
+                   (rc = dalloc(c, &w.bflag, toff)) || (rc = dalloc(c, &w.blist, toff)) ||
+                   (rc = dalloc(c, &w.nblist, 1)))))
         return rc;
     if (tiled) {
         // P2G window buffers: 8^3 nodes (8 KiB) for every possible active-list entry
@@ -234,8 +235,8 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
         hipMemset(w.count, 0, toff * 4);
         hipMemset(w.flags, 0, boff * 4);
         hipMemset(w.nactive, 0, 4);
-        hipMemset(w.nflag, 0, toff * 4);
-        hipMemset(w.wflag, 0, toff * 4);
+        hipMemset(w.bflag, 0, toff * 4);
+        hipMemset(w.nblist, 0, 4);
         hipMemset(w.tslot, 0, toff * 8);
         // scene of every tile slot and grid block (the kernels look scenes up by these)
         std::vector<uint32_t> ts(toff), bs(boff);

@@ -66,17 +66,20 @@ struct TileArgs {
     uint32_t* src_out;       // output position -> input slot (segment tapes only)
     const float4* gv;        // concatenated grids of all scenes
     float4* mp;
-    uint32_t* nflag;         // 4^3 node blocks hit by global-atomic P2G (by tile slot)
-    uint32_t* wflag;         // 4^3 node blocks covered by a buffered window (by tile slot)
+    uint32_t* bflag;         // 4^3 node blocks to update (by tile slot): bit 0 window, bit 1 global atomics
+    uint32_t* blist;         // flagged blocks, first-flagged order
+    uint32_t* nblist;
     float4* wbuf;            // window sums by active-list index
     const SceneInfo* scenes;
     size_t cap;
 };
 
 
-// marks the 4^3 node block of node (gi, gj, gk) as holding global-atomic P2G
-__device__ __forceinline__ void flag_node(uint32_t* nflag, const SceneInfo& sc, int gi, int gj, int gk) {
-    nflag[sc.toff + morton_tile(gi / TT, gj / TT, gk / TT)] = 1u;
+// marks 4^3 node block B (tile slot) for the grid update; the first mark of a
+// step appends it to the block list
+__device__ __forceinline__ void flag_block(uint32_t* bflag, uint32_t* blist, uint32_t* nblist, uint32_t B,
+                                           uint32_t bit) {
+    if (atomicOr(bflag + B, bit) == 0u) blist[atomicAdd(nblist, 1u)] = B;
 }
 
 // tile coordinates of a position, clamped like tile_key
@@ -201,9 +204,9 @@ __device__ __forceinline__ void p2g_window_atomic(float* wf, int nb, const Stenc
 
 // P2G into the global grid with atomics; the node blocks hit are flagged for
 // the grid update.
-__device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* __restrict__ nflag,
-                                           const SceneInfo& sc, const MlsGeom& g, const Stencil& st,
-                                           const P2GVal& p) {
+__device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* bflag, uint32_t* blist,
+                                           uint32_t* nblist, const SceneInfo& sc, const MlsGeom& g,
+                                           const Stencil& st, const P2GVal& p) {
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
         const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
@@ -215,7 +218,7 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* _
         atomic_add_f32(cp + 1, c.y);
         atomic_add_f32(cp + 2, c.z);
         atomic_add_f32(cp + 3, c.w);
-        flag_node(nflag, sc, gi, gj, gk);
+        flag_block(bflag, blist, nblist, sc.toff + morton_tile(gi / TT, gj / TT, gk / TT), 2u);
     }
 }
 
@@ -432,7 +435,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 } else if (l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT) {
                     p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st, p);
                 } else {
-                    p2g_global(mps, a.nflag, sc, g, st, p);
+                    p2g_global(mps, a.bflag, a.blist, a.nblist, sc, g, st, p);
                 }
                 lds_order();
             };
@@ -547,7 +550,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                         p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st,
                                           p);
                     else
-                        p2g_global(mps, a.nflag, sc, g, st, p);
+                        p2g_global(mps, a.bflag, a.blist, a.nblist, sc, g, st, p);
                 }
             }
             // ---- ranks in the new tiles: one global atomic per (tile, direction)
@@ -595,40 +598,36 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 // node blocks this window covers: the tile's and its 26 neighbours
                 const int bx = tc[0] + tid % 3 - 1, by = tc[1] + (tid / 3) % 3 - 1, bz = tc[2] + tid / 9 - 1;
                 if (bx >= 0 && by >= 0 && bz >= 0 && bx < sc.ntx && by < sc.nty && bz < sc.ntz)
-                    a.wflag[sc.toff + morton_tile(bx, by, bz)] = 1u;
+                    flag_block(a.bflag, a.blist, a.nblist, sc.toff + morton_tile(bx, by, bz), 1u);
             }
         }
         __syncthreads();
     }
 }
 
-// Grid update over 4^3 node blocks (one wave per block, lane = node).  The
+// Grid update of one flagged 4^3 node block (one wave, lane = node).  The
 // P2G window of tile T covers nodes [4T - 1, 4T + 7) per axis, so node block
 // B is covered by tiles B + {-1, 0, 1}^3, each node by 8 of them; the node's
 // (m, p) is the sum of the buffered windows of every work item of those tiles
 // plus the global accumulator where the rare global-atomic P2G landed.
-// Writes gv = (p/m + dt g, m) with sticky walls and clears the accumulator.
-// TAPE (segment recompute for the adjoint): also stores the summed (m, p) in
-// the step's grid and flags its 8^3 block, both of which the reverse pass
-// uses and clears.  A block without flags returns after two loads, before the
-// scene lookup (a search over the scene table).
+// Writes gv = (p/m + dt g, m) with sticky walls and clears the accumulator
+// and the block's flags.  TAPE (segment recompute for the adjoint): also
+// stores the summed (m, p) in the step's grid and flags its 8^3 block, both
+// of which the reverse pass uses and clears.
 template <bool TAPE>
-__device__ __forceinline__ void grid_block(int B, const SceneInfo* __restrict__ scenes,
-                                           const uint32_t* __restrict__ tscene, int slots,
-                                           const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
-                                           uint32_t* __restrict__ wflag, const float4* __restrict__ wbuf,
-                                           float4* __restrict__ mp_all, float4* __restrict__ gv_all,
-                                           uint32_t* __restrict__ flags8) {
-    if (B >= slots) return;
-    // covered by a buffered window, or global-atomic P2G here?
-    const bool fb = nflag[B] != 0u, wb = wflag[B] != 0u;
-    if (!fb && !wb) return;
+__device__ __forceinline__ void grid_block(uint32_t B, const SceneInfo* __restrict__ scenes,
+                                           const uint32_t* __restrict__ tscene,
+                                           const uint32_t* __restrict__ tslot, uint32_t* __restrict__ bflag,
+                                           const float4* __restrict__ wbuf, float4* __restrict__ mp_all,
+                                           float4* __restrict__ gv_all, uint32_t* __restrict__ flags8) {
+    // covered by a buffered window (bit 0), global-atomic P2G here (bit 1)
+    const uint32_t fl = bflag[B];
+    const bool fb = (fl & 2u) != 0u, wb = (fl & 1u) != 0u;
     const int lane = threadIdx.x & 63;
     const SceneInfo sc = scenes[tscene[B]];
-    const uint32_t lb = (uint32_t)B - sc.toff;
+    const uint32_t lb = B - sc.toff;
     const int bc[3] = {(int)compact3(lb), (int)compact3(lb >> 1), (int)compact3(lb >> 2)};
     const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
-    if (bc[0] >= nt[0] || bc[1] >= nt[1] || bc[2] >= nt[2]) return;
     // first active-list entry of a tile and its number of entries (chunks)
     auto slot_of = [&](int tx, int ty, int tz, uint32_t& ne) -> uint32_t {
         ne = 0u;
@@ -664,7 +663,7 @@ __device__ __forceinline__ void grid_block(int B, const SceneInfo* __restrict__ 
     }
     float4* __restrict__ mp = mp_all + sc.goff;
     float4* __restrict__ gv = gv_all + sc.goff;
-    if (i < g.nx && j < g.ny && k < g.nz) {
+    if (i < g.nx && j < g.ny && k < g.nz && bc[0] < nt[0] && bc[1] < nt[1] && bc[2] < nt[2]) {
         const size_t id = node_id(g, i, j, k);
         if (fb) {
             const float4 u = mp[id];
@@ -685,23 +684,39 @@ __device__ __forceinline__ void grid_block(int B, const SceneInfo* __restrict__ 
         if (TAPE) mp[id] = q;
     }
     if (lane == 0) {
-        if (fb) nflag[B] = 0u;
-        if (wb) wflag[B] = 0u;
-        if (TAPE)
+        bflag[B] = 0u;
+        if (TAPE && bc[0] < nt[0] && bc[1] < nt[1] && bc[2] < nt[2])
             flags8[sc.boff + ((bc[2] * TT / GB) * sc.gby + bc[1] * TT / GB) * sc.gbx + bc[0] * TT / GB] = 1u;
     }
 }
 
+// The flagged blocks of the list, one per wave, waves striding over the list
+// (nwg workgroups of 4 waves).  The list is emptied by the next binning
+// (k_scan_b) or, after a call's first grid update, by k_list_reset.
+template <bool TAPE>
+__device__ __forceinline__ void grid_list(uint32_t nwg, const SceneInfo* __restrict__ scenes,
+                                          const uint32_t* __restrict__ tscene, const uint32_t* __restrict__ tslot,
+                                          uint32_t* __restrict__ bflag, const uint32_t* __restrict__ blist,
+                                          const uint32_t* __restrict__ nblist, const float4* __restrict__ wbuf,
+                                          float4* __restrict__ mp_all, float4* __restrict__ gv_all,
+                                          uint32_t* __restrict__ flags8) {
+    const uint32_t nl = *nblist;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < nl; i += nwg * 4)
+        grid_block<TAPE>(blist[i], scenes, tscene, tslot, bflag, wbuf, mp_all, gv_all, flags8);
+}
+
+__global__ void k_list_reset(uint32_t* __restrict__ nblist) { *nblist = 0u; }
+
 template <bool TAPE>
 __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict__ scenes,
-                                                    const uint32_t* __restrict__ tscene, int slots,
-                                                    const uint32_t* __restrict__ tslot, uint32_t* __restrict__ nflag,
-                                                    uint32_t* __restrict__ wflag,
+                                                    const uint32_t* __restrict__ tscene,
+                                                    const uint32_t* __restrict__ tslot, uint32_t* __restrict__ bflag,
+                                                    const uint32_t* __restrict__ blist,
+                                                    const uint32_t* __restrict__ nblist,
                                                     const float4* __restrict__ wbuf,
                                                     float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                     uint32_t* __restrict__ flags8) {
-    grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf, mp_all,
-                     gv_all, flags8);
+    grid_list<TAPE>(gridDim.x, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8);
 }
 
 // First binning of a call: tile key per storage slot, ranks from one global
@@ -799,7 +814,9 @@ template <bool TAPE>
 __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restrict__ scenes,
                                                         const uint32_t* __restrict__ tscene, int slots,
                                                         const uint32_t* __restrict__ tslot,
-                                                        uint32_t* __restrict__ nflag, uint32_t* __restrict__ wflag,
+                                                        uint32_t* __restrict__ bflag,
+                                                        const uint32_t* __restrict__ blist,
+                                                        const uint32_t* __restrict__ nblist,
                                                         const float4* __restrict__ wbuf,
                                                         float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                         uint32_t* __restrict__ flags8, int ngrid,
@@ -808,8 +825,7 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
     static_assert(SCAN_T == 256, "both parts run 256 threads");
     __shared__ uint32_t sh[SCAN_T];
     if ((int)blockIdx.x < ngrid)
-        grid_block<TAPE>(blockIdx.x * 4 + (threadIdx.x >> 6), scenes, tscene, slots, tslot, nflag, wflag, wbuf,
-                         mp_all, gv_all, flags8);
+        grid_list<TAPE>((uint32_t)ngrid, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8);
     else
         scan_a_block(blockIdx.x - ngrid, count, slots, bsum, sh);
 }
@@ -817,8 +833,11 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
 __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
                                                    uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
-                                                   uint2* __restrict__ tslot, const uint32_t* __restrict__ tscene) {
+                                                   uint2* __restrict__ tslot, const uint32_t* __restrict__ tscene,
+                                                   uint32_t* __restrict__ nblist) {
     __shared__ uint32_t sh[SCAN_T];
+    // the grid update of the last binning has read the block list: empty it
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nblist = 0u;
     // totals of the preceding blocks
     uint32_t ps = 0, pa = 0;
     for (int b = threadIdx.x; b < (int)blockIdx.x; b += SCAN_T) {
@@ -871,7 +890,7 @@ hipError_t bin_rest(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     ProfScope ps(w.prof, PK_BIN, st);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene);
+                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
 }
@@ -882,7 +901,7 @@ hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene);
+                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
     return hipGetLastError();
 }
@@ -896,20 +915,27 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.rank_out = w.rank;
     a.count = w.count;
     a.src_out = nullptr;
-    a.nflag = w.nflag;
-    a.wflag = w.wflag;
+    a.bflag = w.bflag;
+    a.blist = w.blist;
+    a.nblist = w.nblist;
     a.wbuf = w.wbuf;
     a.scenes = w.scenes;
     a.cap = cap;
 }
+
+// workgroups of the grid update (4 waves each, striding over the block list):
+// one wave per possible block, up to 4,096 workgroups
+int grid_wgs(const TiledWork& w) { return std::max(1, std::min((w.tile_slots + 3) / 4, 4096)); }
 
 // tslot: the binning whose tile kernel filled the window buffers
 template <bool TAPE>
 void grid_update(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* gv, uint32_t* flags8,
                  hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
-    k_grid_nodes<TAPE><<<(w.tile_slots + 3) / 4, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag,
-                                                                w.wflag, w.wbuf, mp, gv, flags8);
+    k_grid_nodes<TAPE><<<grid_wgs(w), 256, 0, st>>>(w.scenes, w.tscene, tslot, w.bflag, w.blist, w.nblist, w.wbuf,
+                                                     mp, gv, flags8);
+    // no binning follows a call's first grid update: empty the list here
+    k_list_reset<<<1, 1, 0, st>>>(w.nblist);
 }
 
 // grid update of the binning whose tile kernel filled the window buffers
@@ -918,10 +944,10 @@ template <bool TAPE>
 void grid_update_and_count(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* gv, uint32_t* flags8,
                            hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
-    const int ngrid = (w.tile_slots + 3) / 4;
+    const int ngrid = grid_wgs(w);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
-    k_grid_and_count<TAPE><<<ngrid + nb, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.nflag, w.wflag,
-                                                        w.wbuf, mp, gv, flags8, ngrid, w.count, w.bsum);
+    k_grid_and_count<TAPE><<<ngrid + nb, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.bflag, w.blist,
+                                                        w.nblist, w.wbuf, mp, gv, flags8, ngrid, w.count, w.bsum);
 }
 
 }  // namespace

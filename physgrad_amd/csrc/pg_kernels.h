@@ -136,8 +136,10 @@ struct TiledWork {
     uint32_t* tslot;       // [tile_slots] uint2 (first active-list entry of the tile or ~0, entry count)
     uint32_t* tscene;      // [tile_slots] scene of each tile slot
     uint32_t* bscene;      // [total_blocks] scene of each 8^3 grid block
-    uint32_t* nflag;       // [tile_slots] 4^3 node blocks hit by out-of-window P2G (global atomics)
-    uint32_t* wflag;       // [tile_slots] 4^3 node blocks covered by a buffered P2G window
+    uint32_t* bflag;       // [tile_slots] 4^3 node blocks to update: bit 0 covered by a buffered P2G
+                           // window, bit 1 hit by out-of-window P2G (global atomics)
+    uint32_t* blist;       // [tile_slots] the flagged blocks, in the order they were first flagged
+    uint32_t* nblist;      // [1] their count; emptied after every grid update
     float4* wbuf;          // [max_entries][512] P2G window sums by active-list entry
     int max_entries;       // capacity of the active list
     int nblocks_launch;    // persistent grid size of the tile kernels
