@@ -25,44 +25,53 @@ PG_DEV float det3(const float* A) {
            A[2] * (A[3] * A[7] - A[4] * A[6]);
 }
 
+// cofactor matrix of A (row-major, so that inv(A)^T = cof(A) / det(A)); returns det(A)
+PG_DEV float cofactors3(const float* A, float* c) {
+    c[0] = A[4] * A[8] - A[5] * A[7];
+    c[1] = A[5] * A[6] - A[3] * A[8];
+    c[2] = A[3] * A[7] - A[4] * A[6];
+    c[3] = A[2] * A[7] - A[1] * A[8];
+    c[4] = A[0] * A[8] - A[2] * A[6];
+    c[5] = A[1] * A[6] - A[0] * A[7];
+    c[6] = A[1] * A[5] - A[2] * A[4];
+    c[7] = A[2] * A[3] - A[0] * A[5];
+    c[8] = A[0] * A[4] - A[1] * A[3];
+    return A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
+}
+
 // inverse transpose via cofactors: inv(A)^T = cof(A)/det(A); returns det
 PG_DEV float inv_t3(const float* A, float* out) {
-    float c0 = A[4] * A[8] - A[5] * A[7];
-    float c1 = A[5] * A[6] - A[3] * A[8];
-    float c2 = A[3] * A[7] - A[4] * A[6];
-    float c3 = A[2] * A[7] - A[1] * A[8];
-    float c4 = A[0] * A[8] - A[2] * A[6];
-    float c5 = A[1] * A[6] - A[0] * A[7];
-    float c6 = A[1] * A[5] - A[2] * A[4];
-    float c7 = A[2] * A[3] - A[0] * A[5];
-    float c8 = A[0] * A[4] - A[1] * A[3];
-    float d = A[0] * c0 + A[1] * c1 + A[2] * c2;
-    float id = 1.0f / d;
-    out[0] = c0 * id; out[1] = c1 * id; out[2] = c2 * id;
-    out[3] = c3 * id; out[4] = c4 * id; out[5] = c5 * id;
-    out[6] = c6 * id; out[7] = c7 * id; out[8] = c8 * id;
+    float c[9];
+    const float d = cofactors3(A, c);
+    const float id = 1.0f / d;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = c[i] * id;
     return d;
 }
 
 // Orthogonal polar factor by Newton's iteration X <- (X + X^-T)/2 (Higham 1986),
-// the same iteration as the oracle, stopped once the relative change drops below
-// 1e-6 (quadratic convergence puts the next error below fp32 rounding), <= 24 its.
+// the oracle's iteration.  Once the step's relative change d is small the
+// iteration is quadratic (a singular value 1 + e goes to 1 + e^2/2 + O(e^3),
+// and d = e to first order), so stopping at d <= 1e-4 leaves an error of about
+// 5e-9, below fp32 rounding.  ||X||_F^2 tends to 3 for a rotation, so the test
+// compares the squared change with 3e-8 instead of with the norm.  X^-T uses
+// the hardware reciprocal (about 1 ulp): a relative error eps in it moves the
+// fixed point X = X^-T by O(eps), which is fp32 rounding again.  <= 24 steps.
 PG_DEV void polar(const float* F, float* R) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = F[i];
     for (int it = 0; it < 24; ++it) {
-        float Y[9];
-        inv_t3(R, Y);
-        float diff = 0.f, nrm = 0.f;
+        float c[9];
+        const float h = 0.5f * __builtin_amdgcn_rcpf(cofactors3(R, c));
+        float diff = 0.f;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            float nx = 0.5f * (R[i] + Y[i]);
-            float d = nx - R[i];
+            const float nx = c[i] * h + 0.5f * R[i];
+            const float d = nx - R[i];
             diff += d * d;
-            nrm += nx * nx;
             R[i] = nx;
         }
-        if (diff <= 1e-12f * nrm) break;
+        if (diff <= 3e-8f) break;
     }
 }
 
