@@ -47,6 +47,7 @@ namespace {
 constexpr int PASS = 1280;       // particles staged per LDS pass of a work item
 static_assert(CHUNK % PASS == 0, "a work item is a whole number of passes");
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
+constexpr int QCAP = 48;         // P2G values of cell changers kept in LDS per pass (more are reloaded)
 
 struct TileArgs {
     const float4* Sin;
@@ -237,10 +238,13 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* b
 // second.  Within a window the lanes of a group sit in distinct cells, so
 // their 27 node updates never meet in one instruction and add with plain
 // read-modify-writes in program order; groups are nearly full whatever the
-// spread of the cell counts.  A particle that left its cell during G2P is
-// queued; after the rounds the queue is compacted and its particles add into
-// a shared, one node wider window with LDS atomics, or, having left even that
-// (more than a cell per step), with global atomics.  Outputs are written
+// spread of the cell counts.  A particle that left its cell during G2P adds
+// into a shared, one node wider window after the rounds: the first QCAP of a
+// pass leave their P2G values in LDS entries, which one wave adds with
+// read-modify-writes (one entry per stencil base at a time, the rare others
+// with LDS atomics); the rest are queued, reloaded and added with LDS atomics,
+// or, having left even the wide window (more than a cell per step), with
+// global atomics.  Outputs are written
 // group by group, so stores coalesce.  At the end of the tile the windows are
 // summed into the tile's buffer.
 template <int MODE>
@@ -259,7 +263,10 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     __shared__ uint16_t ldir[(G2P && P2G) ? PASS : 1];  // (direction << 11 | rank) of the new tile, or FAR
     __shared__ uint32_t lcnt[65], lstart[65];
     __shared__ uint32_t dcnt[27], dbase[27];
-    __shared__ uint32_t s_max, s_tot, s_nq;
+    __shared__ float4 qval[4][(G2P && P2G) ? QCAP : 1];  // changers' (m, q, a0, a1, a2, f)
+    __shared__ int qnb[(G2P && P2G) ? QCAP : 1];         // and their wide-window stencil base
+    __shared__ uint32_t s_max, s_tot, s_nq, s_qe;
+    static_assert(PV <= 4 * PASS, "the changers' cell tags fit in lslot");
     static_assert(PASS / 32 <= 64, "one wave compacts the P2G queue");
 
     const XcdChunk ch(*a.nactive);
@@ -299,8 +306,10 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             // ---- bulk load and count by stencil-base cell
             if (tid < 65) lcnt[tid] = 0;
             if (tid < 27) dcnt[tid] = 0;
-            if (G2P && P2G)
+            if (G2P && P2G) {
                 for (int q = tid; q < PASS / 32; q += TB) lqueue[q] = 0u;
+                if (tid == 0) s_qe = 0u;
+            }
             __syncthreads();
             uint32_t key[PT], ks[PT];
             float4 k0[PT];
@@ -400,10 +409,12 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 
             // P2G of a particle with its (new) state: into the wave's window by
             // read-modify-write when its base is this lane's cell (own).  Otherwise
-            // the fused kernel queues it for the pass after the rounds (so that the
-            // rounds never branch off for the few particles that changed cell), and
-            // the P2G-only kernel adds it into the shared wide window with LDS
-            // atomics when it is nearby, else with global atomics.
+            // the fused kernel queues it for a pass after the rounds (so that the
+            // rounds never branch off into its 27 node updates): its P2G values,
+            // computed alongside the own lanes', go to an LDS entry while there is
+            // one, else the particle is reloaded after the rounds.  The P2G-only
+            // kernel adds it into the shared wide window with LDS atomics when it
+            // is nearby, else with global atomics.
             // ob = window index of the particle's base before its G2P (own_ok only);
             // win = which of the wave's two windows
             auto scatter = [&](const float* x, const float* F, const float* C, const float* v, float Eq, float mq,
@@ -414,12 +425,22 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 const int nb = (l2 * WN + l1) * WN + l0;
                 const bool inside = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
                 const bool own = own_ok && inside && nb == ob;
-                if (G2P && !own) {
-                    atomicOr(&lqueue[oo >> 5], 1u << (oo & 31));
-                    return;
-                }
+                const bool near = l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT;
                 P2GVal p;
                 p2g_prep(g, Eq, mq, F, C, v, st, p);
+                if (G2P && !own) {
+                    const uint32_t e = near ? atomicAdd(&s_qe, 1u) : (uint32_t)QCAP;
+                    if (e < QCAP) {
+                        qval[0][e] = make_float4(p.m, p.q[0], p.q[1], p.q[2]);
+                        qval[1][e] = make_float4(p.a0[0], p.a0[1], p.a0[2], p.a1[0]);
+                        qval[2][e] = make_float4(p.a1[1], p.a1[2], p.a2[0], p.a2[1]);
+                        qval[3][e] = make_float4(p.a2[2], st.f[0], st.f[1], st.f[2]);
+                        qnb[e] = ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1;
+                    } else {
+                        atomicOr(&lqueue[oo >> 5], 1u << (oo & 31));
+                    }
+                    return;
+                }
                 if (G2P || own) {
                     float4* __restrict__ pwin = pw[wave][win];
                     p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
@@ -432,7 +453,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                         *t = u;
                         lds_order();
                     });
-                } else if (l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT) {
+                } else if (near) {
                     p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st, p);
                 } else {
                     p2g_global(mps, a.bflag, a.blist, a.nblist, sc, g, st, p);
@@ -514,10 +535,17 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             const uint32_t nout = lcnt[64], sout = lstart[64];
             for (uint32_t t = tid; t < nout; t += TB) particle(sout + t, tot + t, false, 0);
             __syncthreads();
-            // ---- queued P2G of the particles that changed cell, from the state just
-            // written: the queue is compacted (into lrank, free after the rounds) so
-            // that each wave takes up to 64 of them at once; into the shared wide
-            // window with LDS atomics when nearby, else with global atomics
+            // ---- after the rounds: the last wave adds the changers' LDS entries into
+            // the shared wide window while wave 0 compacts the queue of the rest (into
+            // lrank, free after the rounds) and takes the ranks in the new tiles (one
+            // global atomic per (tile, direction)).  Of 64 entries with the same
+            // stencil base one, picked by a tag per base (in lslot, free after the
+            // rounds), goes in by plain read-modify-writes in program order as in the
+            // rounds (the picked lanes' bases differ, so one instruction never touches
+            // a node twice); the others, rare, with LDS atomics.  Then the queued
+            // particles are reloaded, up to 64 a wave, and added with LDS atomics
+            // (global atomics for one that left the wide window), and the new-tile
+            // ranks are written.
             if (P2G && G2P) {
                 if (wave == 0) {
                     const uint32_t bits = lane < (pc + 31) / 32 ? lqueue[lane] : 0u;
@@ -531,6 +559,46 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     uint32_t q = inc - nb;
                     for (uint32_t r = bits; r; r &= r - 1u) lrank[q++] = (uint16_t)(lane * 32 + __ffs(r) - 1);
                     if (lane == 63) s_nq = inc;
+                    if (lane < 27) {
+                        const uint32_t nd = dcnt[lane];
+                        if (nd) {
+                            const int dx_ = lane % 3 - 1, dy_ = (lane / 3) % 3 - 1, dz_ = lane / 9 - 1;
+                            const uint32_t k = sc.toff + morton_tile(tc[0] + dx_, tc[1] + dy_, tc[2] + dz_);
+                            dbase[lane] = atomicAdd(a.count + k, nd);
+                        }
+                    }
+                } else if (wave == NW - 1) {
+                    uint8_t* __restrict__ qtag = reinterpret_cast<uint8_t*>(lslot);
+                    const uint32_t ne = min(s_qe, (uint32_t)QCAP);
+                    for (uint32_t e0 = 0; e0 < ne; e0 += 64) {
+                        const uint32_t e = min(e0 + lane, ne - 1u);
+                        bool pend = e0 + lane < ne;
+                        const float4 u0 = qval[0][e], u1 = qval[1][e], u2 = qval[2][e], u3 = qval[3][e];
+                        const P2GVal p = {u0.x, {u0.y, u0.z, u0.w}, {u1.x, u1.y, u1.z}, {u1.w, u2.x, u2.y},
+                                          {u2.z, u2.w, u3.x}};
+                        Stencil st;
+                        stencil_weights(u3.y, st.w[0]);
+                        stencil_weights(u3.z, st.w[1]);
+                        stencil_weights(u3.w, st.w[2]);
+                        const int nb = qnb[e];
+                        if (pend) qtag[nb] = (uint8_t)lane;
+                        lds_order();
+                        const bool turn = pend && qtag[nb] == (uint8_t)lane;
+                        if (turn) {
+                            p2g_visit(st, p, [&](int ix, int iy, int iz, float4 c) {
+                                float4* t = mv + nb + (iz * PN + iy) * PN + ix;
+                                float4 u = *t;
+                                u.x += c.x;
+                                u.y += c.y;
+                                u.z += c.z;
+                                u.w += c.w;
+                                *t = u;
+                                lds_order();
+                            });
+                        }
+                        lds_order();
+                        if (pend && !turn) p2g_window_atomic(reinterpret_cast<float*>(mv), nb, st, p);
+                    }
                 }
                 __syncthreads();
                 const uint32_t nq = s_nq;
@@ -552,18 +620,6 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     else
                         p2g_global(mps, a.bflag, a.blist, a.nblist, sc, g, st, p);
                 }
-            }
-            // ---- ranks in the new tiles: one global atomic per (tile, direction)
-            if (G2P && P2G) {
-                if (tid < 27) {
-                    const uint32_t nd = dcnt[tid];
-                    if (nd) {
-                        const int dx_ = tid % 3 - 1, dy_ = (tid / 3) % 3 - 1, dz_ = tid / 9 - 1;
-                        const uint32_t k = sc.toff + morton_tile(tc[0] + dx_, tc[1] + dy_, tc[2] + dz_);
-                        dbase[tid] = atomicAdd(a.count + k, nd);
-                    }
-                }
-                __syncthreads();
                 for (uint32_t jj = tid; jj < pc; jj += TB) {
                     const uint16_t dr = ldir[jj];
                     if (dr != FAR) a.rank_out[beg + pb + jj] = dbase[dr >> 11] + (dr & 0x7FFu);

@@ -201,17 +201,22 @@ PG_DEV int stencil_base(float xs) {
     return (int)floorf(xs - 0.5f);
 }
 
+// Quadratic B-spline weights of the three stencil nodes at offset f in [0.5, 1.5).
+PG_DEV void stencil_weights(float f, float* w) {
+    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+    w[1] = 0.75f - (f - 1.f) * (f - 1.f);
+    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+}
+
 PG_DEV void make_stencil(const float* x, const float* o, float inv_dx, Stencil& s) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float xs = grid_coord(x[a], o[a], inv_dx);
         const int b = stencil_base(xs);
-        float f = xs - (float)b;
+        const float f = xs - (float)b;
         s.b[a] = b;
         s.f[a] = f;
-        s.w[a][0] = 0.5f * (1.5f - f) * (1.5f - f);
-        s.w[a][1] = 0.75f - (f - 1.f) * (f - 1.f);
-        s.w[a][2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+        stencil_weights(f, s.w[a]);
     }
 }
 

@@ -144,6 +144,43 @@ def test_clustered_cells_parity(gpu, oracle_lib):
             x64, v64, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
 
 
+def test_cell_changers_parity(gpu, oracle_lib):
+    """About 70% of the particles change stencil-base cell during the step (a
+    translation plus a rotation moving them 0.2-0.6 cells per step at ~16
+    particles per cell), so a tile pass holds hundreds of them: more than the
+    fused kernel keeps in LDS (the rest are reloaded after the rounds) and
+    many sharing a base cell (one per base in the read-modify-write pass, the
+    rest with atomics).  Two steps of the tiled path against the FP64 oracle,
+    both materials."""
+    from physgrad_amd.scenes import Scene
+    O = oracle_lib
+    n, dt, dx = 4096, 1e-4, 1 / 16
+    cells_per_step = dx / dt
+    for material in (0, 1):
+        rng = np.random.default_rng(60 + material)
+        x = (0.3 + 0.4 * rng.random((n, 3))).astype(np.float32)
+        w = np.array([0.6, -0.8, 0.5]) * 2.5 * cells_per_step
+        v = (np.array([0.35, -0.25, 0.2]) * cells_per_step + np.cross(w, x - 0.5)).astype(np.float32)
+        sc = Scene("changers", (16, 16, 16), dx, dt, 2, material, 2e4, 0.3, 1000.0, 0.4 ** 3 / n, x, v)
+        prm = O.scene_params(sc)
+        x64, v64 = x.astype(np.float64), v.astype(np.float64)
+        C64 = rng.normal(0, 0.3, (n, 9))
+        F64 = np.eye(3).reshape(1, 9) + rng.normal(0, 0.02, (n, 9))
+        for k in range(2):
+            print("changers material", material, "step", k, flush=True)
+            want = O.mls_run(prm, x64, v64, C64, F64, sc.p_mass, sc.E, 1, want_grid=True)
+            base0 = np.floor((x64 - sc.origin) / dx - 0.5)
+            base1 = np.floor((want["x"].reshape(-1, 3) - sc.origin) / dx - 0.5)
+            assert np.mean(np.any(base0 != base1, axis=1)) > 0.5
+            got = gpu_step(sc, 1, x64.astype(np.float32), v64.astype(np.float32), C64.astype(np.float32),
+                           F64.astype(np.float32), None)
+            for key in ("x", "v", "C", "F"):
+                err = rel(got[key].reshape(-1), want[key].reshape(-1))
+                assert err <= TOL_STEP, (key, material, k, err)
+            check_grid(got["grid_m"], got["grid_v"], want["grid_m"], want["grid_v"])
+            x64, v64, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
+
+
 def test_error_vs_fp64_after_n_steps(gpu, oracle_lib):
     """Single-box workload (64^3, 32,768 particles, fixed-corotated): the FP32
     tiled GPU trajectory against the FP64 oracle after 50 steps."""
