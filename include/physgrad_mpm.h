@@ -97,7 +97,11 @@ int pg_mpm_num_particles(const pg_mpm_ctx* ctx);
 /* Grid of the last step: mass [G], momentum [3G] (ref-compat and plain MLS;
  * pre-update momentum), velocity [3G] (post-update).  Any pointer may be NULL. */
 int pg_mpm_get_grid(pg_mpm_ctx* ctx, float* mass, float* momentum, float* velocity, int memory, void* stream);
-/* Advance nsteps steps on the stream. */
+/* Advance nsteps steps on the stream.  On the tiled MLS-MPM path the last
+ * step of a call also scatters the new state to the grid windows and counts
+ * its binning, which the next pg_mpm_step call starts from (so that stepping
+ * one step per call costs what a step inside a long call does); the calls
+ * that replace or re-simulate the particles complete that work first. */
 int pg_mpm_step(pg_mpm_ctx* ctx, int nsteps, void* stream);
 /* Simulated time and step count (reference getCurrentTime/getStepCount). */
 int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);

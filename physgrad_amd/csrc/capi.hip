@@ -56,7 +56,10 @@ struct pg_mpm_ctx {
     std::vector<void*> ck_allocs, tape_allocs;
     // hipGraph replay of pg_mpm_step sequences: key (nsteps, state buffer) -> executable graph
     bool use_graph = false;
-    struct GraphEntry { int nsteps; const void* S; hipGraphExec_t exec; };
+    // the last tiled step left the current state's P2G and binning counts pending
+    // for the next step (consumed by it, or by settle() before anything else)
+    bool primed = false;
+    struct GraphEntry { int nsteps; const void* S; bool primed; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
     hipStream_t gstream = nullptr;
     hipEvent_t gev[2] = {nullptr, nullptr};
@@ -272,6 +275,17 @@ void drop_graphs(pg_mpm_ctx* c) {
     c->graphs.clear();
 }
 
+// Completes the work a tiled step left pending for the next one (its P2G
+// windows, global-accumulator adds, block flags and tile counts: the grid
+// update and the binning) before the particles, the binning buffers or the
+// grid accumulators are used by anything but the next step.  The grid
+// velocities then belong to the step after the last one taken.
+hipError_t settle(pg_mpm_ctx* c, hipStream_t st) {
+    if (!c->primed) return hipSuccess;
+    c->primed = false;
+    return pg::tiled_settle(c->tw, c->n, c->mp, c->gv, st);
+}
+
 // Runs nsteps tiled steps as one hipGraph launch.  The step sequence of a
 // given length starting from a given state buffer is captured once and then
 // replayed; the host-side ping-pong of the state buffers is applied after
@@ -293,7 +307,7 @@ hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
     }
     hipGraphExec_t exec = nullptr;
     for (auto& ge : c->graphs)
-        if (ge.nsteps == nsteps && ge.S == c->S) exec = ge.exec;
+        if (ge.nsteps == nsteps && ge.S == c->S && ge.primed == c->primed) exec = ge.exec;
     if (!exec) {
         const bool prof = c->prof.on;
         c->prof.on = false;  // no event records inside the captured sequence
@@ -307,7 +321,7 @@ hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
         e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
         if (e == hipSuccess) {
             hipError_t ek = pg::tiled_steps(c->tw, c->n, c->cap, S, S2, pid, pid2, E, E2, m, m2, c->mp, c->gv,
-                                            nsteps, st);
+                                            nsteps, st, c->primed, true);
             e = hipStreamEndCapture(st, &graph);
             if (ek != hipSuccess) e = ek;
         }
@@ -315,9 +329,10 @@ hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
         if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (graph) hipGraphDestroy(graph);
         if (e != hipSuccess) return e;
-        c->graphs.push_back({nsteps, c->S, exec});
+        c->graphs.push_back({nsteps, c->S, c->primed, exec});
     }
     if ((e = hipGraphLaunch(exec, st)) != hipSuccess) return e;
+    c->primed = true;
     if (nsteps & 1) {
         std::swap(c->S, c->S2);
         std::swap(c->pid, c->pid2);
@@ -454,6 +469,7 @@ int set_particles_impl(pg_mpm_ctx* c, int n, const int* counts, const float* x, 
     if (memory != PG_HOST && memory != PG_DEVICE) return fail(c, PG_ERR_INVALID, "bad memory kind");
     hipStream_t st = (hipStream_t)stream;
     int rc;
+    if (hipError_t es = settle(c, st); es != hipSuccess) return hip_fail(c, es, "settle");
     c->n = n;
     c->time = 0.0;
     c->steps = 0;
@@ -636,7 +652,8 @@ int pg_mpm_step(pg_mpm_ctx* c, int nsteps, void* stream) {
                 e = graph_steps(c, nsteps, st);
             } else {
                 e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
-                                    nsteps, st);
+                                    nsteps, st, c->primed, true);
+                c->primed = true;
                 if (c->have_E) { c->E = E; c->E2 = E2; }
                 if (c->have_mass) { c->mass = m; c->mass2 = m2; }
             }
@@ -801,7 +818,7 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
     int rc;
     if ((rc = ensure_checkpoints(c, ncp)) || (rc = ensure_tape(c, K))) return rc;
     hipStream_t st = (hipStream_t)stream;
-    hipError_t e = hipSuccess;
+    hipError_t e = settle(c, st);
     const bool plain = c->p.impl == PG_IMPL_PLAIN;
     const pg::MlsGeom g = mls_geom(c);
     // a tape filled by an earlier forward and never reversed: clear its grids
@@ -1120,6 +1137,11 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
         c->E = E0; c->E2 = E20; c->mass = m0; c->mass2 = m20;
     };
     save(false);
+    // after the grid is saved, so that the restored grid is the last step's
+    if (hipError_t es = settle(c, st); es != hipSuccess) {
+        release();
+        return hip_fail(c, es, "settle");
+    }
     hipEvent_t a = nullptr, b = nullptr;
     hipEventCreate(&a);
     hipEventCreate(&b);

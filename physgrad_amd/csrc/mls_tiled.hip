@@ -1016,7 +1016,7 @@ hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t*
 
 hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
                        uint32_t*& pid2, float*& E, float*& E2, float*& mass, float*& mass2, float4* mp, float4* gv,
-                       int nsteps, hipStream_t st) {
+                       int nsteps, hipStream_t st, bool primed, bool leave_primed) {
     if (nsteps <= 0) return hipSuccess;
     TileArgs a;
     fill_args(a, w, cap);
@@ -1031,16 +1031,25 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
         a.E_in = E; a.E_out = E2;
         a.m_in = mass; a.m_out = mass2;
     };
-    hipError_t e = tiled_init_keys(w, n, S, pid, st);
-    if (e != hipSuccess) return e;
-    set_io();
-    {
-        ProfScope ps(w.prof, PK_TILE_P2G, st);
-        k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
+    hipError_t e;
+    if (primed) {
+        // the previous call's fused kernel added this state's P2G into the windows
+        // of its binning (w.tslot) and counted this state's tiles
+        grid_update_and_count<false>(w, w.tslot, mp, gv, nullptr, st);
+        e = bin_rest(w, n, w.perm, w.active, w.nactive, w.tslot, st);
+        if (e != hipSuccess) return e;
+    } else {
+        e = tiled_init_keys(w, n, S, pid, st);
+        if (e != hipSuccess) return e;
+        set_io();
+        {
+            ProfScope ps(w.prof, PK_TILE_P2G, st);
+            k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
+        }
+        // the grid update reads the window buffers by this binning's active list,
+        // so it runs before the next binning
+        grid_update<false>(w, w.tslot, mp, gv, nullptr, st);
     }
-    // the grid update reads the window buffers by this binning's active list,
-    // so it runs before the next binning
-    grid_update<false>(w, w.tslot, mp, gv, nullptr, st);
     for (int s = 0; s + 1 < nsteps; ++s) {
         set_io();
         {
@@ -1053,12 +1062,20 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
         std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
     }
     set_io();
-    {
+    if (leave_primed) {
+        ProfScope ps(w.prof, PK_TILE_G2P2G, st);
+        k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
+    } else {
         ProfScope ps(w.prof, PK_TILE_G2P, st);
         k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(a);
     }
     std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
     return hipGetLastError();
+}
+
+hipError_t tiled_settle(TiledWork& w, int n, float4* mp, float4* gv, hipStream_t st) {
+    grid_update_and_count<false>(w, w.tslot, mp, gv, nullptr, st);
+    return bin_rest(w, n, w.perm, w.active, w.nactive, w.tslot, st);
 }
 
 namespace {

@@ -194,8 +194,19 @@ hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4*
 hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t* pid, hipStream_t st);
 // Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on
 // return the current buffers are S, pid, E, mass (the references are swapped).
+// primed: the previous call left this state's P2G in the window buffers (and
+// the global accumulator) and its tile counts counted, so the call starts with
+// that grid update and binning instead of a binning from scratch and a P2G.
+// leave_primed: the last step runs the fused kernel, leaving the new state's
+// P2G and counts pending in the same way (else a G2P-only kernel, nothing
+// pending).  Pending work must be consumed by the next primed call or by
+// tiled_settle before the particles, the binning buffers or the grid are used
+// otherwise.
 hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2, uint32_t*& pid,
                        uint32_t*& pid2, float*& E, float*& E2, float*& mass, float*& mass2, float4* mp, float4* gv,
-                       int nsteps, hipStream_t st);
+                       int nsteps, hipStream_t st, bool primed = false, bool leave_primed = false);
+// consumes the work a primed call left pending: the grid update of its windows
+// (into gv) and the binning of its counts
+hipError_t tiled_settle(TiledWork& w, int n, float4* mp, float4* gv, hipStream_t st);
 
 }  // namespace pg

@@ -181,6 +181,74 @@ def test_cell_changers_parity(gpu, oracle_lib):
             x64, v64, C64, F64 = want["x"], want["v"], want["C"].reshape(-1, 9), want["F"].reshape(-1, 9)
 
 
+def test_one_step_calls_match_one_call(gpu):
+    """The tiled path leaves the next step's P2G and binning pending at the end
+    of a call and starts the next call from them: six one-step calls (eager and
+    graph-replayed) match one six-step call; new particles after a pending
+    step, and a checkpointed rollout and its backward after one, match fresh
+    contexts given the same state."""
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(21)
+    sc, C, F = small_scene(rng, n=4096, material=0)
+    E = sc.E_particles
+
+    def run(calls, graph=False):
+        ctx = MPMContext(sc.params(impl=1))
+        ctx.set_graph(graph)
+        ctx.set_particles(sc.x, sc.v, C, F, E=E)
+        for k in calls:
+            ctx.step(k)
+        out = {k: t.cpu().numpy() for k, t in ctx.get_particles().items()}
+        out["gv"] = ctx.get_grid()["velocity"].cpu().numpy()
+        ctx.close()
+        return out
+
+    want = run([6])
+    for calls, graph in (([1] * 6, False), ([1] * 6, True), ([2, 1, 3], True)):
+        print("calls", calls, "graph", graph, flush=True)
+        got = run(calls, graph)
+        for key in ("x", "v", "C", "F", "gv"):
+            assert rel(got[key].reshape(-1), want[key].reshape(-1)) <= TOL_STEP, (calls, graph, key)
+
+    # new particles while a step is pending
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, C, F, E=E)
+    ctx.step(1)
+    x2 = (0.3 + 0.4 * rng.random((3000, 3))).astype(np.float32)
+    v2 = rng.normal(0, 0.5, (3000, 3)).astype(np.float32)
+    ctx.set_particles(x2, v2)
+    ctx.step(3)
+    got = {k: t.cpu().numpy() for k, t in ctx.get_particles().items()}
+    ref = MPMContext(sc.params(impl=1))
+    ref.set_particles(x2, v2)
+    ref.step(3)
+    want = {k: t.cpu().numpy() for k, t in ref.get_particles().items()}
+    ref.close()
+    for key in ("x", "v", "C", "F"):
+        assert rel(got[key].reshape(-1), want[key].reshape(-1)) <= TOL_STEP, ("new particles", key)
+
+    # a checkpointed rollout and its backward after a pending step
+    ctx.step(1)
+    s0 = {k: t.cpu().numpy() for k, t in ctx.get_particles().items()}
+    ctx.forward_checkpointed(4, 2)
+    # random weights on the final positions: with equal weights (the centre of
+    # mass) the C and E gradients vanish up to rounding, which compares nothing
+    seed = torch.tensor(rng.normal(0, 1.0 / 3000, (3000, 3)), dtype=torch.float32, device="cuda")
+    g1 = {k: t.cpu().numpy() for k, t in ctx.backward(seed).items() if t is not None}
+    f1 = {k: t.cpu().numpy() for k, t in ctx.get_particles().items()}
+    ctx.close()
+    ref = MPMContext(sc.params(impl=1))
+    ref.set_particles(s0["x"], s0["v"], s0["C"].reshape(-1, 3, 3), s0["F"].reshape(-1, 3, 3))
+    ref.forward_checkpointed(4, 2)
+    g2 = {k: t.cpu().numpy() for k, t in ref.backward(seed).items() if t is not None}
+    f2 = {k: t.cpu().numpy() for k, t in ref.get_particles().items()}
+    ref.close()
+    for key in ("x", "v", "C", "F"):
+        assert rel(f1[key].reshape(-1), f2[key].reshape(-1)) <= TOL_STEP, ("rollout", key)
+    for key in g2:
+        assert rel(g1[key].reshape(-1), g2[key].reshape(-1)) <= 1e-4, ("backward", key)
+
+
 def test_error_vs_fp64_after_n_steps(gpu, oracle_lib):
     """Single-box workload (64^3, 32,768 particles, fixed-corotated): the FP32
     tiled GPU trajectory against the FP64 oracle after 50 steps."""
