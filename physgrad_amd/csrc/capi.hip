@@ -61,8 +61,7 @@ struct pg_mpm_ctx {
     bool primed = false;
     struct GraphEntry { int nsteps; const void* S; bool primed; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
-    hipStream_t gstream = nullptr;
-    hipEvent_t gev[2] = {nullptr, nullptr};
+    hipStream_t gstream = nullptr;   // capture stream for callers on the NULL stream
     float4 *abar = nullptr, *part = nullptr, *agv = nullptr, *apam = nullptr;
     float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
     float *aEs = nullptr, *aEp = nullptr;   // dL/dE by storage slot through a segment, and its carry
@@ -286,24 +285,19 @@ hipError_t settle(pg_mpm_ctx* c, hipStream_t st) {
     return pg::tiled_settle(c->tw, c->n, c->mp, c->gv, st);
 }
 
-// Runs nsteps tiled steps as one hipGraph launch.  The step sequence of a
-// given length starting from a given state buffer is captured once and then
-// replayed; the host-side ping-pong of the state buffers is applied after
-// each launch exactly as the captured sequence did it.  A NULL (legacy)
-// stream cannot be captured, so the work then runs on an internal stream
-// ordered against it with events.
-hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
+// Runs nsteps tiled steps as one hipGraph launch on the caller's stream.  The
+// step sequence of a given length starting from a given state buffer (and
+// from pending work or not) is captured once and then replayed; the host-side
+// ping-pong of the state buffers is applied after each launch exactly as the
+// captured sequence did it.  A NULL (legacy) stream cannot be captured, so
+// the capture then records on an internal stream (capture runs nothing); the
+// graph is launched on the caller's stream either way.
+hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t st) {
     hipError_t e = hipSuccess;
-    hipStream_t st = user_st;
-    if (!st) {
-        if (!c->gstream) {
-            if ((e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking)) != hipSuccess) return e;
-            if ((e = hipEventCreateWithFlags(&c->gev[0], hipEventDisableTiming)) != hipSuccess) return e;
-            if ((e = hipEventCreateWithFlags(&c->gev[1], hipEventDisableTiming)) != hipSuccess) return e;
-        }
-        if ((e = hipEventRecord(c->gev[0], nullptr)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(c->gstream, c->gev[0], 0)) != hipSuccess) return e;
-        st = c->gstream;
+    hipStream_t cap_st = st;
+    if (!cap_st) {
+        if (!c->gstream && (e = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking)) != hipSuccess) return e;
+        cap_st = c->gstream;
     }
     hipGraphExec_t exec = nullptr;
     for (auto& ge : c->graphs)
@@ -318,11 +312,11 @@ hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
         float* m = c->have_mass ? c->mass : nullptr;
         float* m2 = c->have_mass ? c->mass2 : nullptr;
         hipGraph_t graph = nullptr;
-        e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+        e = hipStreamBeginCapture(cap_st, hipStreamCaptureModeRelaxed);
         if (e == hipSuccess) {
             hipError_t ek = pg::tiled_steps(c->tw, c->n, c->cap, S, S2, pid, pid2, E, E2, m, m2, c->mp, c->gv,
-                                            nsteps, st, c->primed, true);
-            e = hipStreamEndCapture(st, &graph);
+                                            nsteps, cap_st, c->primed, true);
+            e = hipStreamEndCapture(cap_st, &graph);
             if (ek != hipSuccess) e = ek;
         }
         c->prof.on = prof;
@@ -338,10 +332,6 @@ hipError_t graph_steps(pg_mpm_ctx* c, int nsteps, hipStream_t user_st) {
         std::swap(c->pid, c->pid2);
         if (c->have_E) std::swap(c->E, c->E2);
         if (c->have_mass) std::swap(c->mass, c->mass2);
-    }
-    if (!user_st) {
-        if ((e = hipEventRecord(c->gev[1], st)) != hipSuccess) return e;
-        e = hipStreamWaitEvent(nullptr, c->gev[1], 0);
     }
     return e;
 }
@@ -440,8 +430,6 @@ int pg_mpm_destroy(pg_mpm_ctx* c) {
     for (void* p : c->tape_allocs) hipFree(p);
     for (auto& ge : c->graphs) hipGraphExecDestroy(ge.exec);
     if (c->gstream) hipStreamDestroy(c->gstream);
-    for (hipEvent_t ev : c->gev)
-        if (ev) hipEventDestroy(ev);
     delete c;
     return PG_OK;
 }
