@@ -249,6 +249,40 @@ def test_one_step_calls_match_one_call(gpu):
         assert rel(g1[key].reshape(-1), g2[key].reshape(-1)) <= 1e-4, ("backward", key)
 
 
+def test_autotune_after_a_pending_step(gpu):
+    """Autotuning right after a step (whose next P2G is pending) leaves the
+    particles and the last step's grid as they were, and later steps match a
+    context that was never tuned."""
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(23)
+    sc, C, F = small_scene(rng, n=4096, material=1)
+
+    def snap(ctx):
+        out = {k: t.cpu().numpy() for k, t in ctx.get_particles().items()}
+        out["gv"] = ctx.get_grid()["velocity"].cpu().numpy()
+        return out
+
+    a = MPMContext(sc.params(impl=1))
+    a.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
+    a.step(2)
+    before = snap(a)
+    a.autotune(trial_steps=3, use_cache=False)
+    after = snap(a)
+    for key in before:
+        assert np.array_equal(before[key], after[key]), ("autotune changed", key)
+    a.step(3)
+    got = snap(a)
+    a.close()
+    b = MPMContext(sc.params(impl=1))
+    b.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
+    b.step(2)
+    b.step(3)
+    want = snap(b)
+    b.close()
+    for key in want:
+        assert rel(got[key].reshape(-1), want[key].reshape(-1)) <= TOL_STEP, ("after autotune", key)
+
+
 def test_error_vs_fp64_after_n_steps(gpu, oracle_lib):
     """Single-box workload (64^3, 32,768 particles, fixed-corotated): the FP32
     tiled GPU trajectory against the FP64 oracle after 50 steps."""
