@@ -135,6 +135,23 @@ int pg_mpm_backward(pg_mpm_ctx* ctx, const float* ax, const float* av, const flo
 int pg_mpm_set_profiling(pg_mpm_ctx* ctx, int on);
 int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long* launches);
 
+/* State validation.  The reference steps on whatever its particles hold and
+ * has no failure detection on this path (MPMSolver<T>::step,
+ * src/mpm_solver.h:228-285); these calls add it.
+ * pg_mpm_check_state scans the particle state: *first_nonfinite receives the
+ * lowest (caller-order) index of a particle whose x, v, C or F is not finite
+ * and *first_outside the lowest index of a particle outside its grid box
+ * [origin, origin + (n-1) dx] per axis (-1: none; either pointer may be NULL).
+ * It also reads the tiled path's binning error word, which the binning kernels
+ * set instead of writing past a buffer.  Returns PG_ERR_STATE, with a message
+ * naming the first bad particle, for non-finite state or a binning error;
+ * particles outside the grid are reported but are not an error (the tiled
+ * path clamps their tile and skips out-of-grid stencil nodes).  Synchronises
+ * the stream.  pg_mpm_set_validation(ctx, 1) makes every pg_mpm_step call end
+ * with this check (and so synchronise). */
+int pg_mpm_check_state(pg_mpm_ctx* ctx, int* first_nonfinite, int* first_outside, void* stream);
+int pg_mpm_set_validation(pg_mpm_ctx* ctx, int on);
+
 /* Batched independent scenes in one context (tiled MLS-MPM).  The reference
  * runs one scene per MPMSolver (src/mpm_solver.h:150-171, one grid and one
  * particle AoSoA each); a batch concatenates the scenes' particles and grids
@@ -159,6 +176,8 @@ typedef struct pg_scene_desc {
     int max_particles;        /* this scene's capacity */
 } pg_scene_desc;
 
+/* Tiled contexts (single scene or batch) take grids of at most 256 nodes per
+ * axis and return PG_ERR_INVALID for larger ones. */
 int pg_mpm_batch_create(const pg_scene_desc* scenes, int nscenes, pg_mpm_ctx** out);
 /* counts[s] particles of scene s, scenes back to back in the arrays (same
  * layouts as pg_mpm_set_particles); mass NULL uses each scene's

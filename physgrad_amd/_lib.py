@@ -78,6 +78,8 @@ _SIGNATURES = {
     "pg_mpm_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_mpm_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_longlong)]),
+    "pg_mpm_check_state": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _vp]),
+    "pg_mpm_set_validation": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(PgSceneDesc), ctypes.c_int, ctypes.POINTER(_vp)]),
     "pg_mpm_batch_set_particles": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "pg_mpm_num_scenes": (ctypes.c_int, [_vp]),

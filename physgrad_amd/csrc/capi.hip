@@ -66,6 +66,8 @@ struct pg_mpm_ctx {
     float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
     float *aEs = nullptr, *aEp = nullptr;   // dL/dE by storage slot through a segment, and its carry
     uint32_t* adst = nullptr;               // storage slot -> output position of a reverse step
+    uint32_t* check = nullptr;              // [4] results of the state validation kernel
+    bool validate = false;                  // pg_mpm_step validates the state after stepping
 };
 
 namespace {
@@ -145,6 +147,7 @@ const char* check_scene(const pg_scene_desc& d) {
     if (!(d.particle_volume > 0.f)) return "particle_volume must be > 0";
     if (d.boundary_thickness < 0) return "boundary_thickness must be >= 0";
     if (d.max_particles < 0 || d.max_particles > (1 << 30)) return "max_particles out of range";
+    if (std::max(d.nx, std::max(d.ny, d.nz)) > pg::MAX_TILED_NODES) return "more than 256 nodes per axis";
     return nullptr;
 }
 
@@ -227,32 +230,39 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, 2 * toff)) ||
                    (rc = dalloc(c, &w.tscene, toff)) || (rc = dalloc(c, &w.bscene, boff)) ||
                    (rc = dalloc(c, &w.bflag, toff)) || (rc = dalloc(c, &w.blist, toff)) ||
-                   (rc = dalloc(c, &w.nblist, 1)))))
+                   (rc = dalloc(c, &w.nblist, 1)) || (rc = dalloc(c, &w.err, 1)))))
         return rc;
     if (tiled) {
         // P2G window buffers: 8^3 nodes (8 KiB) for every possible active-list entry
         if ((rc = dalloc(c, &w.wbuf, (size_t)max_entries * 512))) return rc;
     }
+    if ((rc = dalloc(c, &c->check, 4))) return rc;
     w.scenes = c->scenes_d;
-    hipMemset(c->mp, 0, goff * sizeof(float4));
-    hipMemset(c->gv, 0, goff * sizeof(float4));
+    hipError_t e = hipSuccess;
+    auto zero = [&](void* p, size_t bytes) {
+        if (e == hipSuccess) e = hipMemset(p, 0, bytes);
+    };
+    zero(c->mp, goff * sizeof(float4));
+    zero(c->gv, goff * sizeof(float4));
     if (tiled) {
-        hipMemset(w.count, 0, toff * 4);
-        hipMemset(w.flags, 0, boff * 4);
-        hipMemset(w.nactive, 0, 4);
-        hipMemset(w.bflag, 0, toff * 4);
-        hipMemset(w.nblist, 0, 4);
-        hipMemset(w.tslot, 0, toff * 8);
+        zero(w.count, toff * 4);
+        zero(w.flags, boff * 4);
+        zero(w.nactive, 4);
+        zero(w.bflag, toff * 4);
+        zero(w.nblist, 4);
+        zero(w.err, 4);
+        zero(w.tslot, toff * 8);
         // scene of every tile slot and grid block (the kernels look scenes up by these)
         std::vector<uint32_t> ts(toff), bs(boff);
         for (int s = 0; s < ns; ++s) {
             std::fill(ts.begin() + S[s].toff, ts.begin() + S[s].toff + S[s].tslots, (uint32_t)s);
             std::fill(bs.begin() + S[s].boff, bs.begin() + S[s].boff + S[s].nblk, (uint32_t)s);
         }
-        hipMemcpy(w.tscene, ts.data(), toff * 4, hipMemcpyHostToDevice);
-        hipMemcpy(w.bscene, bs.data(), boff * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w.tscene, ts.data(), toff * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(w.bscene, bs.data(), boff * 4, hipMemcpyHostToDevice);
     }
-    hipError_t e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "context initialisation");
+    e = hipMemcpy(c->scenes_d, S.data(), ns * sizeof(pg::SceneInfo), hipMemcpyHostToDevice);
     return e == hipSuccess ? PG_OK : hip_fail(c, e, "scene table upload");
 }
 
@@ -387,6 +397,9 @@ int pg_mpm_create(const pg_mpm_params* p, pg_mpm_ctx** out) {
             return fail(nullptr, PG_ERR_INVALID, "poisson_ratio must be in (-1, 0.5)");
         if (!(p->particle_volume > 0.f)) return fail(nullptr, PG_ERR_INVALID, "particle_volume must be > 0");
         if (p->boundary_thickness < 0) return fail(nullptr, PG_ERR_INVALID, "boundary_thickness must be >= 0");
+        if (p->impl == PG_IMPL_TILED && std::max(p->nx, std::max(p->ny, p->nz)) > pg::MAX_TILED_NODES)
+            return fail(nullptr, PG_ERR_INVALID, "the tiled path takes at most " +
+                                                     std::to_string(pg::MAX_TILED_NODES) + " nodes per axis");
     }
     pg_mpm_ctx* c = new (std::nothrow) pg_mpm_ctx();
     if (!c) return fail(nullptr, PG_ERR_NOMEM, "host allocation failed");
@@ -403,9 +416,18 @@ int pg_mpm_create(const pg_mpm_params* p, pg_mpm_ctx** out) {
             pg_mpm_destroy(c);
             return fail(nullptr, rc, m);
         }
-        hipMemset(c->rgm, 0, G * 4);
-        hipMemset(c->rgp, 0, 3 * G * 4);
-        hipMemset(c->rgv, 0, 3 * G * 4);
+        hipError_t e = hipMemset(c->rgm, 0, G * 4);
+        if (e == hipSuccess) e = hipMemset(c->rgp, 0, 3 * G * 4);
+        if (e == hipSuccess) e = hipMemset(c->rgv, 0, 3 * G * 4);
+        if (e == hipSuccess && (rc = dalloc(c, &c->check, 4))) {
+            std::string m = c->err;
+            pg_mpm_destroy(c);
+            return fail(nullptr, rc, m);
+        }
+        if (e != hipSuccess) {
+            pg_mpm_destroy(c);
+            return hip_fail(nullptr, e, "grid initialisation");
+        }
     } else {
         pg_scene_desc d = scene_from_params(*p);
         if ((rc = init_mls(c, &d, 1))) {
@@ -577,6 +599,14 @@ int pg_mpm_get_particles(pg_mpm_ctx* c, float* x, float* v, float* C, float* F, 
     return PG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+int check_state_impl(pg_mpm_ctx* c, int* first_nonfinite, int* first_outside, hipStream_t st);
+}  // namespace
+
+extern "C" {
+
 int pg_mpm_get_grid(pg_mpm_ctx* c, float* mass, float* momentum, float* velocity, int memory, void* stream) {
     if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
     if (memory != PG_HOST && memory != PG_DEVICE) return fail(c, PG_ERR_INVALID, "bad memory kind");
@@ -654,6 +684,7 @@ int pg_mpm_step(pg_mpm_ctx* c, int nsteps, void* stream) {
     c->time += (double)c->p.dt * nsteps;
     c->steps += nsteps;
     c->version++;
+    if (c->validate) return check_state_impl(c, nullptr, nullptr, st);
     return PG_OK;
 }
 
@@ -672,6 +703,66 @@ int pg_mpm_kernel_time(pg_mpm_ctx* c, int kernel, double* total_ms, long long* l
     if (e != hipSuccess) return hip_fail(c, e, "profiling flush");
     if (total_ms) *total_ms = c->prof.total_ms[kernel];
     if (launches) *launches = c->prof.count[kernel];
+    return PG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Runs the validation kernel and reads the binning error word; fills the
+// message and returns PG_ERR_STATE when the state holds non-finite values or
+// a binning index was out of bounds.  Synchronises the stream.
+int check_state_impl(pg_mpm_ctx* c, int* first_nonfinite, int* first_outside, hipStream_t st) {
+    if (first_nonfinite) *first_nonfinite = -1;
+    if (first_outside) *first_outside = -1;
+    // reads the state and the error word only: work a tiled step left pending
+    // for the next one stays pending
+    hipError_t e;
+    if (c->p.mode == PG_MODE_REF_COMPAT) {
+        const float lo[3] = {c->p.ox, c->p.oy, c->p.oz};
+        const float hi[3] = {c->p.ox + (c->p.nx - 1) * c->p.dx, c->p.oy + (c->p.ny - 1) * c->p.dy,
+                             c->p.oz + (c->p.nz - 1) * c->p.dz};
+        e = pg::check_state_ref(c->n, c->rx, c->rv, lo, hi, c->check, st);
+    } else {
+        e = pg::check_state_mls(c->n, c->cap, c->S, c->p.impl == PG_IMPL_TILED ? c->pid : nullptr, c->scenes_d,
+                                c->nscenes, c->check, st);
+    }
+    uint32_t out[4] = {0, 0, 0, 0}, berr = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c->check, sizeof(out), hipMemcpyDeviceToHost, st);
+    const bool tiled = c->p.mode == PG_MODE_MLS_MPM && c->p.impl == PG_IMPL_TILED;
+    if (e == hipSuccess && tiled) e = hipMemcpyAsync(&berr, c->tw.err, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && tiled && berr) e = hipMemsetAsync(c->tw.err, 0, 4, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_check_state");
+    if (first_nonfinite && out[1]) *first_nonfinite = (int)out[0];
+    if (first_outside && out[3]) *first_outside = (int)out[2];
+    std::string msg;
+    if (out[1])
+        msg = std::to_string(out[1]) + " particle(s) with a non-finite state; the first is particle " +
+              std::to_string(out[0]);
+    if (berr) {
+        if (!msg.empty()) msg += "; ";
+        msg += "binning index out of bounds (error bits " + std::to_string(berr) + ": 1 active list, "
+               "2 sorted positions, 4 node blocks)";
+    }
+    if (!msg.empty()) return fail(c, PG_ERR_STATE, msg);
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_mpm_check_state(pg_mpm_ctx* c, int* first_nonfinite, int* first_outside, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    return check_state_impl(c, first_nonfinite, first_outside, (hipStream_t)stream);
+}
+
+int pg_mpm_set_validation(pg_mpm_ctx* c, int on) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    c->validate = on != 0;
     return PG_OK;
 }
 

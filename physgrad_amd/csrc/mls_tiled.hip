@@ -73,14 +73,24 @@ struct TileArgs {
     float4* wbuf;            // window sums by active-list index
     const SceneInfo* scenes;
     size_t cap;
+    uint32_t nslots;         // tile slots (capacity of bflag and blist)
+    uint32_t* err;           // binning error word (ERR_* bits)
 };
 
 
 // marks 4^3 node block B (tile slot) for the grid update; the first mark of a
 // step appends it to the block list
 __device__ __forceinline__ void flag_block(uint32_t* bflag, uint32_t* blist, uint32_t* nblist, uint32_t B,
-                                           uint32_t bit) {
-    if (atomicOr(bflag + B, bit) == 0u) blist[atomicAdd(nblist, 1u)] = B;
+                                           uint32_t bit, uint32_t nslots, uint32_t* err) {
+    if (B >= nslots) {   // a block id past the slot space: report, never write
+        atomicOr(err, ERR_BLOCK);
+        return;
+    }
+    if (atomicOr(bflag + B, bit) == 0u) {
+        const uint32_t i = atomicAdd(nblist, 1u);
+        if (i < nslots) blist[i] = B;
+        else atomicOr(err, ERR_BLOCK);
+    }
 }
 
 // tile coordinates of a position, clamped like tile_key
@@ -206,8 +216,8 @@ __device__ __forceinline__ void p2g_window_atomic(float* wf, int nb, const Stenc
 // P2G into the global grid with atomics; the node blocks hit are flagged for
 // the grid update.
 __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* bflag, uint32_t* blist,
-                                           uint32_t* nblist, const SceneInfo& sc, const MlsGeom& g,
-                                           const Stencil& st, const P2GVal& p) {
+                                           uint32_t* nblist, uint32_t nslots, uint32_t* err, const SceneInfo& sc,
+                                           const MlsGeom& g, const Stencil& st, const P2GVal& p) {
 #pragma unroll 1
     for (int n = 0; n < 27; ++n) {
         const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
@@ -219,7 +229,7 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* b
         atomic_add_f32(cp + 1, c.y);
         atomic_add_f32(cp + 2, c.z);
         atomic_add_f32(cp + 3, c.w);
-        flag_block(bflag, blist, nblist, sc.toff + morton_tile(gi / TT, gj / TT, gk / TT), 2u);
+        flag_block(bflag, blist, nblist, sc.toff + morton_tile(gi / TT, gj / TT, gk / TT), 2u, nslots, err);
     }
 }
 
@@ -456,7 +466,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 } else if (near) {
                     p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st, p);
                 } else {
-                    p2g_global(mps, a.bflag, a.blist, a.nblist, sc, g, st, p);
+                    p2g_global(mps, a.bflag, a.blist, a.nblist, a.nslots, a.err, sc, g, st, p);
                 }
                 lds_order();
             };
@@ -618,7 +628,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                         p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st,
                                           p);
                     else
-                        p2g_global(mps, a.bflag, a.blist, a.nblist, sc, g, st, p);
+                        p2g_global(mps, a.bflag, a.blist, a.nblist, a.nslots, a.err, sc, g, st, p);
                 }
                 for (uint32_t jj = tid; jj < pc; jj += TB) {
                     const uint16_t dr = ldir[jj];
@@ -654,7 +664,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 // node blocks this window covers: the tile's and its 26 neighbours
                 const int bx = tc[0] + tid % 3 - 1, by = tc[1] + (tid / 3) % 3 - 1, bz = tc[2] + tid / 9 - 1;
                 if (bx >= 0 && by >= 0 && bz >= 0 && bx < sc.ntx && by < sc.nty && bz < sc.ntz)
-                    flag_block(a.bflag, a.blist, a.nblist, sc.toff + morton_tile(bx, by, bz), 1u);
+                    flag_block(a.bflag, a.blist, a.nblist, sc.toff + morton_tile(bx, by, bz), 1u, a.nslots, a.err);
             }
         }
         __syncthreads();
@@ -890,7 +900,8 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ start,
                                                    uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
                                                    uint2* __restrict__ tslot, const uint32_t* __restrict__ tscene,
-                                                   uint32_t* __restrict__ nblist) {
+                                                   uint32_t* __restrict__ nblist, uint32_t max_entries,
+                                                   uint32_t* __restrict__ err) {
     __shared__ uint32_t sh[SCAN_T];
     // the grid update of the last binning has read the block list: empty it
     if (blockIdx.x == 0 && threadIdx.x == 0) *nblist = 0u;
@@ -922,22 +933,34 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
             tslot[base + k] = make_uint2(v[k] ? pos : 0xFFFFFFFFu, ne);
             if (v[k]) {
                 const uint32_t sc = tscene[base + k];
-                for (uint32_t c = 0; c < ne; ++c)
-                    alist[pos++] = make_uint4((uint32_t)(base + k), run + c * CHUNK, min((uint32_t)CHUNK, v[k] - c * CHUNK),
-                                              (uint32_t)sc);
+                for (uint32_t c = 0; c < ne; ++c, ++pos) {
+                    if (pos < max_entries)
+                        alist[pos] = make_uint4((uint32_t)(base + k), run + c * CHUNK,
+                                                min((uint32_t)CHUNK, v[k] - c * CHUNK), (uint32_t)sc);
+                    else
+                        atomicOr(err, ERR_ALIST);
+                }
             }
             run += v[k];
             count[base + k] = 0u;
         }
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *nactive = pre_a + ta;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        // an overflowing list is cut at its capacity (and reported), so that no
+        // kernel reads past it
+        *nactive = min(pre_a + ta, max_entries);
+    }
 }
 
 __global__ void k_scatter(int n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
-                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm) {
+                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm, uint32_t nslots,
+                          uint32_t* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    perm[start[key[s]] + rank[s]] = (uint32_t)s;
+    const uint32_t k = key[s];
+    const uint32_t d = k < nslots ? start[k] + rank[s] : 0xFFFFFFFFu;
+    if (d < (uint32_t)n) perm[d] = (uint32_t)s;
+    else atomicOr(err, ERR_PERM);
 }
 
 // binning after the first scan pass (k_scan_a or k_grid_and_count)
@@ -946,8 +969,9 @@ hipError_t bin_rest(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     ProfScope ps(w.prof, PK_BIN, st);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist);
-    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
+                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist,
+                                    (uint32_t)w.max_entries, w.err);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm, w.tile_slots, w.err);
     return hipGetLastError();
 }
 
@@ -957,8 +981,9 @@ hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
-                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist);
-    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm);
+                                    nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist,
+                                    (uint32_t)w.max_entries, w.err);
+    if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm, w.tile_slots, w.err);
     return hipGetLastError();
 }
 
@@ -977,6 +1002,8 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.wbuf = w.wbuf;
     a.scenes = w.scenes;
     a.cap = cap;
+    a.nslots = (uint32_t)w.tile_slots;
+    a.err = w.err;
 }
 
 // workgroups of the grid update (4 waves each, striding over the block list):

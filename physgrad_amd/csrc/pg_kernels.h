@@ -118,6 +118,13 @@ struct SceneInfo {
 // of serialising the step on one.
 constexpr int CHUNK = 1280;
 
+// Largest grid side of the tiled path (64 tiles of 4 cells per axis).  One
+// step on a 264^3 grid (66 tiles per axis, 2^21 tile slots) raised an illegal
+// memory access on MI355X with an earlier version of the window buffers whose
+// cause was not found; until such grids have a passing test the tiled path
+// refuses them at context creation.
+constexpr int MAX_TILED_NODES = 256;
+
 struct TiledWork {
     KernelProf* prof;      // optional per-kernel timing
     const SceneInfo* scenes;  // device scene table
@@ -143,7 +150,28 @@ struct TiledWork {
     float4* wbuf;          // [max_entries][512] P2G window sums by active-list entry
     int max_entries;       // capacity of the active list
     int nblocks_launch;    // persistent grid size of the tile kernels
+    uint32_t* err;         // [1] binning error word: ERR_* bits set by guarded writes, never cleared by a step
 };
+
+// Bits of the binning error word.  The binning kernels check every index they
+// form against its buffer and, instead of writing out of bounds, set a bit;
+// pg_mpm_check_state reports and clears them.
+enum BinningError : uint32_t {
+    ERR_ALIST = 1u,   // more active-list entries than max_entries
+    ERR_PERM = 2u,    // a sorted position outside [0, n) (tile counts out of step with the keys)
+    ERR_BLOCK = 4u,   // a node block id outside the tile slots, or a full block list
+};
+
+// Validation of the particle state: out[0] = lowest user index of a particle
+// with a non-finite x, v, C or F (0xFFFFFFFF: none), out[1] = their number,
+// out[2] = lowest user index of a particle outside its scene's grid box,
+// out[3] = their number.  S: the six planes (pid: storage slot -> user index,
+// NULL = identity); out must hold 4 words and is written on the stream.
+hipError_t check_state_mls(int n, size_t cap, const float4* S, const uint32_t* pid, const SceneInfo* scenes,
+                           int nscenes, uint32_t* out, hipStream_t st);
+// the same for the ref-compat layout (x, v interleaved [3n]), one grid box
+hipError_t check_state_ref(int n, const float* x, const float* v, const float* lo, const float* hi, uint32_t* out,
+                           hipStream_t st);
 
 // One step of a segment tape: the step's input state in its storage order, its
 // binning, and its grid (valid on the flagged 8^3 blocks).

@@ -61,12 +61,21 @@ __device__ __forceinline__ void tile_origin(uint32_t tile, int* t0) {
 
 // Workgroups are dispatched round-robin over the 8 XCDs; each XCD takes a
 // contiguous chunk of the (Morton-ordered) tile list so that tiles sharing
-// window nodes share an L2.
+// window nodes share an L2.  Workgroup b runs on XCD b mod 8, which holds
+// ceil((gridDim - xcd) / 8) workgroups; they stride over the XCD's chunk by
+// that count, so every entry is taken for any grid size (with fewer than 8
+// workgroups, all of them stride over the whole list instead).
 struct XcdChunk {
     uint32_t first, end, stride;
     __device__ __forceinline__ XcdChunk(uint32_t nact) {
+        if (gridDim.x < 8u) {
+            first = blockIdx.x;
+            end = nact;
+            stride = gridDim.x;
+            return;
+        }
         const uint32_t xcd = blockIdx.x % 8u;
-        stride = (gridDim.x + 7u) / 8u;
+        stride = (gridDim.x - xcd + 7u) / 8u;
         const uint32_t lane = blockIdx.x / 8u;
         const uint32_t c0 = (uint32_t)(((uint64_t)nact * xcd) / 8u);
         end = (uint32_t)(((uint64_t)nact * (xcd + 1)) / 8u);

@@ -174,6 +174,18 @@ class MPMContext:
     def step(self, nsteps: int = 1):
         check(self._L.pg_mpm_step(self._h, int(nsteps), _stream_ptr()), self._h)
 
+    def check_state(self):
+        """Validates the particle state; returns (first_nonfinite, first_outside),
+        -1 for none.  Raises PgError (PG_ERR_STATE) for a non-finite state or a
+        binning error, with the first bad particle named in the message."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(self._L.pg_mpm_check_state(self._h, ctypes.byref(a), ctypes.byref(b), _stream_ptr()), self._h)
+        return a.value, b.value
+
+    def set_validation(self, on: bool = True):
+        """When on, every step() ends with check_state() (and synchronises)."""
+        check(self._L.pg_mpm_set_validation(self._h, 1 if on else 0), self._h)
+
     def forward_checkpointed(self, nsteps: int, checkpoint_every: int = 32):
         """Advance nsteps steps keeping a checkpoint every `checkpoint_every` steps."""
         check(self._L.pg_mpm_forward_checkpointed(self._h, int(nsteps), int(checkpoint_every), _stream_ptr()),

@@ -86,3 +86,25 @@ def test_spatial_hash_argument_checks():
     assert L.pg_spatial_hash_build(None, 0, 0.0, lo, res, None, None, None, None, None, 0, None) == \
         _lib.PG_ERR_INVALID
     assert L.pg_spatial_hash_workspace_size(1000) >= 3 * 4000
+
+
+@pytest.mark.parametrize("side", [257, 264])
+def test_tiled_path_rejects_grids_over_256_nodes_per_axis(side):
+    """The tiled path refuses more than 64 tiles (256 nodes) per axis before
+    touching HIP (single-scene and batch contexts)."""
+    from physgrad_amd import _lib
+    L = _lib.load()
+    p = _lib.PgMpmParams()
+    L.pg_mpm_default_params(ctypes.byref(p))
+    p.mode, p.impl = _lib.MODE_MLS_MPM, _lib.IMPL_TILED
+    p.nx, p.ny, p.nz = 64, side, 64
+    p.dx = p.dy = p.dz = 1.0 / side
+    h = ctypes.c_void_p()
+    assert L.pg_mpm_create(ctypes.byref(p), ctypes.byref(h)) == _lib.PG_ERR_INVALID
+    assert b"256 nodes per axis" in L.pg_mpm_last_error(None)
+    d = _lib.PgSceneDesc()
+    d.nx, d.ny, d.nz, d.dx, d.dt = side, 32, 32, 1.0 / side, 1e-4
+    d.poisson_ratio, d.particle_volume, d.max_particles = 0.3, 1e-6, 16
+    arr = (_lib.PgSceneDesc * 1)(d)
+    assert L.pg_mpm_batch_create(arr, 1, ctypes.byref(h)) == _lib.PG_ERR_INVALID
+    assert b"256 nodes per axis" in L.pg_mpm_last_error(None)
