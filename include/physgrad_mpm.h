@@ -252,6 +252,21 @@ int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const f
                           const unsigned* res, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
                           unsigned* cell_count, void* workspace, size_t workspace_bytes, void* stream);
 
+/* pg_spatial_hash_build without a workspace argument and without any device
+ * allocation: no counterpart in the reference.  It is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The radix sort's workspace is a
+ * library-owned device array of 12.25 MiB per GPU, held until the process
+ * ends, sized for at most PG_SPATIAL_HASH_NOALLOC_MAX particles; larger n
+ * returns PG_ERR_INVALID.  Same arguments (minus the workspace), results and
+ * errors as pg_spatial_hash_build.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (concurrent streams or host threads)
+ * race and must be serialised by the caller. */
+#define PG_SPATIAL_HASH_NOALLOC_MAX (1u << 20)
+int pg_spatial_hash_build_noalloc(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                                  const unsigned* res, unsigned* cell_ids, unsigned* particle_indices,
+                                  unsigned* cell_start, unsigned* cell_count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

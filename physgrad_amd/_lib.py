@@ -97,7 +97,11 @@ _SIGNATURES = {
     "pg_spatial_hash_workspace_size": (ctypes.c_size_t, [ctypes.c_uint]),
     "pg_spatial_hash_build": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp,
                                              _vp, ctypes.c_size_t, _vp]),
+    "pg_spatial_hash_build_noalloc": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, _vp, _vp, _vp, _vp, _vp,
+                                                     _vp, _vp]),
 }
+
+SPATIAL_HASH_NOALLOC_MAX = 1 << 20
 
 _lib = None
 

@@ -20,6 +20,8 @@ thread_local std::string g_thread_error;
 
 }  // namespace
 
+void pg::set_thread_error(const char* msg) { g_thread_error = msg; }
+
 struct pg_mpm_ctx {
     pg_mpm_params p{};
     std::string err;

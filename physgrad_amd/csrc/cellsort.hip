@@ -13,7 +13,10 @@
 // also defines start for empty cells as the reference's prefix sum does.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
+#include <string>
 #include "physgrad_mpm.h"
+#include "pg_kernels.h"
 
 namespace pg {
 namespace {
@@ -159,20 +162,36 @@ extern "C" size_t pg_spatial_hash_workspace_size(unsigned n) {
     return 3 * a + (((size_t)pg::RADIX * nb * 4 + 255) & ~(size_t)255);
 }
 
-extern "C" int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min,
-                                     const unsigned* res, unsigned* cell_ids, unsigned* particle_indices,
-                                     unsigned* cell_start, unsigned* cell_count, void* workspace,
-                                     size_t workspace_bytes, void* stream) {
-    using namespace pg;
-    if (!domain_min || !res || !(cell_size > 0.f)) return PG_ERR_INVALID;
-    if (res[0] == 0 || res[1] == 0 || res[2] == 0) return PG_ERR_INVALID;
+namespace pg {
+namespace {
+
+// Arguments of a spatial-hash build; returns the cell count or 0 (and sets the
+// thread's error message) when they are invalid.
+unsigned check_hash_args(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                         const unsigned* res, const unsigned* cell_ids, const unsigned* particle_indices,
+                         const unsigned* cell_start, const unsigned* cell_count) {
+    if (!domain_min || !res || !(cell_size > 0.f)) {
+        set_thread_error("spatial hash: domain_min, res and a cell size > 0 are required");
+        return 0;
+    }
     const uint64_t total64 = (uint64_t)res[0] * res[1] * res[2];
-    if (total64 > 0xFFFFFFFFull) return PG_ERR_INVALID;
-    const unsigned total = (unsigned)total64;
-    if (n > 0 && (!pos || !cell_ids || !particle_indices)) return PG_ERR_INVALID;
-    if (!cell_start || !cell_count) return PG_ERR_INVALID;
-    if (workspace_bytes < pg_spatial_hash_workspace_size(n) || (n > 0 && !workspace)) return PG_ERR_INVALID;
-    hipStream_t st = (hipStream_t)stream;
+    if (total64 == 0 || total64 > 0xFFFFFFFFull) {
+        set_thread_error("spatial hash: the cell count res0*res1*res2 must be in [1, 2^32)");
+        return 0;
+    }
+    if ((n > 0 && (!pos || !cell_ids || !particle_indices)) || !cell_start || !cell_count) {
+        set_thread_error("spatial hash: NULL output or position array");
+        return 0;
+    }
+    return (unsigned)total64;
+}
+
+// The build over a device workspace of pg_spatial_hash_workspace_size(n)
+// bytes at ws.  Every workspace word a launch reads was written by an earlier
+// launch of the same build, so the workspace needs no clearing.
+int hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min, const unsigned* res,
+               unsigned total, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
+               unsigned* cell_count, void* workspace, hipStream_t st) {
     const size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
     uint32_t* kb[2] = {(uint32_t*)workspace, (uint32_t*)((char*)workspace + a)};
     uint32_t* v2 = (uint32_t*)((char*)workspace + 2 * a);
@@ -201,5 +220,98 @@ extern "C" int pg_spatial_hash_build(const float* pos, unsigned n, float cell_si
     }
     k_cell_ranges<<<(total + 255) / 256, 256, 0, st>>>(sorted, n, total, cell_start, cell_count);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? PG_OK : PG_ERR_HIP;
+    if (e != hipSuccess) {
+        set_thread_error((std::string("spatial hash: ") + hipGetErrorString(e)).c_str());
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
+}  // namespace
+}  // namespace pg
+
+extern "C" int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                                     const unsigned* res, unsigned* cell_ids, unsigned* particle_indices,
+                                     unsigned* cell_start, unsigned* cell_count, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+    using namespace pg;
+    const unsigned total =
+        check_hash_args(pos, n, cell_size, domain_min, res, cell_ids, particle_indices, cell_start, cell_count);
+    if (!total) return PG_ERR_INVALID;
+    if (workspace_bytes < pg_spatial_hash_workspace_size(n) || (n > 0 && !workspace)) {
+        set_thread_error("spatial hash: workspace smaller than pg_spatial_hash_workspace_size(n)");
+        return PG_ERR_INVALID;
+    }
+    return hash_build(pos, n, cell_size, domain_min, res, total, cell_ids, particle_indices, cell_start, cell_count,
+                      workspace, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- allocation-free build
+//
+// The workspace of pg_spatial_hash_build_noalloc is this file-scope device
+// array, sized for PG_SPATIAL_HASH_NOALLOC_MAX particles; every device the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` only names the symbol: its device address on each device comes
+// from hipGetSymbolAddress, looked up once per device on the first call with
+// that device current, and only that address is handed to the kernels.
+namespace pg {
+namespace {
+
+constexpr size_t NOALLOC_BYTES = 3 * (((size_t)PG_SPATIAL_HASH_NOALLOC_MAX * 4 + 255) & ~(size_t)255) +
+                                 (((size_t)RADIX * ((PG_SPATIAL_HASH_NOALLOC_MAX + RS_TILE - 1) / RS_TILE) * 4 +
+                                   255) & ~(size_t)255);
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of this device's scratch array, or NULL (message set)
+void* scratch_on_current_device() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        set_thread_error("spatial hash (no-alloc): no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_thread_error((std::string("spatial hash (no-alloc): hipGetSymbolAddress failed: ") +
+                              hipGetErrorString(e)).c_str());
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+}  // namespace pg
+
+extern "C" int pg_spatial_hash_build_noalloc(const float* pos, unsigned n, float cell_size, const float* domain_min,
+                                             const unsigned* res, unsigned* cell_ids, unsigned* particle_indices,
+                                             unsigned* cell_start, unsigned* cell_count, void* stream) {
+    using namespace pg;
+    static_assert(NOALLOC_BYTES >= 256, "scratch holds at least the histogram of one block");
+    if (n > PG_SPATIAL_HASH_NOALLOC_MAX) {
+        set_thread_error(("spatial hash (no-alloc): " + std::to_string(n) + " particles exceed the maximum of " +
+                          std::to_string(PG_SPATIAL_HASH_NOALLOC_MAX) + "; use pg_spatial_hash_build with a workspace")
+                             .c_str());
+        return PG_ERR_INVALID;
+    }
+    const unsigned total =
+        check_hash_args(pos, n, cell_size, domain_min, res, cell_ids, particle_indices, cell_start, cell_count);
+    if (!total) return PG_ERR_INVALID;
+    void* ws = scratch_on_current_device();
+    if (!ws) return PG_ERR_HIP;
+    if (pg_spatial_hash_workspace_size(n) > NOALLOC_BYTES) {
+        set_thread_error("spatial hash (no-alloc): scratch smaller than the workspace");
+        return PG_ERR_INVALID;
+    }
+    return hash_build(pos, n, cell_size, domain_min, res, total, cell_ids, particle_indices, cell_start, cell_count,
+                      ws, (hipStream_t)stream);
 }

@@ -29,6 +29,9 @@ struct MlsGeom {
     float p_mass;    // uniform mass (used when no per-particle mass)
 };
 
+// message of the last failure on this host thread (pg_mpm_last_error(NULL))
+void set_thread_error(const char* msg);
+
 struct KernelProf;
 hipError_t refcompat_step(const RefGeom& g, int n, float* x, float* v, const float* m, float* gm, float* gp, float* gv,
                           float dt, const float* grav, hipStream_t st, KernelProf* prof);
