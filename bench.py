@@ -39,12 +39,14 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-bwd", action="store_true", help="skip the forward+backward and gradient legs")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient-accuracy leg")
-    ap.add_argument("--bwd-steps", type=int, default=64)
-    # all 64 steps on the segment tape (~11 GB at 1M particles): the backward
-    # reverses them without recomputing the forward
-    ap.add_argument("--bwd-ckpt", type=int, default=64)
+    # one scene of the batched forward+backward workload: 256 steps with a
+    # checkpoint every 16, so the backward recomputes 15 of its 16 segments
+    ap.add_argument("--bwd-steps", type=int, default=256)
+    ap.add_argument("--bwd-ckpt", type=int, default=16)
+    ap.add_argument("--plain-bwd-steps", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-batch-fb", action="store_true", help="skip the 16-scene forward+backward leg")
     return ap.parse_args()
 
 
@@ -65,34 +67,128 @@ def time_steps(ctx, steps, warmup):
     return a.elapsed_time(b) / 1e3, wall
 
 
+def com_seed(xT, scenes, counts):
+    """Adjoint seed dL/dx_T of L = sum over scenes of |mean x_T - target|^2
+    (FP64 reduction) and the residuals mean x_T - target."""
+    import torch
+    seed = torch.empty_like(xT)
+    resid = []
+    off = 0
+    for s, c in zip(scenes, counts):
+        d = xT[off:off + c].double().mean(0) - torch.tensor(s.target, dtype=torch.float64, device=xT.device)
+        seed[off:off + c] = (2.0 * d / c).float().expand(c, 3)
+        resid.append(d.cpu().numpy())
+        off += c
+    return seed, resid
+
+
+def closed_form_error(grads, scenes, counts, resid, T):
+    """Largest relative error of dL/dx0 and dL/dv0 against the closed form
+    2 (mean x_T - target) / N (times T dt for v0), valid while no particle
+    reaches a sticky wall: the transfers conserve linear momentum exactly."""
+    import numpy as np
+    worst = 0.0
+    off = 0
+    for s, c, d in zip(scenes, counts, resid):
+        wx = 2.0 * d / c
+        for key, want in (("x", wx), ("v", wx * T * s.dt)):
+            g = grads[key][off:off + c].double()
+            w = torch_tensor(want, g)
+            worst = max(worst, float(((g - w).abs().max() / w.abs().max()).item()))
+        off += c
+    return worst
+
+
+def torch_tensor(a, like):
+    import torch
+    return torch.as_tensor(a, dtype=torch.float64, device=like.device).expand_as(like)
+
+
 def fwd_bwd_leg(scene, steps, ckpt, impl):
-    """Forward with a checkpoint every `ckpt` steps plus the full backward (recompute
-    and reverse of every segment) on the same 1M-particle workload; the adjoint
-    seed is that of the centre-of-mass loss."""
+    """Forward with a checkpoint every `ckpt` steps plus the full backward
+    (recompute of every segment but the last, then its reverse) on one scene
+    of the batched forward+backward workload; the adjoint seed is that of its
+    centre-of-mass loss, and the gradients are checked against the closed
+    form."""
     import torch
     from physgrad_amd.solver import MPMContext
     ctx = MPMContext(scene.params(impl=impl))
-    ctx.set_particles(scene.x, scene.v)
     n = scene.n
-    seed = torch.full((n, 3), 1.0 / n, dtype=torch.float32, device="cuda")
-    ctx.forward_checkpointed(ckpt, ckpt)   # warm-up: allocates checkpoints and the tape
-    ctx.backward(seed)
-    ctx.set_particles(scene.x, scene.v)
+    ctx.set_particles(scene.x, scene.v, E=scene.E_particles)
+    ctx.forward_checkpointed(2 * ckpt, ckpt)   # warm-up: allocates checkpoints and the tape
+    ctx.backward(torch.zeros((n, 3), device="cuda"))
+    ctx.set_particles(scene.x, scene.v, E=scene.E_particles)
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     a.record(s)
     ctx.forward_checkpointed(steps, ckpt)
     b.record(s)
-    ctx.backward(seed)
+    seed, resid = com_seed(ctx.get_particles()["x"], [scene], [n])
+    b2 = torch.cuda.Event(enable_timing=True)
+    b2.record(s)
+    grads = ctx.backward(seed)
     c.record(s)
     torch.cuda.synchronize()
-    t_f, t_b = a.elapsed_time(b) / 1e3, b.elapsed_time(c) / 1e3
+    t_f, t_b = a.elapsed_time(b) / 1e3, b2.elapsed_time(c) / 1e3
+    err = closed_form_error(grads, [scene], [n], resid, steps)
+    active = int((ctx.get_grid()["mass"] > 0).sum().item())
     ctx.close()
-    return {"value": n * steps / (t_f + t_b), "unit": "particle-steps/s", "steps": steps, "checkpoint_every": ckpt,
-            "ms_per_step": 1e3 * (t_f + t_b) / steps, "forward_ms_per_step": 1e3 * t_f / steps,
-            "backward_ms_per_step": 1e3 * t_b / steps,
-            "workload": "two_boxes_128 (1,048,576 particles, 128^3), centre-of-mass adjoint seed"}
+    # algorithmic bytes per step: forward, recompute, tape write, adjoint state read/read/write
+    # (96 B each per particle) + per-particle E, and the grid forward (32 B) + adjoint (64 B)
+    fb_bytes = n * (768 + 20) + 96 * active
+    t = t_f + t_b
+    return {"value": n * steps / t, "unit": "particle-steps/s", "steps": steps, "checkpoint_every": ckpt,
+            "ms_per_step": 1e3 * t / steps, "forward_ms_per_step": 1e3 * t_f / steps,
+            "backward_ms_per_step": 1e3 * t_b / steps, "grad_rel_err_closed_form": err,
+            "roofline": {"bound": "hbm", "achieved": fb_bytes * steps / t / 1e9, "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": fb_bytes * steps / t / HBM_PEAK,
+                         "alg_bytes_per_step": fb_bytes, "active_nodes": active},
+            "workload": f"{scene.name}: one scene of batch_fb ({n:,} particles, 128^3, neo-Hookean, per-particle E, "
+                        f"dt 5e-5), {steps} steps, checkpoint every {ckpt} (recompute), centre-of-mass loss"}
+
+
+def batch_fb_leg(ckpt):
+    """The batched forward+backward workload: 16 scenes of 1,048,576 particles
+    on 128^3 in one batch context, 256 steps with a checkpoint every 16,
+    centre-of-mass loss per scene; gradients checked against the closed form."""
+    import numpy as np
+    import torch
+    from physgrad_amd.scenes import batch_fb
+    from physgrad_amd.solver import MPMBatch
+    scenes = batch_fb()
+    counts = [s.n for s in scenes]
+    n = sum(counts)
+    T = scenes[0].steps
+    b = MPMBatch([s.desc() for s in scenes])
+    x = np.concatenate([s.x for s in scenes])
+    v = np.concatenate([s.v for s in scenes])
+    E = np.concatenate([s.E_particles for s in scenes])
+    b.set_particles(counts, x, v, E=E)
+    b.forward_checkpointed(2 * ckpt, ckpt)   # warm-up: allocates checkpoints and the tape
+    b.backward(torch.zeros((n, 3), device="cuda"))
+    b.set_particles(counts, x, v, E=E)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(s)
+    b.forward_checkpointed(T, ckpt)
+    ev[1].record(s)
+    seed, resid = com_seed(b.get_particles()["x"], scenes, counts)
+    ev[2].record(s)
+    grads = b.backward(seed)
+    ev[3].record(s)
+    torch.cuda.synchronize()
+    t_f, t_b = ev[0].elapsed_time(ev[1]) / 1e3, ev[2].elapsed_time(ev[3]) / 1e3
+    err = closed_form_error(grads, scenes, counts, resid, T)
+    free, total = torch.cuda.mem_get_info()
+    b.close()
+    return {"value": n * T / (t_f + t_b), "unit": "particle-steps/s", "scenes": len(scenes), "particles": n,
+            "steps": T, "checkpoint_every": ckpt, "ms_per_step": 1e3 * (t_f + t_b) / T,
+            "forward_ms_per_step": 1e3 * t_f / T, "backward_ms_per_step": 1e3 * t_b / T,
+            "grad_rel_err_closed_form": err, "device_memory_used_gb": (total - free) / 1e9,
+            "workload": "batch_fb: 16 scenes x (128^3, 1,048,576 particles, seed 5), v ~ N(0, 0.5^2), neo-Hookean "
+                        "with per-particle E, dt 5e-5, 256 steps, checkpoint every 16, centre-of-mass loss per scene"}
 
 
 def batch_leg(steps):
@@ -174,11 +270,54 @@ def grad_leg():
     err = float(np.abs(got - want).max() / np.abs(want).max())
     # d loss / d log E against the v0 sensitivity: zero up to round-off
     gE = float((E.grad.double() * E.double()).abs().max().item())
+    out = {"rel_err_dv0": err, "dv0": got.tolist(), "dv0_closed_form": want.tolist(),
+           "max_abs_dlogE": gE, "fwd_bwd_wall_s": wall,
+           "workload": "rollout_64: 64^3, 262,144 particles, fixed-corotated, per-particle E log-uniform [5e3,5e4], "
+                       "512 steps at dt 1e-4, checkpoint every 32"}
+    # an E-sensitive loss, L = sum_p w_p . x_p(T), against FP64 central finite
+    # differences of the oracle (eps 1e-4; relative for E) on v0 and 16 sampled E
+    fd_path = os.path.join(ROOT, "tests", "golden", "rollout_fd.npz")
+    if os.path.exists(fd_path):
+        f = np.load(fd_path)
+        w = torch.as_tensor(np.random.default_rng(int(f["weights_seed"])).normal(size=(n, 3)) / n, device="cuda")
+        v0 = torch.zeros(3, dtype=torch.float32, device="cuda", requires_grad=True)
+        E = torch.as_tensor(sc.E_particles, device="cuda").clone().requires_grad_(True)
+        xT, _, _, _ = physics_simulation(ctx, x0, v0.expand(n, 3), E=E, num_steps=T, checkpoint_every=K)
+        (w * xT.double()).sum().backward()
+        gv = v0.grad.double().cpu().numpy()
+        gE = E.grad.double().cpu().numpy()[f["E_index"]]
+        out["weighted_loss_vs_fp64_fd"] = {
+            "rel_err_dv0": float(np.abs(gv - f["dv0"]).max() / np.abs(f["dv0"]).max()),
+            "rel_err_dE": float(np.abs(gE - f["dE"]).max() / np.abs(f["dE"]).max()),
+            "params": 3 + len(f["dE"]), "eps": float(f["eps"])}
     ctx.close()
-    return {"rel_err_dv0": err, "dv0": got.tolist(), "dv0_closed_form": want.tolist(),
-            "max_abs_dlogE": gE, "fwd_bwd_wall_s": wall,
-            "workload": "rollout_64: 64^3, 262,144 particles, fixed-corotated, per-particle E log-uniform [5e3,5e4], "
-                        "512 steps at dt 1e-4, checkpoint every 32"}
+    return out
+
+
+def fp64_error_leg(scene, steps):
+    """Error of the tiled GPU forward against the FP64 oracle after `steps`
+    steps of the full single-scene workload (inf-norm relative per field)."""
+    import numpy as np
+    from oracle import oracle as O
+    from physgrad_amd import _lib
+    from physgrad_amd.solver import MPMContext
+    ctx = MPMContext(scene.params(impl=_lib.IMPL_TILED))
+    ctx.set_particles(scene.x, scene.v)
+    ctx.step(steps)
+    got = {k: t.cpu().numpy() for k, t in ctx.get_particles().items()}
+    ctx.close()
+    t0 = time.perf_counter()
+    want = O.mls_run(O.scene_params(scene), scene.x.astype(np.float64), scene.v.astype(np.float64), None, None,
+                     scene.p_mass, scene.E, steps)
+    out = {"steps": steps, "oracle_s": time.perf_counter() - t0}
+    for k in ("x", "v", "F"):
+        a, b = got[k].astype(np.float64).reshape(-1), want[k].reshape(-1)
+        out[f"rel_err_{k}"] = float(np.abs(a - b).max() / np.abs(b).max())
+    # C is relative to the velocity-gradient scale max |v| / dx (rigidly moving
+    # bodies have C = 0 up to round-off)
+    dC = np.abs(got["C"].astype(np.float64) - want["C"]).max()
+    out["rel_err_C"] = float(dC / (np.abs(want["v"]).max() / scene.dx))
+    return out
 
 
 def cpu_baseline(scene, steps):
@@ -257,11 +396,15 @@ def main():
     achieved = alg_bytes / avg_kernel_s
     step_bytes_rate = alg_bytes * K / t_tiled
 
-    traffic = None
+    # HBM bytes per launch of the same kernel on the same workload, from the
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs stored in profiles/ (counters
+    # cannot be collected inside this timed run)
+    traffic, traffic_source = None, None
     tf = os.path.join(ROOT, "profiles", "traffic_tile_g2p2g_two_boxes_128.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            traffic_source = "profiles/traffic_tile_g2p2g_two_boxes_128.json (rocprofv3 --pmc, stored)"
         except Exception:
             traffic = None
 
@@ -274,15 +417,21 @@ def main():
         pctx.close()
         plain = {"value": N * kpl / t_plain, "steps": kpl, "ms_per_step": 1e3 * t_plain / kpl}
 
-    fwd_bwd = None if args.no_bwd else fwd_bwd_leg(scene, args.bwd_steps, args.bwd_ckpt, _lib.IMPL_TILED)
-    plain_fwd_bwd = None
-    if not args.no_bwd and not args.no_plain:
-        k = min(args.bwd_steps, args.bwd_ckpt)
-        plain_fwd_bwd = fwd_bwd_leg(scene, k, k, _lib.IMPL_PLAIN)
+    fwd_bwd = plain_fwd_bwd = batch_fb = None
+    if not args.no_bwd:
+        from physgrad_amd.scenes import batch_fb as batch_fb_scenes
+        fb_scene = batch_fb_scenes(nscenes=1)[0]
+        fwd_bwd = fwd_bwd_leg(fb_scene, args.bwd_steps, args.bwd_ckpt, _lib.IMPL_TILED)
+        if not args.no_plain:
+            plain_fwd_bwd = fwd_bwd_leg(fb_scene, args.plain_bwd_steps, args.bwd_ckpt, _lib.IMPL_PLAIN)
+        del fb_scene
+        if not args.no_batch_fb:
+            batch_fb = batch_fb_leg(args.bwd_ckpt)
     grad = None if args.no_bwd or args.no_grad else grad_leg()
     batch = None if args.no_batch else batch_leg(args.batch_steps)
 
-    cpu, cpu_ref = (None, None) if args.no_cpu else cpu_baseline(scene, args.cpu_steps)
+    cpu, cpu_ref, fp64 = (None, None, None) if args.no_cpu else cpu_baseline(scene, args.cpu_steps) + \
+        (fp64_error_leg(scene, 20),)
 
     out = {
         "metric": "particle-steps/s fwd and fwd+bwd at 1M particles; % HBM peak; grad rel err",
@@ -301,14 +450,16 @@ def main():
                                "neo-Hookean E 5e4 nu 0.3 rho 1000, dt 5e-5, sticky walls 3 nodes",
                    "particles": N, "grid": list(scene.dims), "impl": "tiled", "active_nodes": active},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": "k_tile<3> (fused G2P+P2G)", "avg_launch_us": avg_kernel_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes},
         "step_effective_GBps": step_bytes_rate / 1e9,
         "kernel_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
         "fwd_bwd": fwd_bwd,
+        "batch_fwd_bwd": batch_fb,
         "grad": grad,
         "batch": batch,
+        "fp64_error": fp64,
         "plain_port": plain,
         "speedup_vs_plain": (N * K / t_tiled) / plain["value"] if plain else None,
         "plain_port_fwd_bwd": plain_fwd_bwd,

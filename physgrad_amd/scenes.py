@@ -22,6 +22,17 @@
                       size nor density: each box holds 8 particles per cell
                       (2 per axis, the usual MPM sampling) with aspect ratios
                       in [0.7, 1.4], placed at random inside the walls.
+    batch_fb          16 independent scenes (seed 5) of 128^3 (dx 1/128) with
+                      1,048,576 particles each, uniform in one seeded box per
+                      scene, initial velocity N(0, 0.5^2) per component,
+                      neo-Hookean, nu 0.3, per-particle E log-uniform in
+                      [5e3,5e4], rho 1000, dt 5e-5, 256 steps, centre-of-mass
+                      loss per scene.  The workload fixes neither box size nor
+                      target: each box holds 8 particles per cell with aspect
+                      ratios in [0.8, 1.25], at least 0.12 from every wall
+                      (about 15 cells; the particles move less than one cell
+                      in 256 steps), and the target is the box centre moved by
+                      (0.05, -0.03, 0.02).
 Particle volume V0 = box volume / N; mass = rho V0.
 """
 from __future__ import annotations
@@ -133,4 +144,23 @@ def batch_scenes(nscenes: int = 64, seed: int = 4, min_n: int = 4096, max_n: int
         nu = float(rng.uniform(0.2, 0.35))
         out.append(Scene(f"batch_{s}_{side}", (side,) * 3, dx, 1e-4, 500, material, E, nu, 1000.0,
                          float(np.prod(size)) / n, x, v))
+    return out
+
+
+def batch_fb(nscenes: int = 16, n: int = 1 << 20, seed: int = 5, grid: int = 128, ppc: float = 8.0) -> list:
+    """Scenes of the batched forward+backward workload (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    dx = 1.0 / grid
+    for s in range(nscenes):
+        aspect = rng.uniform(0.8, 1.25, 3)
+        aspect /= np.prod(aspect) ** (1.0 / 3.0)
+        size = np.minimum(aspect * (n * dx ** 3 / ppc) ** (1.0 / 3.0), 0.76)
+        lo = 0.12 + (0.76 - size) * rng.random(3)
+        x = _uniform(rng, n, lo, lo + size)
+        v = rng.normal(0.0, 0.5, (n, 3)).astype(np.float32)
+        E = np.exp(rng.uniform(np.log(5e3), np.log(5e4), n)).astype(np.float32)
+        target = tuple(float(c) for c in (lo + 0.5 * size + np.array([0.05, -0.03, 0.02])))
+        out.append(Scene(f"batch_fb_{s}", (grid,) * 3, dx, 5e-5, 256, 1, 2e4, 0.3, 1000.0,
+                         float(np.prod(size)) / n, x, v, E_particles=E, target=target))
     return out
