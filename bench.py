@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_PARTICLE = 192   # x,v,C,F (96 B) read + written once per step
 BYTES_PER_NODE = 32        # grid (m,p) written once + (v) read once per active node
-COPY_BYTES = 256 << 20     # size of the calibration copy
+COPY_BYTES = 1 << 30       # size of the calibration copy and sum (4x the 256 MB Infinity Cache)
 
 
 def parse():
@@ -274,15 +274,19 @@ def grad_leg():
            "max_abs_dlogE": gE, "fwd_bwd_wall_s": wall,
            "workload": "rollout_64: 64^3, 262,144 particles, fixed-corotated, per-particle E log-uniform [5e3,5e4], "
                        "512 steps at dt 1e-4, checkpoint every 32"}
-    # an E-sensitive loss, L = sum_p w_p . x_p(T), against FP64 central finite
-    # differences of the oracle (eps 1e-4; relative for E) on v0 and 16 sampled E
+    # The workload's body falls rigidly (F stays I), so no loss depends on E.
+    # From a deforming initial state v0 + A (x - c), an E-sensitive loss
+    # L = sum_p w_p . x_p(T) is checked against FP64 central finite differences
+    # of the oracle (eps 1e-4; relative for E) on v0 and 16 sampled E.
     fd_path = os.path.join(ROOT, "tests", "golden", "rollout_fd.npz")
     if os.path.exists(fd_path):
         f = np.load(fd_path)
         w = torch.as_tensor(np.random.default_rng(int(f["weights_seed"])).normal(size=(n, 3)) / n, device="cuda")
+        x64 = sc.x.astype(np.float64)
+        vdef = torch.as_tensor(((x64 - x64.mean(0)) @ f["A_rate"].T).astype(np.float32), device="cuda")
         v0 = torch.zeros(3, dtype=torch.float32, device="cuda", requires_grad=True)
         E = torch.as_tensor(sc.E_particles, device="cuda").clone().requires_grad_(True)
-        xT, _, _, _ = physics_simulation(ctx, x0, v0.expand(n, 3), E=E, num_steps=T, checkpoint_every=K)
+        xT, _, _, _ = physics_simulation(ctx, x0, v0.expand(n, 3) + vdef, E=E, num_steps=T, checkpoint_every=K)
         (w * xT.double()).sum().backward()
         gv = v0.grad.double().cpu().numpy()
         gE = E.grad.double().cpu().numpy()[f["E_index"]]
@@ -382,12 +386,15 @@ def main():
     kt = ctx.kernel_times()
     ctx.set_profiling(False)
     ctx.close()
-    # a plain device copy of known size (256 MiB read + 256 MiB written): the
-    # calibration point of the HBM byte counters when this run is profiled
+    # a plain device copy (1 GiB read + 1 GiB written) and a sum (1 GiB read)
+    # of known size: the calibration points of the HBM byte counters
+    # (FETCH_SIZE, WRITE_SIZE) when this run is profiled
     src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
+    for _ in range(3):
+        dst.sum()
     torch.cuda.synchronize()
     del src, dst
     g2p2g_ms, g2p2g_n = kt.get("tile_g2p2g", (float("nan"), 1))

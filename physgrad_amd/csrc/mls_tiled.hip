@@ -93,13 +93,12 @@ __device__ __forceinline__ void flag_block(uint32_t* bflag, uint32_t* blist, uin
     }
 }
 
-// tile coordinates of a position, clamped like tile_key
-__device__ __forceinline__ void tile_coords(const float* x, const MlsGeom& g, int ntx, int nty, int ntz, int* t) {
+// tile coordinates of a stencil base, clamped like tile_key
+__device__ __forceinline__ void tile_of_base(const int* b, int ntx, int nty, int ntz, int* t) {
     const int nt[3] = {ntx, nty, ntz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int b = stencil_base(grid_coord(x[a], g.o[a], g.inv_dx));
-        int tc = b >= 0 ? b / TT : 0;
+        const int tc = b[a] >= 0 ? b[a] / TT : 0;
         t[a] = tc < nt[a] ? tc : nt[a] - 1;
     }
 }
@@ -114,8 +113,10 @@ struct P2GVal {
 __device__ __forceinline__ void p2g_prep(const MlsGeom& g, float Eq, float mq, const float* F, const float* C,
                                          const float* v, const Stencil& st, P2GVal& p) {
     const float kscale = -g.dt * g.p_vol * 4.f * g.inv_dx * g.inv_dx;
-    const float mu = Eq / (2.f * (1.f + g.nu));
-    const float lam = Eq * g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+    // Lame parameters per unit E: scene constants, so the divisions leave the loops
+    const float emu = 1.f / (2.f * (1.f + g.nu));
+    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+    const float mu = Eq * emu, lam = Eq * elam;
     float tau[9];
     kirchhoff(g.material, mu, lam, F, tau);
     float A[9];
@@ -386,9 +387,9 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 
             // binning of a moved particle: key, and its rank in the new tile through the
             // per-direction LDS counters (FAR: a global atomic on the tile count)
-            auto bin_out = [&](const float* xn, uint32_t o, uint32_t oo, bool aggregate) {
+            auto bin_out = [&](const Stencil& stn, uint32_t o, uint32_t oo, bool aggregate) {
                 int nt[3];
-                tile_coords(xn, g, sc.ntx, sc.nty, sc.ntz, nt);
+                tile_of_base(stn.b, sc.ntx, sc.nty, sc.ntz, nt);
                 const int dx_ = nt[0] - tc[0], dy_ = nt[1] - tc[1], dz_ = nt[2] - tc[2];
                 const uint32_t k = sc.toff + morton_tile(nt[0], nt[1], nt[2]);
                 a.key_out[o] = k;
@@ -427,10 +428,8 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             // is nearby, else with global atomics.
             // ob = window index of the particle's base before its G2P (own_ok only);
             // win = which of the wave's two windows
-            auto scatter = [&](const float* x, const float* F, const float* C, const float* v, float Eq, float mq,
+            auto scatter = [&](const Stencil& st, const float* F, const float* C, const float* v, float Eq, float mq,
                                bool own_ok, uint32_t oo, int ob, int win) {
-                Stencil st;
-                make_stencil(x, g.o, g.inv_dx, st);
                 const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
                 const int nb = (l2 * WN + l1) * WN + l0;
                 const bool inside = l0 >= 0 && l0 < TT && l1 >= 0 && l1 < TT && l2 >= 0 && l2 < TT;
@@ -486,6 +485,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 float Eq = g.E, mq = g.p_mass;
                 if (a.E_in) Eq = a.E_in[s];
                 if (a.m_in) mq = a.m_in[s];
+                Stencil stn;   // stencil of the position P2G scatters from
                 if (G2P) {
                     Stencil st;
                     make_stencil(x, g.o, g.inv_dx, st);
@@ -513,7 +513,10 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     if (a.E_out && a.E_in) a.E_out[o] = Eq;
                     if (a.m_out && a.m_in) a.m_out[o] = mq;
                     if (a.src_out) a.src_out[o] = s;
-                    if (P2G) bin_out(x, o, oo, own_ok);
+                    if (P2G) {
+                        make_stencil(x, g.o, g.inv_dx, stn);
+                        bin_out(stn, o, oo, own_ok);
+                    }
                 } else {
                     F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
                     F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
@@ -522,11 +525,10 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     C[1] = q4.x; C[2] = q4.y; C[3] = q4.z; C[4] = q4.w;
                     C[5] = q5.x; C[6] = q5.y; C[7] = q5.z; C[8] = q5.w;
                     // no G2P: the base is still the staged cell
-                    Stencil st;
-                    make_stencil(x, g.o, g.inv_dx, st);
-                    ob = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
+                    make_stencil(x, g.o, g.inv_dx, stn);
+                    ob = ((stn.b[2] - t0[2]) * WN + (stn.b[1] - t0[1])) * WN + (stn.b[0] - t0[0]);
                 }
-                if (P2G) scatter(x, F, C, v, Eq, mq, own_ok, oo, ob, win);
+                if (P2G) scatter(stn, F, C, v, Eq, mq, own_ok, oo, ob, win);
             };
 
             // ---- rounds: cell-order index j = grp + lane G of group grp (grp < G); the

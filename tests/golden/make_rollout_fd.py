@@ -2,14 +2,21 @@
 workload (rollout_64: 64^3, 262,144 particles, fixed-corotated, per-particle
 E, 512 steps at dt 1e-4) for an E-sensitive loss.
 
+With the workload's own initial state (all particles at rest, then free fall
+far from the walls) the body moves rigidly, F stays I and no loss depends on
+E.  The differences are therefore taken around a deforming state: the initial
+velocity is v0 + A (x - c) with the fixed non-symmetric rate matrix A below
+(rotation, shear and dilation of about 0.1 m/s at the box faces) and c the
+initial centre of mass, and the loss weighs every final position:
+
     L = sum_p w_p . x_p(T),   w = default_rng(33).normal(size=(n, 3)) / n
 
-with the shared initial velocity v0 (zero at the operating point) and the
-per-particle Young's moduli as parameters.  Differences are taken with the
+with the shared v0 (zero at the operating point) and the per-particle Young's
+moduli as parameters.  Differences are taken with the
 FP64 oracle (oracle/liboracle.so): v0 components with the absolute step 1e-4,
 E of 16 sampled particles (default_rng(34)) with the relative step 1e-4
-(E (1 +- 1e-4)), and, as a check of the step, two of the E parameters and one
-v0 component again with twice the step.  Each evaluation is 512 FP64 steps of
+(E (1 +- 1e-4)), and, as a check of the step, one E parameter and one v0
+component again with twice the step.  Each evaluation is 512 FP64 steps of
 the full workload (about 4 minutes on one core); the evaluations run in
 parallel processes.  Run in the build container:
 
@@ -31,6 +38,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 EPS = 1e-4
 N_E = 16
+A_RATE = np.array([[0.8, 0.3, -0.2], [-0.4, -0.6, 0.25], [0.2, -0.35, 0.5]])   # 1/s
 
 
 def setup():
@@ -46,13 +54,14 @@ def loss(task):
     from oracle import oracle as O
     sc, w, _ = setup()
     prm = O.scene_params(sc)
-    v = np.zeros((sc.n, 3))
+    x0 = sc.x.astype(np.float64)
+    v = (x0 - x0.mean(0)) @ A_RATE.T
     E = sc.E_particles.astype(np.float64).copy()
     if kind == "v":
         v[:, i] += h
     elif kind == "E":
         E[i] *= 1.0 + h
-    out = O.mls_run(prm, sc.x.astype(np.float64), v, None, None, sc.p_mass, E, sc.steps)
+    out = O.mls_run(prm, x0, v, None, None, sc.p_mass, E, sc.steps)
     return task, float((w * out["x"]).sum())
 
 
@@ -64,7 +73,7 @@ def main():
         tasks += [("v", c, EPS), ("v", c, -EPS)]
     for i in idx:
         tasks += [("E", int(i), EPS), ("E", int(i), -EPS)]
-    check = [("v", 0), ("E", int(idx[0])), ("E", int(idx[1]))]
+    check = [("v", 0), ("E", int(idx[0]))]
     for kind, i in check:
         tasks += [(kind, i, 2 * EPS), (kind, i, -2 * EPS)]
     t0 = time.time()
@@ -83,10 +92,10 @@ def main():
         d2.append((res[(kind, i, 2 * EPS)] - res[(kind, i, -2 * EPS)]) / (4 * EPS * scale))
     np.savez(os.path.join(HERE, "rollout_fd.npz"), loss=res[("base", 0, 0.0)], dv0=dv0, E_index=idx, dE=dE,
              eps=EPS, check_kind=np.array([k for k, _ in check]), check_index=np.array([i for _, i in check]),
-             check_2eps=np.array(d2), weights_seed=33, index_seed=34)
+             check_2eps=np.array(d2), weights_seed=33, index_seed=34, A_rate=A_RATE)
     print("dv0", dv0)
     print("dE", dE)
-    print("step check (eps, 2 eps):", [(dv0[0], d2[0]), (dE[0], d2[1]), (dE[1], d2[2])])
+    print("step check (eps, 2 eps):", [(dv0[0], d2[0]), (dE[0], d2[1])])
 
 
 if __name__ == "__main__":

@@ -5,10 +5,12 @@
                   steps, then conservation of mass and momentum, finite state,
                   every particle inside the grid, and the collision itself.
   rollout_64      262,144 particles, 512 steps, checkpoint every 32: dL/dv0 of
-                  the centre-of-mass loss against its closed form, and dL/dv0
-                  and dL/dE of 16 sampled particles of a weighted-position loss
-                  against FP64 central finite differences
-                  (tests/golden/rollout_fd.npz, make_rollout_fd.py).
+                  the centre-of-mass loss against its closed form (dL/dE is
+                  zero: the body falls rigidly, F stays I), and, from a
+                  deforming initial state (v0 + A (x - c)), dL/dv0 and dL/dE
+                  of 16 sampled particles of a weighted-position loss against
+                  FP64 central finite differences (tests/golden/rollout_fd.npz,
+                  written by tests/golden/make_rollout_fd.py).
   batch_64        64 scenes (1.23 M particles): one step of every scene against
                   the FP64 oracle, then 500 steps (autotuned launch, hipGraph
                   replay) with per-scene mass conservation and finite state.
@@ -88,11 +90,14 @@ def test_rollout_gradients_vs_fp64_finite_differences(gpu):
     x0 = torch.as_tensor(sc.x, device="cuda")
     w = torch.as_tensor(np.random.default_rng(int(f["weights_seed"])).normal(size=(n, 3)) / n, device="cuda")
     target = torch.tensor(sc.target, dtype=torch.float64, device="cuda")
+    x64 = sc.x.astype(np.float64)
+    vdef = torch.as_tensor(((x64 - x64.mean(0)) @ f["A_rate"].T).astype(np.float32), device="cuda")
     for loss_kind in ("com", "weighted"):
         print("rollout loss", loss_kind, flush=True)
         v0 = torch.zeros(3, dtype=torch.float32, device="cuda", requires_grad=True)
         E = torch.as_tensor(sc.E_particles, device="cuda").clone().requires_grad_(True)
-        xT, _, _, _ = physics_simulation(ctx, x0, v0.expand(n, 3), E=E, num_steps=T, checkpoint_every=K)
+        vinit = v0.expand(n, 3) + (vdef if loss_kind == "weighted" else 0.0)
+        xT, _, _, _ = physics_simulation(ctx, x0, vinit, E=E, num_steps=T, checkpoint_every=K)
         if loss_kind == "com":
             d = xT.double().mean(0) - target
             loss = (d * d).sum()
