@@ -105,6 +105,10 @@ int pg_mpm_get_grid(pg_mpm_ctx* ctx, float* mass, float* momentum, float* veloci
 int pg_mpm_step(pg_mpm_ctx* ctx, int nsteps, void* stream);
 /* Simulated time and step count (reference getCurrentTime/getStepCount). */
 int pg_mpm_get_time(const pg_mpm_ctx* ctx, double* time, long long* steps);
+/* Sets the clock, for resuming from a checkpoint (call after set_particles,
+ * which resets it).  No reference counterpart: the reference solver has no
+ * resume (its outputData, src/mpm_solver.h:501-504, only prints). */
+int pg_mpm_set_time(pg_mpm_ctx* ctx, double time, long long steps);
 
 /* Differentiable rollout (MLS-MPM, tiled or plain implementation).
  * pg_mpm_forward_checkpointed advances nsteps steps like pg_mpm_step and keeps

@@ -73,6 +73,7 @@ _SIGNATURES = {
     "pg_mpm_get_grid": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "pg_mpm_step": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "pg_mpm_get_time": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
+    "pg_mpm_set_time": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_longlong]),
     "pg_mpm_forward_checkpointed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pg_mpm_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "pg_mpm_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -26,11 +26,17 @@ class PhysicsSimulationFunction(torch.autograd.Function):
         mpm.forward_checkpointed(nsteps, checkpoint_every)
         out = mpm.get_particles()
         ctx.mpm = mpm
+        ctx.rollout = mpm.rollouts
         ctx.has_E = E is not None
         return out["x"], out["v"], out["C"], out["F"]
 
     @staticmethod
     def backward(ctx, gx, gv, gC, gF):
+        # the context keeps the checkpoints of its last rollout only
+        if ctx.mpm.rollouts != ctx.rollout:
+            raise RuntimeError("physics_simulation: the MPM context ran another rollout since this one; "
+                               "call backward before the next forward on the same context, or use one context "
+                               "per rollout")
         n = gx.shape[0] if gx is not None else ctx.mpm.num_particles
         grads = ctx.mpm.backward(gx, gv, None if gC is None else gC.reshape(n, 9),
                                  None if gF is None else gF.reshape(n, 9))

@@ -470,6 +470,14 @@ int pg_mpm_get_time(const pg_mpm_ctx* c, double* t, long long* s) {
     return PG_OK;
 }
 
+int pg_mpm_set_time(pg_mpm_ctx* c, double t, long long s) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (!std::isfinite(t) || s < 0) return fail(c, PG_ERR_INVALID, "time must be finite and steps >= 0");
+    c->time = t;
+    c->steps = s;
+    return PG_OK;
+}
+
 }  // extern "C"
 
 namespace {

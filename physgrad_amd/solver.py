@@ -148,6 +148,11 @@ class MPMContext:
                 raise ValueError(f"{name} has {t.numel()} values, expected {n * cols}")
         check(self._L.pg_mpm_set_particles(self._h, n, _dptr(x), _dptr(v), _dptr(C), _dptr(F), _dptr(mass),
                                            _dptr(E), _lib.DEVICE, _stream_ptr()), self._h)
+        # per-particle constants travel with the particles unchanged: kept in the
+        # caller's order for checkpoints
+        self.particle_E = None if E is None else E.clone()
+        self.particle_mass = None if mass is None else mass.clone()
+        self.rollouts = getattr(self, "rollouts", 0)
 
     def get_particles(self):
         n = self.num_particles
@@ -190,6 +195,7 @@ class MPMContext:
         """Advance nsteps steps keeping a checkpoint every `checkpoint_every` steps."""
         check(self._L.pg_mpm_forward_checkpointed(self._h, int(nsteps), int(checkpoint_every), _stream_ptr()),
               self._h)
+        self.rollouts = getattr(self, "rollouts", 0) + 1
 
     def backward(self, ax=None, av=None, aC=None, aF=None):
         """Adjoints of the initial state (and per-particle E) from adjoints of the
@@ -228,6 +234,10 @@ class MPMContext:
         s = ctypes.c_longlong()
         check(self._L.pg_mpm_get_time(self._h, ctypes.byref(t), ctypes.byref(s)), self._h)
         return t.value, s.value
+
+    def set_time(self, time: float, steps: int):
+        """Sets the clock (resume from a checkpoint; set_particles resets it)."""
+        check(self._L.pg_mpm_set_time(self._h, float(time), int(steps)), self._h)
 
 
 def scene_desc(dims, dx, origin=(0.0, 0.0, 0.0), dt=1e-4, gravity=(0.0, -9.8, 0.0), material=0,
@@ -291,6 +301,8 @@ class MPMBatch(MPMContext):
                                                  _dptr(mass), _dptr(E), _lib.DEVICE,
                                                  _stream_ptr()), self._h)
         self.counts = counts
+        self.particle_E = None if E is None else E.clone()
+        self.particle_mass = None if mass is None else mass.clone()
 
 
 @dataclass
