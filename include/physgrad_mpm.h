@@ -49,7 +49,7 @@ extern "C" {
 #define PG_ERR_STATE 4     /* call out of order (e.g. backward without forward) */
 
 enum pg_mode { PG_MODE_REF_COMPAT = 0, PG_MODE_MLS_MPM = 1 };
-enum pg_material { PG_MAT_FIXED_COROTATED = 0, PG_MAT_NEO_HOOKEAN = 1 };
+enum pg_material { PG_MAT_FIXED_COROTATED = 0, PG_MAT_NEO_HOOKEAN = 1, PG_MAT_FLUID = 2 };
 enum pg_impl { PG_IMPL_PLAIN = 0, PG_IMPL_TILED = 1 };
 enum pg_memory { PG_HOST = 0, PG_DEVICE = 1 };
 
@@ -155,6 +155,54 @@ int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long*
  * with this check (and so synchronise). */
 int pg_mpm_check_state(pg_mpm_ctx* ctx, int* first_nonfinite, int* first_outside, void* stream);
 int pg_mpm_set_validation(pg_mpm_ctx* ctx, int on);
+
+/* Per-particle materials (MLS-MPM, tiled or plain, forward and adjoint).
+ * Replaces the reference's MaterialDatabase lookup per particle
+ * (src/mpm_data_structures.h:539-612; MaterialParameters :63-104; the kernel's
+ * per-particle model switch src/mpm_kernels.cu:295, :405-421).  table[m]
+ * describes material m; ids [n] (the caller's particle order, host or device
+ * memory) gives each particle's material, all < nmat <= 256.  A particle then
+ * uses its material's model, Poisson ratio, viscosity and Young's modulus
+ * (a per-particle E given to set_particles takes precedence) and, when the
+ * material has a density, the initial volume mass / density (else the
+ * context's particle_volume).  PG_MAT_FLUID is the reference's Newtonian
+ * fluid (src/mpm_kernels.cu:131-149): Cauchy stress 2 viscosity sym(grad v),
+ * grad v = the particle's APIC C, entering P2G like the elastic stresses as
+ * J sigma.  table NULL or nmat 0 restores the scenes' materials.
+ * set_particles resets to the scenes' materials, so call this after it. */
+typedef struct pg_material_desc {
+    int model;              /* pg_material */
+    float youngs_modulus;   /* E (elastic models) */
+    float poisson_ratio;    /* nu (elastic models), in (-1, 0.5) */
+    float density;          /* rho > 0: particle volume = mass / rho; 0: the context's particle_volume */
+    float viscosity;        /* dynamic viscosity (PG_MAT_FLUID) */
+} pg_material_desc;
+int pg_mpm_set_materials(pg_mpm_ctx* ctx, const pg_material_desc* table, int nmat, const uint8_t* ids, int memory,
+                         void* stream);
+
+/* Per-node boundary flags (MLS-MPM, tiled or plain, forward and adjoint):
+ * flags [G] (node index (k ny + j) nx + i; the concatenated grids of a batch)
+ * with the reference's BoundaryType bits (src/mpm_data_structures.h:617-627),
+ * applied by the grid update after v = p/m + dt g as the reference's
+ * updateGridKernel does (src/mpm_kernels.cu:536-539) and extended to the bits
+ * it declares but never applies:
+ *   PG_BC_DIRICHLET_X/Y/Z  zero that velocity component
+ *   PG_BC_STICKY           zero the whole velocity (no slip)
+ *   PG_BC_SLIP             remove each component that points out of the domain
+ *                          through the nearer wall of its axis (node index <
+ *                          n/2: negative components; otherwise positive ones)
+ *   PG_BC_NEUMANN_X/Y/Z    accepted, no effect (none in the reference either)
+ * The sticky band of boundary_thickness nodes applies as well.  NULL clears
+ * the flags. */
+#define PG_BC_DIRICHLET_X 1u
+#define PG_BC_DIRICHLET_Y 2u
+#define PG_BC_DIRICHLET_Z 4u
+#define PG_BC_NEUMANN_X 8u
+#define PG_BC_NEUMANN_Y 16u
+#define PG_BC_NEUMANN_Z 32u
+#define PG_BC_STICKY 64u
+#define PG_BC_SLIP 128u
+int pg_mpm_set_boundary(pg_mpm_ctx* ctx, const uint32_t* flags, int memory, void* stream);
 
 /* Batched independent scenes in one context (tiled MLS-MPM).  The reference
  * runs one scene per MPMSolver (src/mpm_solver.h:150-171, one grid and one

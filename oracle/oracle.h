@@ -43,7 +43,7 @@ void orc_ref_run(const orc_ref_params* p, int n, float* x, float* v, const float
                  int nsteps, float* grid_mass, float* grid_mom, float* grid_vel);
 
 /* ---------------- MLS-MPM ---------------- */
-enum { ORC_FIXED_COROTATED = 0, ORC_NEO_HOOKEAN = 1 };
+enum { ORC_FIXED_COROTATED = 0, ORC_NEO_HOOKEAN = 1, ORC_FLUID = 2 };
 
 typedef struct {
     int nx, ny, nz;       /* grid nodes per axis */
@@ -80,6 +80,18 @@ void orc_mls_adjoint_step_f64(const orc_mls_params* p, int n,
                               const double* x, const double* v, const double* C, const double* F,
                               const double* mass, const double* E,
                               double* ax, double* av, double* aC, double* aF, double* aE);
+
+/* The FP64 step with per-particle materials and per-node boundary flags
+ * (mlsmpm_ext.c): model [n] (ORC_*), nu, visc, vol [n] (NULL: the scene's
+ * material / Poisson ratio / 0 / p_vol), flags [G] node BoundaryType bits
+ * (NULL: none).  Identical to orc_mls_run_f64 when all of them are NULL. */
+void orc_mls_run_ext_f64(const orc_mls_params* p, int n, double* x, double* v, double* C, double* F,
+                         const double* mass, const double* E, const int* model, const double* nu,
+                         const double* visc, const double* vol, const unsigned* flags, int nsteps,
+                         double* grid_m, double* grid_v);
+/* tau of one particle for any model: the Newtonian fluid gives J visc (C + C^T) */
+void orc_kirchhoff_ext_f64(int model, double mu, double lam, double visc, const double* F, const double* C,
+                           double* tau);
 
 /* FP32 MLS-MPM on all host cores (OpenMP; nthreads <= 0: the OpenMP default);
  * the CPU baseline timed beside the GPU path */

@@ -24,6 +24,7 @@ _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 
 FIXED_COROTATED = 0
 NEO_HOOKEAN = 1
+FLUID = 2
 
 
 class RefParams(ctypes.Structure):
@@ -76,6 +77,13 @@ def lib():
         L.orc_mls_adjoint_step_f64.argtypes = [ctypes.POINTER(MlsParams), ctypes.c_int,
                                                _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                                _f64p, _f64p, _f64p, _f64p, _f64p]
+        _ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.orc_mls_run_ext_f64.argtypes = [ctypes.POINTER(MlsParams), ctypes.c_int, _f64p, _f64p, _f64p, _f64p,
+                                          _f64p, _f64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+        L.orc_kirchhoff_ext_f64.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            _f64p, _f64p, _f64p]
         L.orc_morton3.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]
         L.orc_morton3.restype = ctypes.c_uint
         L.orc_cell_ids.argtypes = [_f32p, ctypes.c_uint, ctypes.c_float, _f32p, _u32p, _u32p]
@@ -189,6 +197,38 @@ def mls_run(prm: MlsParams, x, v, C, F, mass, E, nsteps, dtype=np.float64, want_
         out["grid_m"] = gm
         out["grid_v"] = gv.reshape(-1, 3)
     return out
+
+
+def mls_run_ext(prm: MlsParams, x, v, C, F, mass, E, nsteps, model=None, nu=None, visc=None, vol=None,
+                flags=None, want_grid=False):
+    """FP64 MLS-MPM with per-particle model / Poisson ratio / viscosity /
+    volume and per-node boundary flags (None: the scene's values, no flags).
+    Returns dict(x, v, C, F[, grid_m, grid_v])."""
+    n, x, v, C, F = _state(x, v, C, F, np.float64)
+    arr = lambda a, dt: None if a is None else np.ascontiguousarray(np.broadcast_to(np.asarray(a, dt), (n,)))  # noqa: E731
+    mass = arr(mass, np.float64)
+    E = arr(E, np.float64)
+    model, nu, visc, vol = arr(model, np.int32), arr(nu, np.float64), arr(visc, np.float64), arr(vol, np.float64)
+    G = prm.nx * prm.ny * prm.nz
+    fl = None if flags is None else np.ascontiguousarray(np.asarray(flags, np.uint32).reshape(G))
+    gm = np.zeros(G) if want_grid else None
+    gv = np.zeros(3 * G) if want_grid else None
+    ptr = (lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None)
+    lib().orc_mls_run_ext_f64(ctypes.byref(prm), n, x, v, C, F, mass, E, ptr(model), ptr(nu), ptr(visc), ptr(vol),
+                              ptr(fl), int(nsteps), ptr(gm), ptr(gv))
+    out = dict(x=x.reshape(n, 3), v=v.reshape(n, 3), C=C.reshape(n, 3, 3), F=F.reshape(n, 3, 3))
+    if want_grid:
+        out["grid_m"] = gm
+        out["grid_v"] = gv.reshape(-1, 3)
+    return out
+
+
+def kirchhoff_ext(model, mu, lam, visc, F, C):
+    F = np.ascontiguousarray(F, np.float64).reshape(9)
+    C = np.ascontiguousarray(C, np.float64).reshape(9)
+    tau = np.zeros(9, np.float64)
+    lib().orc_kirchhoff_ext_f64(int(model), float(mu), float(lam), float(visc), F, C, tau)
+    return tau.reshape(3, 3)
 
 
 def mls_run_mt(prm: MlsParams, x, v, C, F, mass, E, nsteps, nthreads: int = 0):

@@ -55,6 +55,23 @@ class PgSceneDesc(ctypes.Structure):
                 ("boundary_thickness", ctypes.c_int), ("max_particles", ctypes.c_int)]
 
 
+class PgMaterialDesc(ctypes.Structure):
+    """Mirror of struct pg_material_desc (one entry of a material table)."""
+    _fields_ = [("model", ctypes.c_int), ("youngs_modulus", ctypes.c_float), ("poisson_ratio", ctypes.c_float),
+                ("density", ctypes.c_float), ("viscosity", ctypes.c_float)]
+
+
+MAT_FLUID = 2
+BC_DIRICHLET_X = 1
+BC_DIRICHLET_Y = 2
+BC_DIRICHLET_Z = 4
+BC_NEUMANN_X = 8
+BC_NEUMANN_Y = 16
+BC_NEUMANN_Z = 32
+BC_STICKY = 64
+BC_SLIP = 128
+
+
 class PgError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[pg error {code}] {msg}")
@@ -81,6 +98,8 @@ _SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_longlong)]),
     "pg_mpm_check_state": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _vp]),
     "pg_mpm_set_validation": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pg_mpm_set_materials": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp]),
+    "pg_mpm_set_boundary": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     "pg_mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(PgSceneDesc), ctypes.c_int, ctypes.POINTER(_vp)]),
     "pg_mpm_batch_set_particles": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "pg_mpm_num_scenes": (ctypes.c_int, [_vp]),

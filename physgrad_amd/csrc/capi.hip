@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "physgrad_mpm.h"
+#include "pg_device.h"
 #include "pg_kernels.h"
 
 namespace {
@@ -70,6 +71,16 @@ struct pg_mpm_ctx {
     uint32_t* adst = nullptr;               // storage slot -> output position of a reverse step
     uint32_t* check = nullptr;              // [4] results of the state validation kernel
     bool validate = false;                  // pg_mpm_step validates the state after stepping
+    // per-particle materials (pg_mpm_set_materials) and per-node boundary flags (pg_mpm_set_boundary)
+    pg::MatRec* mats_d = nullptr;           // [MAX_MATERIALS]
+    uint8_t* mat_ids = nullptr;             // [cap] material per caller index
+    int nmat = 0;                           // 0: every particle has its scene's material
+    uint32_t* node_bc = nullptr;            // [G]
+    bool have_bc = false;
+    pg::MatTable mat_table() const {
+        return pg::MatTable{nmat > 0 ? mats_d : nullptr, nmat > 0 ? mat_ids : nullptr};
+    }
+    const uint32_t* bc_flags() const { return have_bc ? node_bc : nullptr; }
 };
 
 namespace {
@@ -497,6 +508,9 @@ int set_particles_impl(pg_mpm_ctx* c, int n, const int* counts, const float* x, 
     c->time = 0.0;
     c->steps = 0;
     c->version++;
+    // material ids are per particle: a new particle set starts from the scenes' materials
+    c->nmat = 0;
+    c->tw.mt = c->mat_table();
     if (c->p.mode == PG_MODE_REF_COMPAT) {
         if ((rc = put(c, c->rx, x, 12 * (size_t)n, memory, st)) || (rc = put(c, c->rv, v, 12 * (size_t)n, memory, st)))
             return rc;
@@ -673,7 +687,8 @@ int pg_mpm_step(pg_mpm_ctx* c, int nsteps, void* stream) {
         if (c->p.impl == PG_IMPL_PLAIN) {
             for (int s = 0; s < nsteps && e == hipSuccess; ++s)
                 e = pg::mls_plain_step(g, c->n, c->cap, c->S, c->have_mass ? c->mass : nullptr,
-                                       c->have_E ? c->E : nullptr, c->mp, c->gv, st, &c->prof);
+                                       c->have_E ? c->E : nullptr, c->mat_table(), c->bc_flags(), c->mp, c->gv, st,
+                                       &c->prof);
         } else {
             float* E = c->have_E ? c->E : nullptr;
             float* E2 = c->have_E ? c->E2 : nullptr;
@@ -773,6 +788,102 @@ int pg_mpm_check_state(pg_mpm_ctx* c, int* first_nonfinite, int* first_outside, 
 int pg_mpm_set_validation(pg_mpm_ctx* c, int on) {
     if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
     c->validate = on != 0;
+    return PG_OK;
+}
+
+int pg_mpm_set_materials(pg_mpm_ctx* c, const pg_material_desc* table, int nmat, const uint8_t* ids, int memory,
+                         void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM) return fail(c, PG_ERR_INVALID, "materials need mode MLS_MPM");
+    if (memory != PG_HOST && memory != PG_DEVICE) return fail(c, PG_ERR_INVALID, "bad memory kind");
+    hipStream_t st = (hipStream_t)stream;
+    if (hipError_t es = settle(c, st); es != hipSuccess) return hip_fail(c, es, "settle");
+    drop_graphs(c);
+    if (!table || nmat == 0) {   // back to the scenes' materials
+        c->nmat = 0;
+        c->tw.mt = c->mat_table();
+        return PG_OK;
+    }
+    if (nmat < 0 || nmat > pg::MAX_MATERIALS)
+        return fail(c, PG_ERR_INVALID, "nmat must be in [1, " + std::to_string(pg::MAX_MATERIALS) + "]");
+    if (c->n > 0 && !ids) return fail(c, PG_ERR_INVALID, "ids is required when the context has particles");
+    std::vector<pg::MatRec> recs(pg::MAX_MATERIALS);
+    for (int m = 0; m < nmat; ++m) {
+        const pg_material_desc& d = table[m];
+        pg::MatRec& r = recs[m];
+        std::memset(&r, 0, sizeof(r));
+        if (d.model != PG_MAT_FIXED_COROTATED && d.model != PG_MAT_NEO_HOOKEAN && d.model != PG_MAT_FLUID)
+            return fail(c, PG_ERR_INVALID, "material " + std::to_string(m) + ": bad model");
+        if (!(d.density >= 0.f) || !std::isfinite(d.density))
+            return fail(c, PG_ERR_INVALID, "material " + std::to_string(m) + ": density must be >= 0");
+        r.model = d.model;
+        r.inv_rho = d.density > 0.f ? 1.f / d.density : 0.f;
+        if (d.model == PG_MAT_FLUID) {
+            if (!(d.viscosity >= 0.f) || !std::isfinite(d.viscosity))
+                return fail(c, PG_ERR_INVALID, "material " + std::to_string(m) + ": viscosity must be >= 0");
+            r.visc = d.viscosity;
+        } else {
+            if (!(d.poisson_ratio > -1.f && d.poisson_ratio < 0.5f))
+                return fail(c, PG_ERR_INVALID, "material " + std::to_string(m) + ": poisson_ratio must be in (-1, 0.5)");
+            if (!(d.youngs_modulus >= 0.f) || !std::isfinite(d.youngs_modulus))
+                return fail(c, PG_ERR_INVALID, "material " + std::to_string(m) + ": youngs_modulus must be >= 0");
+            const float nu = d.poisson_ratio;
+            r.E = d.youngs_modulus;
+            r.emu = 1.f / (2.f * (1.f + nu));
+            r.elam = nu / ((1.f + nu) * (1.f - 2.f * nu));
+        }
+    }
+    int rc;
+    if (!c->mats_d && ((rc = dalloc(c, &c->mats_d, (size_t)pg::MAX_MATERIALS)) || (rc = dalloc(c, &c->mat_ids, c->cap))))
+        return rc;
+    hipError_t e = hipMemcpyAsync(c->mats_d, recs.data(), recs.size() * sizeof(pg::MatRec), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && c->n > 0) {
+        // ids are checked on the host: copy device ids back first
+        std::vector<uint8_t> h((size_t)c->n);
+        if (memory == PG_HOST) {
+            std::memcpy(h.data(), ids, h.size());
+        } else {
+            e = hipMemcpyAsync(h.data(), ids, h.size(), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+        }
+        if (e == hipSuccess) {
+            for (size_t i = 0; i < h.size(); ++i)
+                if (h[i] >= nmat) return fail(c, PG_ERR_INVALID, "material id " + std::to_string(h[i]) +
+                                                                     " of particle " + std::to_string(i) + " >= nmat");
+            e = hipMemcpyAsync(c->mat_ids, h.data(), h.size(), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+        }
+    } else if (e == hipSuccess) {
+        e = hipStreamSynchronize(st);
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_set_materials");
+    c->nmat = nmat;
+    c->tw.mt = c->mat_table();
+    c->version++;
+    return PG_OK;
+}
+
+int pg_mpm_set_boundary(pg_mpm_ctx* c, const uint32_t* flags, int memory, void* stream) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM) return fail(c, PG_ERR_INVALID, "boundary flags need mode MLS_MPM");
+    if (memory != PG_HOST && memory != PG_DEVICE) return fail(c, PG_ERR_INVALID, "bad memory kind");
+    hipStream_t st = (hipStream_t)stream;
+    // the pending grid update of a tiled step uses the flags in force when it was issued
+    if (hipError_t es = settle(c, st); es != hipSuccess) return hip_fail(c, es, "settle");
+    drop_graphs(c);
+    if (!flags) {
+        c->have_bc = false;
+        c->tw.node_bc = c->bc_flags();
+        return PG_OK;
+    }
+    int rc;
+    if (!c->node_bc && (rc = dalloc(c, &c->node_bc, c->G))) return rc;
+    hipError_t e = hipMemcpyAsync(c->node_bc, flags, c->G * 4, kind_to_dev(memory), st);
+    if (e == hipSuccess && memory == PG_HOST) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(c, e, "pg_mpm_set_boundary");
+    c->have_bc = true;
+    c->tw.node_bc = c->bc_flags();
+    c->version++;
     return PG_OK;
 }
 
@@ -926,7 +1037,8 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
             const int L = std::min(K, nsteps - k * K);
             for (int s = 0; s < L && e == hipSuccess; ++s)
                 e = pg::mls_plain_step(g, c->n, c->cap, c->S, c->have_mass ? c->mass : nullptr,
-                                       c->have_E ? c->E : nullptr, c->mp, c->gv, st, &c->prof);
+                                       c->have_E ? c->E : nullptr, c->mat_table(), c->bc_flags(), c->mp, c->gv, st,
+                                       &c->prof);
             continue;
         }
         if (e == hipSuccess) e = copy_u32(ck.pid, c->pid, c->n, st);
@@ -1020,14 +1132,15 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
             for (int s = 0; s < L && e == hipSuccess; ++s) {
                 e = copy_planes(T[s + 1].S, T[s].S, c->cap, n, 6, st);
                 if (e == hipSuccess)
-                    e = pg::mls_plain_step(g, n, c->cap, T[s + 1].S, pm, pE, T[s].mp, T[s].gv, st, &c->prof);
+                    e = pg::mls_plain_step(g, n, c->cap, T[s + 1].S, pm, pE, c->mat_table(), c->bc_flags(), T[s].mp,
+                                           T[s].gv, st, &c->prof);
             }
             if (e == hipSuccess)
                 e = pg::adjoint_from_user(n, c->cap, nullptr, c->ux, c->uv, c->uC, c->uF, c->abar, nullptr, nullptr,
                                           st);
             if (e == hipSuccess)
-                e = pg::adjoint_segment_plain(g, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aE, pE, pm, st,
-                                              &c->prof);
+                e = pg::adjoint_segment_plain(g, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aE, pE, pm,
+                                              c->mat_table(), c->bc_flags(), st, &c->prof);
             if (e == hipSuccess)
                 e = pg::adjoint_to_user(n, c->cap, nullptr, c->abar, c->ux, c->uv, c->uC, c->uF, nullptr, nullptr, st);
             continue;

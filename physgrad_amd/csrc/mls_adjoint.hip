@@ -56,6 +56,7 @@ struct AdjArgs {
     float4* wbuf;           // per-tile window sums of the grid adjoint (by index in alist)
     const SceneInfo* scenes;
     size_t cap;
+    MatTable mt;            // per-particle materials (recs NULL: the scene's)
 };
 
 PG_DEV void load_adj(const float4* __restrict__ A, size_t cap, size_t i, float* ax, float* aF, float* av, float* aC) {
@@ -432,7 +433,8 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
                                                   const uint32_t* __restrict__ bscene,
                                                   const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
                                                   float4* __restrict__ agv_all, float4* __restrict__ apam_all,
-                                                  float4* __restrict__ mp_all, uint32_t* __restrict__ flags) {
+                                                  float4* __restrict__ mp_all, uint32_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ node_bc) {
     const int b = blockIdx.x;
     if (flags[b] == 0u) return;
     const SceneInfo sc = scenes[bscene[b]];
@@ -470,15 +472,7 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
             }
         }
         const float4 q = mp[id];
-        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q.x > 0.f && !is_sticky(g, n3[0], n3[1], n3[2])) {
-            const float im = 1.f / q.x;
-            r.y = av.x * im;
-            r.z = av.y * im;
-            r.w = av.z * im;
-            r.x = -(av.x * q.y + av.y * q.z + av.z * q.w) * im * im;
-        }
-        apam[id] = r;
+        apam[id] = node_velocity_vjp(g, n3[0], n3[1], n3[2], node_bc ? node_bc[sc.goff + id] : 0u, q, av);
         agv[id] = make_float4(0.f, 0.f, 0.f, 0.f);
         mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
@@ -501,18 +495,15 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
 // Returns the adjoints of x, F, v, C before the step and dL/dE.
 template <bool ROLLED, class Load, class Part, class GetF>
 __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, GetF getF, const MlsGeom& g,
-                                                 const Stencil& st, const float* v, const float* C, float Eq,
+                                                 const Stencil& st, const float* v, const float* C, const PMat& pm,
                                                  float mq, float* ax, float* aF, float* avq, float* aC, float& dE) {
     const float s4 = 4.f * g.inv_dx * g.inv_dx;
-    const float kscale = -g.dt * g.p_vol * s4;
-    const float emu = 1.f / (2.f * (1.f + g.nu));
-    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
-    const float mu = Eq * emu, lam = Eq * elam;
+    const float kscale = -g.dt * pm.vol * s4;
     float A[9];
     {
         float F[9], tau[9];
         getF(F);
-        kirchhoff(g.material, mu, lam, F, tau);
+        stress_of(pm, F, C, tau);
 #pragma unroll
         for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
     }
@@ -628,11 +619,10 @@ __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, GetF getF
         at[c] = kscale * aA[c];
         aC[c] = mq * aA[c];
     }
-    float amu = 0.f, alam = 0.f;
     float F[9];
     getF(F);
-    kirchhoff_vjp(g.material, mu, lam, F, at, aF, amu, alam);
-    dE = amu * emu + alam * elam;
+    dE = 0.f;
+    stress_vjp(pm, F, C, at, aF, aC, dE);
 }
 
 __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per SIMD: latency-bound gather
@@ -659,8 +649,8 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             float x[3], v[3], C[9];
             load_xvC(a.S, a.cap, s, x, v, C);
             auto getF = [&](float* F) { load_F(a.S, a.cap, s, F); };
-            const float Eq = a.E ? a.E[s] : g.E;
             const float mq = a.mass ? a.mass[s] : g.p_mass;
+            const PMat pm = particle_mat(g, a.mt, a.pid[s], a.E, s, mq);
             Stencil st;
             make_stencil(x, g.o, g.inv_dx, st);
             const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
@@ -673,7 +663,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             if (inwin)
                 adj_p2g_particle<false>(
                     [&](int ii, int jj, int kk) { return sp[((l2 + kk) * WN + (l1 + jj)) * WN + (l0 + ii)]; }, part,
-                    getF, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
+                    getF, g, st, v, C, pm, mq, ax, aF, avq, aC, dE);
             else
                 adj_p2g_particle<true>(
                     [&](int ii, int jj, int kk) {
@@ -681,7 +671,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
                         return in_grid(g, gi, gj, gk) ? apams[node_id(g, gi, gj, gk)]
                                                       : make_float4(0.f, 0.f, 0.f, 0.f);
                     },
-                    part, getF, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
+                    part, getF, g, st, v, C, pm, mq, ax, aF, avq, aC, dE);
             store_state(a.abar, a.cap, s, ax, aF, avq, aC);
             if (a.aEs) a.aEs[s] = a.aEp[q] + dE;
         }
@@ -723,20 +713,12 @@ __global__ void __launch_bounds__(256) k_adj_g2p_plain(int n, size_t cap, MlsGeo
 }
 
 __global__ void k_adj_grid_plain(MlsGeom g, float4* __restrict__ agv, float4* __restrict__ apam,
-                                 const float4* __restrict__ mp) {
+                                 const float4* __restrict__ mp, const uint32_t* __restrict__ node_bc) {
     const size_t G = (size_t)g.nx * g.ny * g.nz;
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < G; id += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(id % g.nx), j = (int)((id / g.nx) % g.ny), k = (int)(id / ((size_t)g.nx * g.ny));
         const float4 q = mp[id], av = agv[id];
-        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q.x > 0.f && !is_sticky(g, i, j, k)) {
-            const float im = 1.f / q.x;
-            r.y = av.x * im;
-            r.z = av.y * im;
-            r.w = av.z * im;
-            r.x = -(av.x * q.y + av.y * q.z + av.z * q.w) * im * im;
-        }
-        apam[id] = r;
+        apam[id] = node_velocity_vjp(g, i, j, k, node_bc ? node_bc[id] : 0u, q, av);
         agv[id] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
@@ -744,13 +726,14 @@ __global__ void k_adj_grid_plain(MlsGeom g, float4* __restrict__ agv, float4* __
 __global__ void __launch_bounds__(256) k_adj_p2g_plain(int n, size_t cap, MlsGeom g, const float4* __restrict__ S,
                                                        const float* __restrict__ E, const float* __restrict__ mass,
                                                        const float4* __restrict__ apam, const float4* __restrict__ part,
-                                                       float4* __restrict__ abar, float* __restrict__ aE) {
+                                                       float4* __restrict__ abar, float* __restrict__ aE,
+                                                       MatTable mt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float x[3], v[3], C[9];
     load_xvC(S, cap, i, x, v, C);
-    const float Eq = E ? E[i] : g.E;
     const float mq = mass ? mass[i] : g.p_mass;
+    const PMat pm = particle_mat(g, mt, (uint32_t)i, E, i, mq);
     Stencil st;
     make_stencil(x, g.o, g.inv_dx, st);
     float ax[3], aF[9], avq[3], aC[9], dE;
@@ -762,7 +745,7 @@ __global__ void __launch_bounds__(256) k_adj_p2g_plain(int n, size_t cap, MlsGeo
         [&](float4& p0, float4& p1, float4& p2) {
             p0 = part[i]; p1 = part[cap + i]; p2 = part[2 * cap + i];
         },
-        [&](float* F) { load_F(S, cap, i, F); }, g, st, v, C, Eq, mq, ax, aF, avq, aC, dE);
+        [&](float* F) { load_F(S, cap, i, F); }, g, st, v, C, pm, mq, ax, aF, avq, aC, dE);
     store_state(abar, cap, i, ax, aF, avq, aC);
     if (aE) aE[i] += dE;
 }
@@ -833,13 +816,15 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.wbuf = w.wbuf;
         a.scenes = w.scenes;
         a.cap = cap;
+        a.mt = w.mt;
         {
             ProfScope ps(w.prof, PK_ADJ_G2P, st);
             k_adj_g2p<<<w.nblocks_launch, TB, 0, st>>>(a);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, agv, apam, t.mp, t.flags);
+            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, agv, apam, t.mp, t.flags,
+                                             w.node_bc);
         }
         {
             ProfScope ps(w.prof, PK_ADJ_P2G, st);
@@ -851,7 +836,8 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
 
 hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* tape, int L, float4* abar,
                                  float4* part, float4* agv, float4* apam, float* aE_user, const float* E,
-                                 const float* mass, hipStream_t st, KernelProf* prof) {
+                                 const float* mass, const MatTable& mt, const uint32_t* node_bc, hipStream_t st,
+                                 KernelProf* prof) {
     const size_t G = (size_t)g.nx * g.ny * g.nz;
     const int nb = (n + 255) / 256;
     const int gb = (int)std::min<size_t>((G + 255) / 256, 16384);
@@ -863,11 +849,11 @@ hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* 
         }
         {
             ProfScope ps(prof, PK_ADJ_GRID, st);
-            k_adj_grid_plain<<<gb, 256, 0, st>>>(g, agv, apam, t.mp);
+            k_adj_grid_plain<<<gb, 256, 0, st>>>(g, agv, apam, t.mp, node_bc);
         }
         if (n > 0) {
             ProfScope ps(prof, PK_ADJ_P2G, st);
-            k_adj_p2g_plain<<<nb, 256, 0, st>>>(n, cap, g, t.S, E, mass, apam, part, abar, aE_user);
+            k_adj_p2g_plain<<<nb, 256, 0, st>>>(n, cap, g, t.S, E, mass, apam, part, abar, aE_user, mt);
         }
     }
     return hipGetLastError();

@@ -4,8 +4,7 @@
 // P2G, a dense grid clear and a dense grid update, separate kernels.  It is the
 // baseline the tiled path is measured against, and a second independent GPU
 // implementation for parity tests.  Also holds the layout helpers.
-#include "pg_device.h"
-#include "pg_kernels.h"
+#include "pg_tile.h"
 
 namespace pg {
 namespace {
@@ -76,18 +75,16 @@ __global__ void k_grid_split(size_t G, const float4* __restrict__ mp, const floa
 // P2G: mp[node] = (m, px, py, pz) += w (m, m v + A (x_i - x_p))
 __global__ void __launch_bounds__(256) k_p2g_plain(int n, size_t cap, const float4* __restrict__ S,
                                                    const float* __restrict__ mass, const float* __restrict__ E,
-                                                   MlsGeom g, float4* __restrict__ mp) {
+                                                   MlsGeom g, MatTable mt, float4* __restrict__ mp) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     float x[3], F[9], v[3], C[9];
     load_state(S, cap, q, x, F, v, C);
     const float mq = mass ? mass[q] : g.p_mass;
-    const float Eq = E ? E[q] : g.E;
-    const float mu = Eq / (2.f * (1.f + g.nu));
-    const float lam = Eq * g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+    const PMat pm = particle_mat(g, mt, (uint32_t)q, E, q, mq);
     float tau[9], A[9];
-    kirchhoff(g.material, mu, lam, F, tau);
-    const float kscale = -g.dt * g.p_vol * 4.f * g.inv_dx * g.inv_dx;
+    stress_of(pm, F, C, tau);
+    const float kscale = -g.dt * pm.vol * 4.f * g.inv_dx * g.inv_dx;
 #pragma unroll
     for (int i = 0; i < 9; ++i) A[i] = kscale * tau[i] + mq * C[i];
     Stencil s;
@@ -110,22 +107,14 @@ __global__ void __launch_bounds__(256) k_p2g_plain(int n, size_t cap, const floa
 }
 
 // grid update: v = p/m + dt g (m > 0), sticky walls; gv = (vx, vy, vz, m)
-__global__ void k_grid_plain(MlsGeom g, const float4* __restrict__ mp, float4* __restrict__ gv) {
+__global__ void k_grid_plain(MlsGeom g, const float4* __restrict__ mp, float4* __restrict__ gv,
+                             const uint32_t* __restrict__ node_bc) {
     const size_t G = (size_t)g.nx * g.ny * g.nz;
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < G; id += (size_t)gridDim.x * blockDim.x) {
         int i = (int)(id % g.nx);
         int j = (int)((id / g.nx) % g.ny);
         int k = (int)(id / ((size_t)g.nx * g.ny));
-        float4 a = mp[id];
-        bool sticky = i < g.bc || i >= g.nx - g.bc || j < g.bc || j >= g.ny - g.bc || k < g.bc || k >= g.nz - g.bc;
-        float4 r = make_float4(0.f, 0.f, 0.f, a.x);
-        if (a.x > 0.f && !sticky) {
-            float im = 1.f / a.x;
-            r.x = a.y * im + g.dt * g.g[0];
-            r.y = a.z * im + g.dt * g.g[1];
-            r.z = a.w * im + g.dt * g.g[2];
-        }
-        gv[id] = r;
+        gv[id] = node_velocity(g, i, j, k, node_bc ? node_bc[id] : 0u, mp[id]);
     }
 }
 
@@ -212,7 +201,8 @@ hipError_t grid_split(size_t G, const float4* mp, const float4* gv, float* mass,
 }
 
 hipError_t mls_plain_step(const MlsGeom& g, int n, size_t cap, float4* S, const float* mass, const float* E,
-                          float4* mp, float4* gv, hipStream_t st, KernelProf* prof) {
+                          const MatTable& mt, const uint32_t* node_bc, float4* mp, float4* gv, hipStream_t st,
+                          KernelProf* prof) {
     const size_t G = (size_t)g.nx * g.ny * g.nz;
     {
         ProfScope ps(prof, PK_CLEAR, st);
@@ -220,11 +210,11 @@ hipError_t mls_plain_step(const MlsGeom& g, int n, size_t cap, float4* S, const 
     }
     if (n > 0) {
         ProfScope ps(prof, PK_PLAIN_P2G, st);
-        k_p2g_plain<<<blocks_for(n, 256), 256, 0, st>>>(n, cap, S, mass, E, g, mp);
+        k_p2g_plain<<<blocks_for(n, 256), 256, 0, st>>>(n, cap, S, mass, E, g, mt, mp);
     }
     {
         ProfScope ps(prof, PK_GRID, st);
-        k_grid_plain<<<blocks_for(G, 256, 16384), 256, 0, st>>>(g, mp, gv);
+        k_grid_plain<<<blocks_for(G, 256, 16384), 256, 0, st>>>(g, mp, gv, node_bc);
     }
     if (n > 0) {
         ProfScope ps(prof, PK_PLAIN_G2P, st);

@@ -75,6 +75,7 @@ struct TileArgs {
     size_t cap;
     uint32_t nslots;         // tile slots (capacity of bflag and blist)
     uint32_t* err;           // binning error word (ERR_* bits)
+    MatTable mt;             // per-particle materials (recs NULL: the scene's)
 };
 
 
@@ -110,15 +111,11 @@ struct P2GVal {
     float q[3], a0[3], a1[3], a2[3];
 };
 
-__device__ __forceinline__ void p2g_prep(const MlsGeom& g, float Eq, float mq, const float* F, const float* C,
+__device__ __forceinline__ void p2g_prep(const MlsGeom& g, const PMat& pm, float mq, const float* F, const float* C,
                                          const float* v, const Stencil& st, P2GVal& p) {
-    const float kscale = -g.dt * g.p_vol * 4.f * g.inv_dx * g.inv_dx;
-    // Lame parameters per unit E: scene constants, so the divisions leave the loops
-    const float emu = 1.f / (2.f * (1.f + g.nu));
-    const float elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
-    const float mu = Eq * emu, lam = Eq * elam;
+    const float kscale = -g.dt * pm.vol * 4.f * g.inv_dx * g.inv_dx;
     float tau[9];
-    kirchhoff(g.material, mu, lam, F, tau);
+    stress_of(pm, F, C, tau);
     float A[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) A[c] = kscale * tau[c] + mq * C[c];
@@ -428,7 +425,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
             // is nearby, else with global atomics.
             // ob = window index of the particle's base before its G2P (own_ok only);
             // win = which of the wave's two windows
-            auto scatter = [&](const Stencil& st, const float* F, const float* C, const float* v, float Eq, float mq,
+            auto scatter = [&](const Stencil& st, const float* F, const float* C, const float* v, const PMat& pm, float mq,
                                bool own_ok, uint32_t oo, int ob, int win) {
                 const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
                 const int nb = (l2 * WN + l1) * WN + l0;
@@ -436,7 +433,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 const bool own = own_ok && inside && nb == ob;
                 const bool near = l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT;
                 P2GVal p;
-                p2g_prep(g, Eq, mq, F, C, v, st, p);
+                p2g_prep(g, pm, mq, F, C, v, st, p);
                 if (G2P && !own) {
                     const uint32_t e = near ? atomicAdd(&s_qe, 1u) : (uint32_t)QCAP;
                     if (e < QCAP) {
@@ -482,9 +479,9 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                 x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
                 F[0] = q0.w;
                 int ob = -1;
-                float Eq = g.E, mq = g.p_mass;
-                if (a.E_in) Eq = a.E_in[s];
+                float mq = g.p_mass;
                 if (a.m_in) mq = a.m_in[s];
+                const uint32_t uid = a.pid_in[s];
                 Stencil stn;   // stencil of the position P2G scatters from
                 if (G2P) {
                     Stencil st;
@@ -509,8 +506,8 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) x[c] += g.dt * v[c];
                     store_state(a.Sout, a.cap, o, x, F, v, C);
-                    a.pid_out[o] = a.pid_in[s];
-                    if (a.E_out && a.E_in) a.E_out[o] = Eq;
+                    a.pid_out[o] = uid;
+                    if (a.E_out && a.E_in) a.E_out[o] = a.E_in[s];
                     if (a.m_out && a.m_in) a.m_out[o] = mq;
                     if (a.src_out) a.src_out[o] = s;
                     if (P2G) {
@@ -528,7 +525,7 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     make_stencil(x, g.o, g.inv_dx, stn);
                     ob = ((stn.b[2] - t0[2]) * WN + (stn.b[1] - t0[1])) * WN + (stn.b[0] - t0[0]);
                 }
-                if (P2G) scatter(stn, F, C, v, Eq, mq, own_ok, oo, ob, win);
+                if (P2G) scatter(stn, F, C, v, particle_mat(g, a.mt, uid, a.E_in, s, mq), mq, own_ok, oo, ob, win);
             };
 
             // ---- rounds: cell-order index j = grp + lane G of group grp (grp < G); the
@@ -618,14 +615,15 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
                     const uint32_t o = beg + pb + lrank[q];
                     float x[3], F[9], v[3], C[9];
                     load_state(a.Sout, a.cap, o, x, F, v, C);
-                    float Eq = g.E, mq = g.p_mass;
-                    if (a.E_out && a.E_in) Eq = a.E_out[o];
+                    float mq = g.p_mass;
                     if (a.m_out && a.m_in) mq = a.m_out[o];
+                    const PMat pm = particle_mat(g, a.mt, a.pid_out[o], (a.E_out && a.E_in) ? a.E_out : nullptr, o,
+                                                 mq);
                     Stencil st;
                     make_stencil(x, g.o, g.inv_dx, st);
                     const int l0 = st.b[0] - t0[0], l1 = st.b[1] - t0[1], l2 = st.b[2] - t0[2];
                     P2GVal p;
-                    p2g_prep(g, Eq, mq, F, C, v, st, p);
+                    p2g_prep(g, pm, mq, F, C, v, st, p);
                     if (l0 >= -1 && l0 <= TT && l1 >= -1 && l1 <= TT && l2 >= -1 && l2 <= TT)
                         p2g_window_atomic(reinterpret_cast<float*>(mv), ((l2 + 1) * PN + l1 + 1) * PN + l0 + 1, st,
                                           p);
@@ -687,7 +685,8 @@ __device__ __forceinline__ void grid_block(uint32_t B, const SceneInfo* __restri
                                            const uint32_t* __restrict__ tscene,
                                            const uint32_t* __restrict__ tslot, uint32_t* __restrict__ bflag,
                                            const float4* __restrict__ wbuf, float4* __restrict__ mp_all,
-                                           float4* __restrict__ gv_all, uint32_t* __restrict__ flags8) {
+                                           float4* __restrict__ gv_all, uint32_t* __restrict__ flags8,
+                                           const uint32_t* __restrict__ node_bc) {
     // covered by a buffered window (bit 0), global-atomic P2G here (bit 1)
     const uint32_t fl = bflag[B];
     const bool fb = (fl & 2u) != 0u, wb = (fl & 1u) != 0u;
@@ -741,14 +740,7 @@ __device__ __forceinline__ void grid_block(uint32_t B, const SceneInfo* __restri
             q.w += u.w;
             if (!TAPE) mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        float4 r = make_float4(0.f, 0.f, 0.f, q.x);
-        if (q.x > 0.f && !is_sticky(g, i, j, k)) {
-            const float im = 1.f / q.x;
-            r.x = q.y * im + g.dt * g.g[0];
-            r.y = q.z * im + g.dt * g.g[1];
-            r.z = q.w * im + g.dt * g.g[2];
-        }
-        gv[id] = r;
+        gv[id] = node_velocity(g, i, j, k, node_bc ? node_bc[sc.goff + id] : 0u, q);
         if (TAPE) mp[id] = q;
     }
     if (lane == 0) {
@@ -767,10 +759,10 @@ __device__ __forceinline__ void grid_list(uint32_t nwg, const SceneInfo* __restr
                                           uint32_t* __restrict__ bflag, const uint32_t* __restrict__ blist,
                                           const uint32_t* __restrict__ nblist, const float4* __restrict__ wbuf,
                                           float4* __restrict__ mp_all, float4* __restrict__ gv_all,
-                                          uint32_t* __restrict__ flags8) {
+                                          uint32_t* __restrict__ flags8, const uint32_t* __restrict__ node_bc) {
     const uint32_t nl = *nblist;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < nl; i += nwg * 4)
-        grid_block<TAPE>(blist[i], scenes, tscene, tslot, bflag, wbuf, mp_all, gv_all, flags8);
+        grid_block<TAPE>(blist[i], scenes, tscene, tslot, bflag, wbuf, mp_all, gv_all, flags8, node_bc);
 }
 
 __global__ void k_list_reset(uint32_t* __restrict__ nblist) { *nblist = 0u; }
@@ -783,8 +775,9 @@ __global__ void __launch_bounds__(256) k_grid_nodes(const SceneInfo* __restrict_
                                                     const uint32_t* __restrict__ nblist,
                                                     const float4* __restrict__ wbuf,
                                                     float4* __restrict__ mp_all, float4* __restrict__ gv_all,
-                                                    uint32_t* __restrict__ flags8) {
-    grid_list<TAPE>(gridDim.x, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8);
+                                                    uint32_t* __restrict__ flags8,
+                                                    const uint32_t* __restrict__ node_bc) {
+    grid_list<TAPE>(gridDim.x, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8, node_bc);
 }
 
 // First binning of a call: tile key per storage slot, ranks from one global
@@ -889,11 +882,13 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
                                                         float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                         uint32_t* __restrict__ flags8, int ngrid,
                                                         const uint32_t* __restrict__ count,
-                                                        uint32_t* __restrict__ bsum) {
+                                                        uint32_t* __restrict__ bsum,
+                                                        const uint32_t* __restrict__ node_bc) {
     static_assert(SCAN_T == 256, "both parts run 256 threads");
     __shared__ uint32_t sh[SCAN_T];
     if ((int)blockIdx.x < ngrid)
-        grid_list<TAPE>((uint32_t)ngrid, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8);
+        grid_list<TAPE>((uint32_t)ngrid, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8,
+                        node_bc);
     else
         scan_a_block(blockIdx.x - ngrid, count, slots, bsum, sh);
 }
@@ -1006,6 +1001,7 @@ void fill_args(TileArgs& a, const TiledWork& w, size_t cap) {
     a.cap = cap;
     a.nslots = (uint32_t)w.tile_slots;
     a.err = w.err;
+    a.mt = w.mt;
 }
 
 // workgroups of the grid update (4 waves each, striding over the block list):
@@ -1018,7 +1014,7 @@ void grid_update(const TiledWork& w, const uint32_t* tslot, float4* mp, float4* 
                  hipStream_t st) {
     ProfScope ps(w.prof, PK_GRID, st);
     k_grid_nodes<TAPE><<<grid_wgs(w), 256, 0, st>>>(w.scenes, w.tscene, tslot, w.bflag, w.blist, w.nblist, w.wbuf,
-                                                     mp, gv, flags8);
+                                                     mp, gv, flags8, w.node_bc);
     // no binning follows a call's first grid update: empty the list here
     k_list_reset<<<1, 1, 0, st>>>(w.nblist);
 }
@@ -1032,7 +1028,8 @@ void grid_update_and_count(const TiledWork& w, const uint32_t* tslot, float4* mp
     const int ngrid = grid_wgs(w);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_grid_and_count<TAPE><<<ngrid + nb, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.bflag, w.blist,
-                                                        w.nblist, w.wbuf, mp, gv, flags8, ngrid, w.count, w.bsum);
+                                                        w.nblist, w.wbuf, mp, gv, flags8, ngrid, w.count, w.bsum,
+                                                        w.node_bc);
 }
 
 }  // namespace

@@ -75,7 +75,49 @@ PG_DEV void polar(const float* F, float* R) {
     }
 }
 
-enum { MAT_FIXED_COROTATED = 0, MAT_NEO_HOOKEAN = 1 };
+enum { MAT_FIXED_COROTATED = 0, MAT_NEO_HOOKEAN = 1, MAT_FLUID = 2 };
+
+// Constitutive data of one particle: the model and its Lame parameters,
+// viscosity and initial volume (resolved from the scene or the material table).
+struct PMat {
+    int model;
+    float mu, lam, visc, vol;
+    float emu, elam;   // d mu / dE, d lam / dE (the E adjoint)
+};
+
+// One material of a context's table (device copy of pg_material_desc).
+struct MatRec {
+    int model;
+    float E;
+    float emu, elam;   // Lame parameters per unit E
+    float visc;
+    float inv_rho;     // 1 / density: particle volume = mass / density (0: the scene's volume)
+    float pad[2];
+};
+
+// Boundary flags of a grid node (the reference's BoundaryType bits).
+enum : uint32_t { BC_DIRICHLET_X = 1u, BC_DIRICHLET_Y = 2u, BC_DIRICHLET_Z = 4u, BC_STICKY = 64u, BC_SLIP = 128u };
+
+// Velocity components a node keeps (bit c = component c): none in the sticky
+// band of `band` nodes or on a STICKY node; DIRICHLET_X/Y/Z drop their
+// component; SLIP drops each component that points out of the domain through
+// the nearer wall of its axis (index < n/2: negative; else positive), given the
+// node's velocity v before the mask.  NEUMANN bits leave it unchanged.
+PG_DEV uint32_t bc_keep(const int* idx, const int* n, int band, uint32_t flags, const float* v) {
+    bool wall = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) wall = wall || idx[a] < band || idx[a] >= n[a] - band;
+    if (wall || (flags & BC_STICKY)) return 0u;
+    uint32_t keep = 7u & ~flags;   // DIRICHLET_X/Y/Z are bits 0..2
+    if (flags & BC_SLIP) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const bool lower = 2 * idx[a] < n[a];
+            if (lower ? v[a] < 0.f : v[a] > 0.f) keep &= ~(1u << a);
+        }
+    }
+    return keep;
+}
 
 // Kirchhoff stress tau = P(F) F^T.
 //   fixed-corotated: 2 mu (F - R) F^T + lam (J-1) J I
@@ -179,6 +221,49 @@ PG_DEV void kirchhoff_vjp(int material, float mu, float lam, const float* F, con
 #pragma unroll
         for (int i = 0; i < 9; ++i) aF[i] += 2.f * mu * (aFm[i] + aTF[i] - aTR[i] - Ru[i]) + vol * FiT[i];
     }
+}
+
+// Kirchhoff stress of any model: the elastic models as kirchhoff(); the
+// Newtonian fluid (src/mpm_kernels.cu:131-149: Cauchy sigma = 2 visc sym(grad v),
+// grad v = the APIC C) as tau = J sigma = J visc (C + C^T).
+PG_DEV void stress_of(const PMat& m, const float* F, const float* C, float* tau) {
+    if (m.model == MAT_FLUID) {
+        const float jv = det3(F) * m.visc;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tau[3 * i + j] = jv * (C[3 * i + j] + C[3 * j + i]);
+    } else {
+        kirchhoff(m.model, m.mu, m.lam, F, tau);
+    }
+}
+
+// VJP of stress_of: aF += (d tau / d F)^T : at, aC += (d tau / d C)^T : at (fluid
+// only) and dE += (d tau / d E) : at (elastic models only).
+//   fluid: d tau = visc (C + C^T) dJ + J visc (dC + dC^T), dJ = J tr(F^-1 dF)
+PG_DEV void stress_vjp(const PMat& m, const float* F, const float* C, const float* at, float* aF, float* aC,
+                       float& dE) {
+    if (m.model == MAT_FLUID) {
+        float FiT[9];
+        const float J = inv_t3(F, FiT);
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s += at[3 * i + j] * (C[3 * i + j] + C[3 * j + i]);
+        const float js = m.visc * J * s;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) aF[i] += js * FiT[i];
+        const float jv = J * m.visc;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) aC[3 * i + j] += jv * (at[3 * i + j] + at[3 * j + i]);
+        return;
+    }
+    float amu = 0.f, alam = 0.f;
+    kirchhoff_vjp(m.model, m.mu, m.lam, F, at, aF, amu, alam);
+    dE += amu * m.emu + alam * m.elam;
 }
 
 // Quadratic B-spline stencil of MLS-MPM: base = floor(xs - 1/2), f = xs - base,

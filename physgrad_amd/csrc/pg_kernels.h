@@ -32,13 +32,25 @@ struct MlsGeom {
 // message of the last failure on this host thread (pg_mpm_last_error(NULL))
 void set_thread_error(const char* msg);
 
+// Per-particle material table of a context (device memory): recs[ids[u]] is
+// the material of the particle with caller index u.  recs == NULL: every
+// particle has its scene's material, Poisson ratio and volume.
+struct MatRec;
+struct MatTable {
+    const MatRec* recs;
+    const uint8_t* ids;
+};
+constexpr int MAX_MATERIALS = 256;
+
 struct KernelProf;
 hipError_t refcompat_step(const RefGeom& g, int n, float* x, float* v, const float* m, float* gm, float* gp, float* gv,
                           float dt, const float* grav, hipStream_t st, KernelProf* prof);
 
 // ---- MLS-MPM plain port: one thread per particle, global float atomics, dense grid
+// mt: material table (recs NULL: the scene's); node_bc: boundary flags per node (NULL: none)
 hipError_t mls_plain_step(const MlsGeom& g, int n, size_t cap, float4* S, const float* mass, const float* E,
-                          float4* mp, float4* gv, hipStream_t st, KernelProf* prof);
+                          const MatTable& mt, const uint32_t* node_bc, float4* mp, float4* gv, hipStream_t st,
+                          KernelProf* prof);
 
 // ---- layout conversion between user AoS arrays and the six float4 planes
 // pid (optional) maps storage slot -> user index; NULL means identity.
@@ -154,6 +166,8 @@ struct TiledWork {
     int max_entries;       // capacity of the active list
     int nblocks_launch;    // persistent grid size of the tile kernels
     uint32_t* err;         // [1] binning error word: ERR_* bits set by guarded writes, never cleared by a step
+    MatTable mt;           // per-particle materials (recs NULL: the scenes')
+    const uint32_t* node_bc;  // [total grid nodes] boundary flags per node, or NULL
 };
 
 // Bits of the binning error word.  The binning kernels check every index they
@@ -213,7 +227,8 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
 // tape[s].mp / .gv the dense grid of step s.
 hipError_t adjoint_segment_plain(const MlsGeom& g, int n, size_t cap, TapeStep* tape, int L, float4* abar,
                                  float4* part, float4* agv, float4* apam, float* aE_user, const float* E,
-                                 const float* mass, hipStream_t st, KernelProf* prof);
+                                 const float* mass, const MatTable& mt, const uint32_t* node_bc, hipStream_t st,
+                                 KernelProf* prof);
 // user-order adjoint arrays (x,v [3n], C,F [9n]; NULL = 0) -> adjoint planes in storage order (pid;
 // NULL = identity), and dL/dE aE (user order) -> aEs (storage order) when aEs is not NULL
 hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float* ax, const float* av,

@@ -171,4 +171,69 @@ __device__ __forceinline__ bool is_sticky(const MlsGeom& g, int i, int j, int k)
     return i < g.bc || i >= g.nx - g.bc || j < g.bc || j >= g.ny - g.bc || k < g.bc || k >= g.nz - g.bc;
 }
 
+// Material of the particle with caller index uid: its table entry when the
+// context has a table, else the scene's.  Young's modulus: Ep[ei] when the
+// particle array is given, else the material's (or scene's); initial volume:
+// mass / density for a table entry with a density, else the scene's V0.
+__device__ __forceinline__ PMat particle_mat(const MlsGeom& g, const MatTable& mt, uint32_t uid, const float* Ep,
+                                             size_t ei, float mq) {
+    PMat m;
+    if (mt.recs) {
+        const MatRec& r = mt.recs[mt.ids[uid]];
+        const float E = Ep ? Ep[ei] : r.E;
+        m.model = r.model;
+        m.emu = r.emu;
+        m.elam = r.elam;
+        m.mu = E * r.emu;
+        m.lam = E * r.elam;
+        m.visc = r.visc;
+        m.vol = r.inv_rho > 0.f ? mq * r.inv_rho : g.p_vol;
+    } else {
+        const float E = Ep ? Ep[ei] : g.E;
+        m.model = g.material;
+        m.emu = 1.f / (2.f * (1.f + g.nu));
+        m.elam = g.nu / ((1.f + g.nu) * (1.f - 2.f * g.nu));
+        m.mu = E * m.emu;
+        m.lam = E * m.elam;
+        m.visc = 0.f;
+        m.vol = g.p_vol;
+    }
+    return m;
+}
+
+// Grid velocity of a node from its summed (m, p): v = p/m + dt g masked by
+// the walls and the node's boundary flags (zero on an empty node).
+__device__ __forceinline__ float4 node_velocity(const MlsGeom& g, int i, int j, int k, uint32_t flags, float4 q) {
+    float4 r = make_float4(0.f, 0.f, 0.f, q.x);
+    if (q.x > 0.f) {
+        const float im = 1.f / q.x;
+        const float v[3] = {q.y * im + g.dt * g.g[0], q.z * im + g.dt * g.g[1], q.w * im + g.dt * g.g[2]};
+        const int idx[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+        const uint32_t keep = bc_keep(idx, n, g.bc, flags, v);
+        r.x = (keep & 1u) ? v[0] : 0.f;
+        r.y = (keep & 2u) ? v[1] : 0.f;
+        r.z = (keep & 4u) ? v[2] : 0.f;
+    }
+    return r;
+}
+
+// Reverse of node_velocity: the adjoints (am, ap) of the node's (m, p) from
+// the adjoint av of its velocity (components the mask dropped pass nothing).
+__device__ __forceinline__ float4 node_velocity_vjp(const MlsGeom& g, int i, int j, int k, uint32_t flags, float4 q,
+                                                    float4 av) {
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q.x > 0.f) {
+        const float im = 1.f / q.x;
+        const float v[3] = {q.y * im + g.dt * g.g[0], q.z * im + g.dt * g.g[1], q.w * im + g.dt * g.g[2]};
+        const int idx[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+        const uint32_t keep = bc_keep(idx, n, g.bc, flags, v);
+        const float a0 = (keep & 1u) ? av.x : 0.f, a1 = (keep & 2u) ? av.y : 0.f, a2 = (keep & 4u) ? av.z : 0.f;
+        r.y = a0 * im;
+        r.z = a1 * im;
+        r.w = a2 * im;
+        r.x = -(a0 * q.y + a1 * q.z + a2 * q.w) * im * im;
+    }
+    return r;
+}
+
 }  // namespace pg

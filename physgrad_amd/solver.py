@@ -153,6 +153,7 @@ class MPMContext:
         self.particle_E = None if E is None else E.clone()
         self.particle_mass = None if mass is None else mass.clone()
         self.rollouts = getattr(self, "rollouts", 0)
+        self.material_table, self.material_ids = None, None   # the library resets them too
 
     def get_particles(self):
         n = self.num_particles
@@ -178,6 +179,49 @@ class MPMContext:
 
     def step(self, nsteps: int = 1):
         check(self._L.pg_mpm_step(self._h, int(nsteps), _stream_ptr()), self._h)
+
+    def set_materials(self, table, ids=None):
+        """Per-particle materials: table = list of dicts (model, youngs_modulus,
+        poisson_ratio, density, viscosity) or PgMaterialDesc; ids [n] material
+        index per particle (caller order).  table None restores the scenes'
+        materials.  Call after set_particles (which resets them)."""
+        if table is None or len(table) == 0:
+            check(self._L.pg_mpm_set_materials(self._h, None, 0, None, _lib.DEVICE, _stream_ptr()), self._h)
+            self.material_table, self.material_ids = None, None
+            return
+        descs = []
+        for t in table:
+            if isinstance(t, _lib.PgMaterialDesc):
+                descs.append(t)
+                continue
+            d = _lib.PgMaterialDesc()
+            d.model = int(t.get("model", _lib.MAT_NEO_HOOKEAN))
+            d.youngs_modulus = float(t.get("youngs_modulus", 0.0))
+            d.poisson_ratio = float(t.get("poisson_ratio", 0.0))
+            d.density = float(t.get("density", 0.0))
+            d.viscosity = float(t.get("viscosity", 0.0))
+            descs.append(d)
+        arr = (_lib.PgMaterialDesc * len(descs))(*descs)
+        idt = None
+        if ids is not None:
+            idt = torch.as_tensor(ids).to(torch.uint8).to("cuda").contiguous()
+        check(self._L.pg_mpm_set_materials(self._h, arr, len(descs), _dptr(idt), _lib.DEVICE, _stream_ptr()),
+              self._h)
+        self.material_table = [{f: getattr(d, f) for f, _ in _lib.PgMaterialDesc._fields_} for d in descs]
+        self.material_ids = None if idt is None else idt.clone()
+
+    def set_boundary(self, flags=None):
+        """Per-node boundary flags ([num_nodes] uint32 BoundaryType bits, node
+        index (k ny + j) nx + i); None clears them."""
+        if flags is None:
+            check(self._L.pg_mpm_set_boundary(self._h, None, _lib.DEVICE, _stream_ptr()), self._h)
+            self.boundary_flags = None
+            return
+        t = torch.as_tensor(flags).to(torch.int64).to(torch.int32).to("cuda").contiguous().reshape(-1)
+        if t.numel() != self.num_nodes:
+            raise ValueError(f"{t.numel()} flags for {self.num_nodes} grid nodes")
+        check(self._L.pg_mpm_set_boundary(self._h, _dptr(t), _lib.DEVICE, _stream_ptr()), self._h)
+        self.boundary_flags = t.clone()
 
     def check_state(self):
         """Validates the particle state; returns (first_nonfinite, first_outside),
@@ -303,6 +347,7 @@ class MPMBatch(MPMContext):
         self.counts = counts
         self.particle_E = None if E is None else E.clone()
         self.particle_mass = None if mass is None else mass.clone()
+        self.material_table, self.material_ids = None, None
 
 
 @dataclass

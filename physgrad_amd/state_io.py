@@ -10,8 +10,9 @@ The reference's solver only prints at its output cadence
                                      point fields), readable by ParaView / VisIt
   save_checkpoint(ctx, path)         the whole solver state (parameters or scene
                                      table, particle x, v, C, F and per-particle
-                                     E and mass in the caller's order, clock) in
-                                     one .npz file
+                                     E and mass in the caller's order, material
+                                     table and ids, node boundary flags, clock)
+                                     in one .npz file
   load_checkpoint(path)              a new context in the saved state; stepping
                                      it continues the saved run
   run(ctx, total_time, ...)          the reference's run() loop: steps until the
@@ -118,12 +119,16 @@ def write_snapshot(ctx, path: str):
 # ----------------------------------------------------------------------------- checkpoints
 
 def pack_checkpoint(kind: str, config: Dict[str, np.ndarray], particles: Dict[str, np.ndarray], time: float,
-                    steps: int, counts=None) -> Dict[str, np.ndarray]:
+                    steps: int, counts=None, extra: Optional[Dict[str, np.ndarray]] = None) -> Dict[str, np.ndarray]:
     """The arrays of a checkpoint file: kind ("context" or "batch"), the
     parameter struct or scene table fields, particle arrays in the caller's
-    order, the clock and, for a batch, the per-scene particle counts."""
+    order, the clock, for a batch the per-scene particle counts, and extra
+    arrays stored as given (material table and ids, boundary flags)."""
     out = {"version": np.int32(CHECKPOINT_VERSION), "kind": np.array(kind), "time": np.float64(time),
            "steps": np.int64(steps)}
+    for k, v in (extra or {}).items():
+        if v is not None:
+            out["x_" + k] = np.asarray(v)
     for k, v in config.items():
         out["cfg_" + k] = np.asarray(v)
     for k, v in particles.items():
@@ -140,8 +145,35 @@ def unpack_checkpoint(arrays) -> dict:
     out = {"kind": str(arrays["kind"]), "time": float(arrays["time"]), "steps": int(arrays["steps"]),
            "config": {k[4:]: arrays[k] for k in arrays.files if k.startswith("cfg_")},
            "particles": {k[2:]: arrays[k] for k in arrays.files if k.startswith("p_")},
-           "counts": arrays["counts"].tolist() if "counts" in arrays.files else None}
+           "counts": arrays["counts"].tolist() if "counts" in arrays.files else None,
+           "extra": {k[2:]: arrays[k] for k in arrays.files if k.startswith("x_")}}
     return out
+
+
+_MAT_FIELDS = ("model", "youngs_modulus", "poisson_ratio", "density", "viscosity")
+
+
+def _extras(ctx) -> Dict[str, np.ndarray]:
+    ex = {}
+    table = getattr(ctx, "material_table", None)
+    if table:
+        for f in _MAT_FIELDS:
+            ex["mat_" + f] = np.array([t[f] for t in table])
+        ids = getattr(ctx, "material_ids", None)
+        ex["mat_ids"] = None if ids is None else ids.detach().cpu().numpy().astype(np.uint8)
+    flags = getattr(ctx, "boundary_flags", None)
+    if flags is not None:
+        ex["bc_flags"] = flags.detach().cpu().numpy().astype(np.uint32)
+    return ex
+
+
+def _restore_extras(ctx, ex: Dict[str, np.ndarray]):
+    if "mat_model" in ex:
+        n = len(ex["mat_model"])
+        table = [{f: ex["mat_" + f][m].item() for f in _MAT_FIELDS} for m in range(n)]
+        ctx.set_materials(table, ex.get("mat_ids"))
+    if "bc_flags" in ex:
+        ctx.set_boundary(ex["bc_flags"])
 
 
 def save_checkpoint(ctx, path: str):
@@ -155,10 +187,10 @@ def save_checkpoint(ctx, path: str):
     t, steps = ctx.time()
     if isinstance(ctx, MPMBatch):
         cfg = {f: np.array([getattr(d, f) for d in ctx.scenes]) for f in _SCENE_FIELDS}
-        arrays = pack_checkpoint("batch", cfg, p, t, steps, counts=ctx.counts)
+        arrays = pack_checkpoint("batch", cfg, p, t, steps, counts=ctx.counts, extra=_extras(ctx))
     else:
         cfg = {f: np.array(getattr(ctx.params, f)) for f in _PARAM_FIELDS}
-        arrays = pack_checkpoint("context", cfg, p, t, steps)
+        arrays = pack_checkpoint("context", cfg, p, t, steps, extra=_extras(ctx))
     tmp = path + ".tmp.npz"
     np.savez(tmp, **arrays)
     os.replace(tmp, path)
@@ -185,6 +217,7 @@ def load_checkpoint(path: str):
             setattr(prm, k, type(getattr(prm, k))(cfg[k].item()))
         ctx = MPMContext(prm)
         ctx.set_particles(p["x"], p["v"], p.get("C"), p.get("F"), p.get("mass"), p.get("E"))
+    _restore_extras(ctx, ck["extra"])
     ctx.set_time(ck["time"], ck["steps"])
     return ctx
 
