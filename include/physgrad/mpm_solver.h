@@ -13,27 +13,36 @@
 //
 // The step runs on the GPU through the C-ABI (include/physgrad_mpm.h) as one
 // MLS-MPM context (tiled implementation, hipGraph replay of each step):
-// APIC transfers, F <- (I + dt C) F, the reference's neo-Hookean stress
-// (mpm_kernels.cu:97-128, used as Kirchhoff stress tau = J sigma), grid
-// v = p/m + dt g, sticky walls.  Differences from the reference, each because
-// the reference's version cannot run as written:
+// APIC transfers, F <- (I + dt C) F, per-particle materials as the reference's
+// fusedG2P2GKernel selects them (mpm_kernels.cu:407-421): FLUID particles the
+// Newtonian fluid (:131-149), every other type the neo-Hookean model
+// (:97-128), each with its database entry's Young's modulus, Poisson ratio,
+// viscosity and density (volume = mass / density, mpm_solver.h:490-493), the
+// stress entering P2G as the Kirchhoff stress tau = J sigma; grid
+// v = p/m + dt g and the node's BoundaryType flags (mpm_kernels.cu:536-539,
+// plus STICKY and SLIP, which the reference declares and never applies).
+// Differences from the reference, each because the reference's version
+// cannot run as written:
 //   * The reference builds ParticleAoSoA with max_particles entries, so its
 //     addParticles always throws; here the container starts empty and
 //     max_particles is the capacity.
 //   * Material lookup: the reference indexes the database by the enum value
 //     (ELASTIC -> entry 0 = water, nu = 0.5, a singular elastic model); here a
 //     particle of type t uses the first database entry whose type is t (or,
-//     failing that, entry t).  Every type is integrated with the neo-Hookean
-//     model, as the reference's kernel does for all types but FLUID; FLUID
-//     particles are rejected.  All particles must share one Poisson ratio and
-//     one volume (mass / density): the per-particle Young's modulus and mass
-//     are kept.
-//   * Boundaries: the reference marks the y = 0 node plane DIRICHLET_Y; here
-//     every node within config.boundary_nodes of a wall is sticky (v = 0, flag
-//     STICKY in getGrid()).
+//     failing that, entry t).
+//   * Boundaries: the reference marks the y = 0 node plane DIRICHLET_Y but
+//     never copies the flags to the device (d_grid_boundary_conditions_ is
+//     left unset); here the flags of grid_ are uploaded and applied.  The
+//     plane is the whole y = 0 plane (the reference's loop runs its z index
+//     up to dims.y).  Every node within config.boundary_nodes of a wall is
+//     also sticky (flag STICKY), so particles cannot leave the grid.
 //   * getGrid(): momentum is m v of the post-update velocity and forces are
 //     zero (the fused kernels never store them); getParticles() refreshes x, v,
-//     F and the Cauchy stress computed from F.
+//     F and the Cauchy stress (neo-Hookean from F, or the fluid's
+//     2 viscosity sym(C)).
+//   * outputData() (print-only in the reference, :501-504) writes a legacy
+//     binary VTK file of the particles to output_directory, or calls the
+//     callback given to setOutputCallback.
 //   * Informational printing happens only with enable_debug_output.
 //
 // Link with libphysgrad_mpm.so.  T must be float.  Do not include together
@@ -51,6 +60,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
 #include <functional>
 #include <iostream>
 #include <stdexcept>
@@ -499,7 +511,9 @@ public:
     using grid_container = MPMGrid<T>;
 
     explicit MPMSolver(const MPMSolverConfig& config = MPMSolverConfig{})
-        : config_(config), grid_(config.grid_dimensions, config.grid_spacing, config.domain_origin) {}
+        : config_(config), grid_(config.grid_dimensions, config.grid_spacing, config.domain_origin) {
+        setupBoundaryConditions();
+    }
     ~MPMSolver() { cleanup(); }
     MPMSolver(const MPMSolver&) = delete;
     MPMSolver& operator=(const MPMSolver&) = delete;
@@ -526,11 +540,9 @@ public:
         p.material = PG_MAT_NEO_HOOKEAN;
         p.boundary_thickness = config_.boundary_nodes;
         p.max_particles = config_.max_particles;
-        resolveMaterials(p);
         check(pg_mpm_create(&p, &ctx_));
         check(pg_mpm_set_graph(ctx_, 1));
         if (config_.enable_profiling) check(pg_mpm_set_profiling(ctx_, 1));
-        setupBoundaryConditions();
         upload();
         const size_t n = particles_.size(), G = grid_.getTotalNodes();
         metrics_.particle_memory = n * (24 * sizeof(T) + 2 * sizeof(uint32_t));
@@ -549,8 +561,6 @@ public:
             throw std::runtime_error("addParticles: positions, velocities and masses differ in length");
         if (old + add > static_cast<size_t>(config_.max_particles))
             throw std::runtime_error("Exceeded maximum particle count");
-        if (material_type == MaterialType::FLUID)
-            throw std::runtime_error("addParticles: FLUID is not integrated by the elastic MLS-MPM path");
         syncParticles();
         particles_.resize(old + add);
         const MaterialParameters& mat = material_db_.getMaterial(material_db_.idForType(material_type));
@@ -579,6 +589,7 @@ public:
         if (!ctx_) throw std::runtime_error("Solver not initialized");
         const auto t0 = std::chrono::high_resolution_clock::now();
         if (dirty_) upload();
+        if (bc_dirty_) uploadBoundary();
         check(pg_mpm_step(ctx_, 1, nullptr));
         host_fresh_ = false;
         grid_fresh_ = false;
@@ -646,9 +657,76 @@ public:
     float getCurrentTime() const { return current_time_; }
     int getCurrentStep() const { return current_step_; }
 
-    // not in the reference: replaces the print-only outputData (:501-504)
+    // not in the reference: replaces the VTK output of outputData
     void setOutputCallback(std::function<void(const MPMSolver&)> cb) { output_cb_ = std::move(cb); }
+    // not in the reference: the material database (changes apply from the
+    // next addParticles or initialize)
     MaterialDatabase& getMaterialDatabase() { return material_db_; }
+    // not in the reference: BoundaryType bits of a grid node (index
+    // getLinearIndex(i, j, k)); applied from the next step()
+    void setBoundaryCondition(size_t node, uint32_t flags) {
+        if (node >= grid_.getTotalNodes()) throw std::runtime_error("setBoundaryCondition: node out of range");
+        grid_.setBoundaryCondition(node, flags);
+        bc_dirty_ = true;
+    }
+    // Writes the particles as legacy binary VTK polydata: points, VECTORS
+    // velocity, SCALARS mass and J = det F.
+    void writeVTK(const std::string& path) const {
+        const particle_container& P = getParticles();
+        const size_t n = P.size();
+        std::FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f) throw std::runtime_error("writeVTK: cannot open " + path);
+        std::string hdr = "# vtk DataFile Version 3.0\nphysgrad_amd MPMSolver t=" + std::to_string(current_time_) +
+                          "\nBINARY\nDATASET POLYDATA\nPOINTS " + std::to_string(n) + " float\n";
+        std::fwrite(hdr.data(), 1, hdr.size(), f);
+        std::vector<uint32_t> buf;
+        auto put = [&](const std::vector<uint32_t>& b) { std::fwrite(b.data(), 4, b.size(), f); };
+        auto be = [](float v) {
+            uint32_t u;
+            std::memcpy(&u, &v, 4);
+            return __builtin_bswap32(u);
+        };
+        buf.resize(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            T x, y, z;
+            P.getPosition(i, x, y, z);
+            buf[3 * i] = be(x); buf[3 * i + 1] = be(y); buf[3 * i + 2] = be(z);
+        }
+        put(buf);
+        hdr = "\nVERTICES " + std::to_string(n) + " " + std::to_string(2 * n) + "\n";
+        std::fwrite(hdr.data(), 1, hdr.size(), f);
+        buf.resize(2 * n);
+        for (size_t i = 0; i < n; ++i) {
+            buf[2 * i] = __builtin_bswap32(1u);
+            buf[2 * i + 1] = __builtin_bswap32(static_cast<uint32_t>(i));
+        }
+        put(buf);
+        hdr = "\nPOINT_DATA " + std::to_string(n) + "\nVECTORS velocity float\n";
+        std::fwrite(hdr.data(), 1, hdr.size(), f);
+        buf.resize(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            T x, y, z;
+            P.getVelocity(i, x, y, z);
+            buf[3 * i] = be(x); buf[3 * i + 1] = be(y); buf[3 * i + 2] = be(z);
+        }
+        put(buf);
+        hdr = "\nSCALARS mass float 1\nLOOKUP_TABLE default\n";
+        std::fwrite(hdr.data(), 1, hdr.size(), f);
+        buf.resize(n);
+        for (size_t i = 0; i < n; ++i) buf[i] = be(P.getMass(i));
+        put(buf);
+        hdr = "\nSCALARS J float 1\nLOOKUP_TABLE default\n";
+        std::fwrite(hdr.data(), 1, hdr.size(), f);
+        for (size_t i = 0; i < n; ++i) {
+            T F[9];
+            P.getDeformationGradient(i, F);
+            buf[i] = be(F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                        F[2] * (F[3] * F[7] - F[4] * F[6]));
+        }
+        put(buf);
+        std::fputc('\n', f);
+        if (std::fclose(f) != 0) throw std::runtime_error("writeVTK: write to " + path + " failed");
+    }
 
 private:
     static void fail(const pg_mpm_ctx* c) {
@@ -658,28 +736,38 @@ private:
         if (rc != PG_OK) fail(ctx_);
     }
 
-    // One Poisson ratio and one particle volume for the context; E per particle.
-    void resolveMaterials(pg_mpm_params& p) {
+    // Material table of the database entries the particles use, and each
+    // particle's index into it.
+    void resolveMaterials() {
         const size_t n = particles_.size();
-        float nu = -1.f, vol = -1.f;
-        E_.resize(n);
+        std::vector<int> slot(material_db_.getNumMaterials(), -1);
+        table_.clear();
+        ids_.assign(n, 0);
         for (size_t i = 0; i < n; ++i) {
-            const MaterialParameters& m = material_db_.getMaterial(material_db_.idForType(particles_.getMaterialType(i)));
-            if (nu < 0.f) nu = m.poisson_ratio;
-            if (m.poisson_ratio != nu)
-                throw std::runtime_error("MPMSolver: particles with different Poisson ratios in one solver");
-            const float v = particles_.getVolume(i);
-            if (vol < 0.f) vol = v;
-            if (std::fabs(v - vol) > 1e-5f * vol)
-                throw std::runtime_error("MPMSolver: particles with different volumes (mass / density) in one solver");
-            E_[i] = m.youngs_modulus;
+            const uint32_t id = material_db_.idForType(particles_.getMaterialType(i));
+            if (id >= slot.size())
+                throw std::runtime_error("MPMSolver: no material database entry for particle type " +
+                                         std::to_string(static_cast<uint32_t>(particles_.getMaterialType(i))));
+            if (slot[id] < 0) {
+                if (table_.size() == 256) throw std::runtime_error("MPMSolver: more than 256 materials in use");
+                const MaterialParameters& m = material_db_.getMaterial(id);
+                pg_material_desc d;
+                const bool fluid = m.type == MaterialType::FLUID;
+                d.model = fluid ? PG_MAT_FLUID : PG_MAT_NEO_HOOKEAN;
+                d.youngs_modulus = fluid ? 0.f : m.youngs_modulus;
+                d.poisson_ratio = fluid ? 0.f : m.poisson_ratio;
+                d.density = m.density;
+                d.viscosity = fluid ? m.viscosity : 0.f;
+                slot[id] = static_cast<int>(table_.size());
+                table_.push_back(d);
+            }
+            ids_[i] = static_cast<uint8_t>(slot[id]);
         }
-        if (nu >= 0.5f) throw std::runtime_error("MPMSolver: Poisson ratio must be below 0.5");
-        p.poisson_ratio = nu < 0.f ? 0.3f : nu;
-        p.particle_volume = vol < 0.f ? 1e-6f : vol;
-        p.density = n ? particles_.getMass(0) / p.particle_volume : 1000.f;
     }
 
+    // The reference's floor (DIRICHLET_Y on the y = 0 plane, mpm_solver.h:
+    // 472-484) plus the sticky band of config.boundary_nodes nodes; set at
+    // construction, so flags a caller sets afterwards are kept.
     void setupBoundaryConditions() {
         const int3 d = config_.grid_dimensions;
         const int b = config_.boundary_nodes;
@@ -687,9 +775,16 @@ private:
             for (int j = 0; j < d.y; ++j)
                 for (int i = 0; i < d.x; ++i) {
                     const bool s = i < b || j < b || k < b || i >= d.x - b || j >= d.y - b || k >= d.z - b;
-                    grid_.setBoundaryCondition(grid_.getLinearIndex(i, j, k),
-                                               s ? static_cast<uint32_t>(BoundaryType::STICKY) : 0u);
+                    uint32_t f = s ? static_cast<uint32_t>(BoundaryType::STICKY) : 0u;
+                    if (j == 0) f |= static_cast<uint32_t>(BoundaryType::DIRICHLET_Y);
+                    grid_.setBoundaryCondition(grid_.getLinearIndex(i, j, k), f);
                 }
+        bc_dirty_ = true;
+    }
+
+    void uploadBoundary() {
+        check(pg_mpm_set_boundary(ctx_, grid_.getBoundaryConditionData(), PG_HOST, nullptr));
+        bc_dirty_ = false;
     }
 
     void upload() {
@@ -702,8 +797,12 @@ private:
             m[i] = particles_.getMass(i);
         }
         C_.resize(9 * n, T{0});
+        resolveMaterials();
         check(pg_mpm_set_particles(ctx_, static_cast<int>(n), x.data(), v.data(), C_.data(), F.data(), m.data(),
-                                   E_.data(), PG_HOST, nullptr));
+                                   nullptr, PG_HOST, nullptr));
+        if (n) check(pg_mpm_set_materials(ctx_, table_.data(), static_cast<int>(table_.size()), ids_.data(), PG_HOST,
+                                          nullptr));
+        uploadBoundary();
         dirty_ = false;
         host_fresh_ = true;
     }
@@ -714,14 +813,14 @@ private:
         std::vector<T> x(3 * n), v(3 * n), F(9 * n);
         C_.resize(9 * n);
         check(pg_mpm_get_particles(ctx_, x.data(), v.data(), C_.data(), F.data(), PG_HOST, nullptr));
-        const float nu = n ? material_db_.getMaterial(material_db_.idForType(particles_.getMaterialType(0))).poisson_ratio
-                           : 0.3f;
         for (size_t i = 0; i < n; ++i) {
             particles_.setPosition(i, x[3 * i], x[3 * i + 1], x[3 * i + 2]);
             particles_.setVelocity(i, v[3 * i], v[3 * i + 1], v[3 * i + 2]);
             particles_.setDeformationGradient(i, &F[9 * i]);
+            const pg_material_desc& d = table_[ids_[i]];
             T s[6];
-            cauchy(&F[9 * i], E_[i], nu, s);
+            if (d.model == PG_MAT_FLUID) fluidStress(&C_[9 * i], d.viscosity, s);
+            else cauchy(&F[9 * i], d.youngs_modulus, d.poisson_ratio, s);
             particles_.setStress(i, s);
         }
         host_fresh_ = true;
@@ -745,6 +844,17 @@ private:
         s[5] = static_cast<T>(a * B(1, 2));
     }
 
+    // Newtonian fluid stress 2 viscosity sym(grad v) with grad v the APIC C
+    // (mpm_kernels.cu:131-149), stored xx yy zz xy xz yz.
+    static void fluidStress(const T* C, float visc, T* s) {
+        s[0] = 2.f * visc * C[0];
+        s[1] = 2.f * visc * C[4];
+        s[2] = 2.f * visc * C[8];
+        s[3] = visc * (C[1] + C[3]);
+        s[4] = visc * (C[2] + C[6]);
+        s[5] = visc * (C[5] + C[7]);
+    }
+
     void syncGrid() {
         if (!ctx_ || grid_fresh_) return;
         const size_t G = grid_.getTotalNodes();
@@ -759,8 +869,15 @@ private:
     }
 
     void outputData() {
-        if (output_cb_) output_cb_(*this);
-        else if (config_.enable_debug_output) std::cout << "Writing output at t=" << current_time_ << "\n";
+        if (config_.enable_debug_output) std::cout << "Writing output at t=" << current_time_ << "\n";
+        if (output_cb_) {
+            output_cb_(*this);
+            return;
+        }
+        std::filesystem::create_directories(config_.output_directory);
+        char name[32];
+        std::snprintf(name, sizeof(name), "particles_%06d.vtk", current_step_);
+        writeVTK((std::filesystem::path(config_.output_directory) / name).string());
     }
 
     void cleanup() {
@@ -774,9 +891,11 @@ private:
     particle_container particles_;
     grid_container grid_;
     pg_mpm_ctx* ctx_ = nullptr;
-    std::vector<T> C_, E_;
+    std::vector<T> C_;
+    std::vector<pg_material_desc> table_;
+    std::vector<uint8_t> ids_;
     std::function<void(const MPMSolver&)> output_cb_;
-    bool dirty_ = true, host_fresh_ = true, grid_fresh_ = true;
+    bool dirty_ = true, host_fresh_ = true, grid_fresh_ = true, bc_dirty_ = true;
     double host_time_ = 0.0;
     float current_time_ = 0.0f;
     int current_step_ = 0;

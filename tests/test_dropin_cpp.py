@@ -93,52 +93,139 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-@pytest.mark.gpu
-def test_gpu_solver_dropin_vs_oracle(gpu, oracle_lib, tmp_path):
-    """MPMSolver<float> (jello: neo-Hookean E 1e4 nu 0.3 rho 1100) for 5 steps
-    against the FP64 MLS-MPM oracle with the same model."""
-    from oracle import oracle as O
-    from physgrad_amd.scenes import Scene
-    exe = build_driver(tmp_path, "dropin_gpu_solver")
-    rng = np.random.default_rng(12)
-    nx, n, steps, dt = 16, 512, 5, 1e-4
-    x = (0.3 + 0.3 * rng.random((n, 3))).astype(np.float32)
-    v = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
-    vol = 0.3 ** 3 / n
-    m = np.full(n, 1100.0 * vol, np.float32)
+# MaterialDatabase defaults of the drop-in (the reference's, mpm_data_structures.h:
+# 548-611): type -> (model, E, nu, density, viscosity) of the first entry of that type
+DB = {0: (1, 1e4, 0.3, 1100.0, 0.0),          # ELASTIC -> jello, neo-Hookean
+      2: (2, 0.0, 0.0, 1000.0, 0.001),        # FLUID -> water, Newtonian
+      4: (1, 1.4e5, 0.2, 400.0, 0.0)}         # SNOW -> snow, neo-Hookean (no plasticity)
+BC_DY, BC_STICKY = 2, 64
+
+
+def _drive_gpu_solver(exe, tmp_path, nx, x, v, m, types, steps, dt, use_run=0, jplane=-1, outdir=None):
+    n = len(m)
     inp, out = os.path.join(str(tmp_path), "in.bin"), os.path.join(str(tmp_path), "out.bin")
     with open(inp, "wb") as fh:
-        np.array([nx, n, steps], np.int32).tofile(fh)
+        np.array([nx, n, steps, use_run, jplane], np.int32).tofile(fh)
         np.array([dt], np.float32).tofile(fh)
-        x.tofile(fh)
-        v.tofile(fh)
-        m.tofile(fh)
-    print("gpu solver drop-in", n, "particles", steps, "steps", flush=True)
-    r = subprocess.run([exe, inp, out], capture_output=True, text=True, timeout=120)
+        x.astype(np.float32).tofile(fh)
+        v.astype(np.float32).tofile(fh)
+        m.astype(np.float32).tofile(fh)
+        types.astype(np.int32).tofile(fh)
+    r = subprocess.run([exe, inp, out] + ([outdir] if outdir else []), capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert "threw 1" in r.stdout and f"steps {steps}" in r.stdout, r.stdout
     d = np.fromfile(out, np.float32)
-    gx, gv, gF, gs = d[:3 * n], d[3 * n:6 * n], d[6 * n:15 * n], d[15 * n:21 * n]
-    gm = d[21 * n:]
-    sc = Scene("dropin", (nx,) * 3, 1.0 / nx, dt, steps, 1, 1e4, 0.3, 1100.0, vol, x, v, gravity=(0.0, -9.81, 0.0))
-    want = O.mls_run(O.scene_params(sc), x, v, None, None, m.astype(np.float64), 1e4, steps)
+    G = nx ** 3
+    res = dict(x=d[:3 * n].reshape(n, 3), v=d[3 * n:6 * n].reshape(n, 3), F=d[6 * n:15 * n].reshape(n, 3, 3),
+               s=d[15 * n:21 * n].reshape(n, 6), gm=d[21 * n:21 * n + G],
+               flags=np.fromfile(out, np.uint32)[21 * n + G:])
+    return r.stdout, res
+
+
+def _solver_scene(rng, n, types_pool):
+    x = (0.3 + 0.3 * rng.random((n, 3))).astype(np.float32)
+    v = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+    types = rng.choice(types_pool, n).astype(np.int32)
+    vol = 0.3 ** 3 / n
+    rho = np.array([DB[t][3] for t in types])
+    m = (rho * vol).astype(np.float32)
+    return x, v, m, types
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["elastic", "mixed"])
+def test_gpu_solver_dropin_vs_oracle(gpu, oracle_lib, tmp_path, kind):
+    """MPMSolver<float> for 5 steps against the FP64 MLS-MPM oracle with the
+    same per-particle materials and node flags.  elastic: all jello
+    (neo-Hookean E 1e4 nu 0.3 rho 1100); mixed: jello, water (Newtonian
+    fluid) and snow particles, plus a DIRICHLET_Y node plane at y = nx/2 set
+    through setBoundaryCondition."""
+    O = oracle_lib
+    exe = build_driver(tmp_path, "dropin_gpu_solver")
+    rng = np.random.default_rng(12 if kind == "elastic" else 13)
+    nx, n, steps, dt = 16, 512, 5, 1e-4
+    x, v, m, types = _solver_scene(rng, n, [0] if kind == "elastic" else [0, 2, 4])
+    jplane = -1 if kind == "elastic" else nx // 2
+    print("gpu solver drop-in", kind, n, "particles", steps, "steps", flush=True)
+    stdout, res = _drive_gpu_solver(exe, tmp_path, nx, x, v, m, types, steps, dt, jplane=jplane)
+    assert "threw 1" in stdout and f"steps {steps}" in stdout, stdout
+    # the driver adds particles grouped by type
+    order = np.argsort(types, kind="stable")
+    x, v, m, types = x[order], v[order], m[order], types[order]
+    # node flags: sticky band of 3, the reference's y = 0 floor, the caller's plane
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nx), indexing="ij")
+    ii, jj, kk = (a.transpose(2, 1, 0).reshape(-1) for a in (ii, jj, kk))   # node (k ny + j) nx + i
+    band = (np.minimum(np.minimum(ii, jj), kk) < 3) | (np.maximum(np.maximum(ii, jj), kk) >= nx - 3)
+    want_flags = np.where(band, BC_STICKY, 0) | np.where(jj == 0, BC_DY, 0) | np.where(jj == jplane, BC_DY, 0)
+    np.testing.assert_array_equal(res["flags"], want_flags.astype(np.uint32))
+    prm = O.mls_params((nx,) * 3, 1.0 / nx, (0, 0, 0), dt, (0.0, -9.81, 0.0), 3, 1, 0.3, 0.0)
+    model = np.array([DB[t][0] for t in types])
+    E = np.array([DB[t][1] for t in types])
+    nu = np.array([DB[t][2] for t in types])
+    visc = np.array([DB[t][4] for t in types])
+    vol = m.astype(np.float64) / np.array([DB[t][3] for t in types])
+    want = O.mls_run_ext(prm, x, v, None, None, m.astype(np.float64), E, steps, model=model, nu=nu, visc=visc,
+                         vol=vol, flags=res["flags"])
+    gx, gv, gF = res["x"], res["v"], res["F"]
     # positions to FP32 resolution; the 5-step displacement (~1e-4) carries
     # the rounding of x itself (ulp 6e-8 at 0.5), hence the looser bar
-    assert _rel(gx, want["x"].reshape(-1)) <= 1e-6
-    assert _rel(gx - x.reshape(-1), want["x"].reshape(-1) - x.reshape(-1)) <= 5e-3
-    assert _rel(gv, want["v"].reshape(-1)) <= 1e-4
-    I = np.tile(np.eye(3).reshape(-1), n)
-    assert _rel(gF - I, want["F"].reshape(-1) - I) <= 5e-3
-    assert _rel(gF, want["F"].reshape(-1)) <= 1e-6
-    # Cauchy stress of the returned F, neo-Hookean (mpm_kernels.cu:97-128)
-    F = gF.reshape(n, 3, 3).astype(np.float64)
-    mu, lam = 1e4 / 2.6, 1e4 * 0.3 / (1.3 * 0.4)
+    assert _rel(gx, want["x"]) <= 1e-6
+    assert _rel(gx - x, want["x"] - x) <= 5e-3
+    assert _rel(gv, want["v"]) <= 1e-4
+    I = np.eye(3)
+    assert _rel(gF - I, want["F"] - I) <= 5e-3
+    assert _rel(gF, want["F"]) <= 1e-6
+    # Cauchy stress: neo-Hookean of the returned F (mpm_kernels.cu:97-128);
+    # water 2 visc sym(C) of the oracle's C (:131-149)
+    F = gF.astype(np.float64)
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
     J = np.linalg.det(F)
     B = F @ F.transpose(0, 2, 1)
-    S = mu / J[:, None, None] * (B - np.eye(3)) + (lam * np.log(J) / J)[:, None, None] * np.eye(3)
+    S = (mu / J)[:, None, None] * (B - I) + (lam * np.log(J) / J)[:, None, None] * I
+    Cw = want["C"]
+    Sf = visc[:, None, None] * (Cw + Cw.transpose(0, 2, 1))
+    fluid = model == 2
+    S[fluid] = Sf[fluid]
     want_s = np.stack([S[:, 0, 0], S[:, 1, 1], S[:, 2, 2], S[:, 0, 1], S[:, 0, 2], S[:, 1, 2]], 1)
-    assert _rel(gs, want_s.reshape(-1)) <= 1e-4
-    assert abs(gm.sum() / m.sum() - 1) <= 1e-5
+    assert _rel(res["s"][~fluid], want_s[~fluid]) <= 1e-4
+    if fluid.any():
+        assert _rel(res["s"][fluid], want_s[fluid]) <= 1e-4
+    assert abs(res["gm"].sum() / m.sum() - 1) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_solver_dropin_run_writes_vtk(gpu, tmp_path):
+    """run() to simulation_time = 6 dt with output_frequency 2 dt writes one
+    legacy VTK file per output time into output_directory, read back with
+    the package's VTK reader: n points, velocity, mass and J fields; the last
+    file holds the final particles."""
+    from physgrad_amd.state_io import read_vtk_points
+    exe = build_driver(tmp_path, "dropin_gpu_solver")
+    rng = np.random.default_rng(14)
+    nx, n, steps, dt = 16, 300, 6, 1e-4
+    x, v, m, types = _solver_scene(rng, n, [0, 2])
+    outdir = os.path.join(str(tmp_path), "vtk")
+    stdout, res = _drive_gpu_solver(exe, tmp_path, nx, x, v, m, types, steps, dt, use_run=1, outdir=outdir)
+    # the solver's clock and output test, in float32 as the header computes them
+    t, k, frames = np.float32(0), 0, []
+    dt32, freq, end = np.float32(dt), np.float32(2) * np.float32(dt), np.float32(steps) * np.float32(dt)
+    while t < end:
+        t = np.float32(t + dt32)
+        k += 1
+        if int(np.float32(t / freq)) > int(np.float32(np.float32(t - dt32) / freq)):
+            frames.append(k)
+    assert f"steps {k} " in stdout, stdout
+    assert sorted(os.listdir(outdir)) == [f"particles_{f:06d}.vtk" for f in frames]
+    for f in frames:
+        px, fields = read_vtk_points(os.path.join(outdir, f"particles_{f:06d}.vtk"))
+        assert px.shape == (n, 3) and np.isfinite(px).all()
+        assert set(fields) == {"velocity", "mass", "J"}
+        np.testing.assert_array_equal(np.sort(fields["mass"]), np.sort(m))
+    if frames[-1] == k:
+        px, fields = read_vtk_points(os.path.join(outdir, f"particles_{k:06d}.vtk"))
+        np.testing.assert_array_equal(px, res["x"])
+        np.testing.assert_array_equal(fields["velocity"], res["v"])
+        np.testing.assert_allclose(fields["J"], np.linalg.det(res["F"].astype(np.float64)), rtol=1e-5)
 
 
 @pytest.mark.gpu
