@@ -12,7 +12,8 @@
 // getCurrentStep (:107-328).
 //
 // The step runs on the GPU through the C-ABI (include/physgrad_mpm.h) as one
-// MLS-MPM context (tiled implementation, hipGraph replay of each step):
+// MLS-MPM context (tiled implementation, hipGraph replay of each step unless
+// enable_profiling asks for per-kernel times):
 // APIC transfers, F <- (I + dt C) F, per-particle materials as the reference's
 // fusedG2P2GKernel selects them (mpm_kernels.cu:407-421): FLUID particles the
 // Newtonian fluid (:131-149), every other type the neo-Hookean model
@@ -541,8 +542,9 @@ public:
         p.boundary_thickness = config_.boundary_nodes;
         p.max_particles = config_.max_particles;
         check(pg_mpm_create(&p, &ctx_));
-        check(pg_mpm_set_graph(ctx_, 1));
+        // per-kernel timing needs eager launches: a replayed graph records no events
         if (config_.enable_profiling) check(pg_mpm_set_profiling(ctx_, 1));
+        else check(pg_mpm_set_graph(ctx_, 1));
         upload();
         const size_t n = particles_.size(), G = grid_.getTotalNodes();
         metrics_.particle_memory = n * (24 * sizeof(T) + 2 * sizeof(uint32_t));

@@ -148,6 +148,8 @@ def test_gpu_solver_dropin_vs_oracle(gpu, oracle_lib, tmp_path, kind):
     print("gpu solver drop-in", kind, n, "particles", steps, "steps", flush=True)
     stdout, res = _drive_gpu_solver(exe, tmp_path, nx, x, v, m, types, steps, dt, jplane=jplane)
     assert "threw 1" in stdout and f"steps {steps}" in stdout, stdout
+    # enable_profiling: per-kernel event times reach getMetrics()
+    assert float(stdout.split("avg_step_ms")[1].split()[0]) > 0, stdout
     # the driver adds particles grouped by type
     order = np.argsort(types, kind="stable")
     x, v, m, types = x[order], v[order], m[order], types[order]
