@@ -31,6 +31,15 @@
  *           launch_mpm_timestep_forward/backward src/pytorch_autograd.cu:371-402
  *   pg_spatial_hash_build
  *       SpatialHashGrid<T,uint32_t>::buildHashTable src/sparse_data_structures.h:155-190
+ *   pg_hash_neighbors_count / pg_hash_neighbors_fill
+ *       SpatialHashGrid<T,uint32_t>::getNeighbors   src/sparse_data_structures.h:193-228
+ *   pg_coupling_create / pg_coupling_destroy
+ *       SPHMPMCouplingSystem<T>::SPHMPMCouplingSystem(const CouplingConfig<T>&)
+ *           src/sph_mpm_coupling.h:391-427
+ *   pg_coupling_find_pairs / pg_coupling_get_pairs
+ *       SPHMPMCouplingSystem<T>::findCouplingPairs    src/sph_mpm_coupling.h:454-486
+ *   pg_coupling_compute_forces
+ *       SPHMPMCouplingSystem<T>::computeCouplingForces src/sph_mpm_coupling.h:488-545
  */
 #ifndef PHYSGRAD_MPM_H
 #define PHYSGRAD_MPM_H
@@ -318,6 +327,80 @@ int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const f
 int pg_spatial_hash_build_noalloc(const float* pos, unsigned n, float cell_size, const float* domain_min,
                                   const unsigned* res, unsigned* cell_ids, unsigned* particle_indices,
                                   unsigned* cell_start, unsigned* cell_count, void* stream);
+
+/* ---------------------------------------------------------------- neighbour lists
+ * SpatialHashGrid::getNeighbors (src/sparse_data_structures.h:193-228) for nq
+ * query points at once, over a hash built by pg_spatial_hash_build with the
+ * same cell_size, domain_min and res: the (2R+1)^3 cells around each query's
+ * cell, R = ceil(radius / cell_size), x outermost and z innermost, cells whose
+ * coordinates reach res skipped, all particles of each visited hash slot in
+ * sorted order (no distance test and no de-duplication, as the reference).
+ * Result in CSR form: row_ptr [nq+1], neighbours [row_ptr[nq]].
+ * Cell coordinates saturate where the reference's conversion is undefined
+ * (a query below domain_min has coordinate 0 on that axis).
+ *   pg_hash_neighbors_count: row_ptr (device), *total = row_ptr[nq] (host;
+ *       synchronises the stream); workspace of
+ *       pg_hash_neighbors_workspace_size(nq) device bytes.  PG_ERR_INVALID if
+ *       the total reaches 2^32.
+ *   pg_hash_neighbors_fill: neighbours (device, row_ptr[nq] entries) from the
+ *       row_ptr of the count call. */
+size_t pg_hash_neighbors_workspace_size(unsigned nq);
+int pg_hash_neighbors_count(const float* queries, unsigned nq, float radius, float cell_size, const float* domain_min,
+                            const unsigned* res, const unsigned* cell_count, unsigned* row_ptr,
+                            unsigned long long* total, void* workspace, size_t workspace_bytes, void* stream);
+int pg_hash_neighbors_fill(const float* queries, unsigned nq, float radius, float cell_size, const float* domain_min,
+                           const unsigned* res, const unsigned* particle_indices, const unsigned* cell_start,
+                           const unsigned* cell_count, const unsigned* row_ptr, unsigned* neighbors, void* stream);
+
+/* ---------------------------------------------------------------- SPH-MPM coupling
+ * SPHMPMCouplingSystem (src/sph_mpm_coupling.h:391-579) on the GPU.  The
+ * reference tests every (SPH, MPM) pair on the host; here the MPM particles
+ * are binned with the spatial hash (cells of coupling_radius (1 + 2^-8) over
+ * [domain_min, domain_max], so the 27 cells around an SPH particle's cell hold
+ * every MPM particle within the radius, also for particles outside the domain,
+ * which the hash clamps into its border cells) and each SPH particle tests the
+ * candidates of its 27 cells.  Results are identical to the reference's:
+ *   pairs     all (s, m) with |x_m - x_s| < coupling_radius, in the
+ *             reference's order (SPH index major, MPM index ascending), in CSR
+ *             form: row_ptr [ns+1], col [npairs] (MPM index),
+ *             penetration [npairs] = coupling_radius - distance;
+ *   forces    per pair the reference's spring-damper normal force plus
+ *             friction, solid_force = -fluid_force [3 npairs]; with
+ *             enable_two_way_coupling each fluid force is added to the SPH
+ *             particle's force in pair order.  contact_area is
+ *             pi coupling_radius^2 for every pair.
+ * Positions, velocities and outputs are device arrays ([3n] xyz interleaved).
+ * The pairs of the last find_pairs stay in the context. */
+typedef struct pg_coupling_config {
+    float coupling_stiffness;    /* reference default 1e5 */
+    float coupling_damping;      /* 1e3 */
+    float friction_coefficient;  /* 0.3 */
+    float coupling_radius;       /* 0.15 */
+    int enable_two_way_coupling; /* 1 */
+    float domain_min[3];         /* binning box of the MPM particles */
+    float domain_max[3];
+} pg_coupling_config;
+typedef struct pg_coupling pg_coupling;
+/* Reference defaults (CouplingConfig, src/sph_mpm_coupling.h:391-411) and the unit box. */
+void pg_coupling_default_config(pg_coupling_config* cfg);
+/* PG_ERR_INVALID for a radius <= 0, an empty box or more than 2^28 binning cells. */
+int pg_coupling_create(const pg_coupling_config* cfg, pg_coupling** out);
+void pg_coupling_destroy(pg_coupling* c);
+const char* pg_coupling_last_error(const pg_coupling* c);
+/* findCouplingPairs: bins mpm_x, finds the pairs, *npairs (host) = their
+ * count; synchronises the stream.  PG_ERR_INVALID if the count reaches 2^32. */
+int pg_coupling_find_pairs(pg_coupling* c, const float* sph_x, unsigned ns, const float* mpm_x, unsigned nm,
+                           unsigned long long* npairs, void* stream);
+/* Copies the pairs of the last find_pairs into caller arrays (any may be
+ * NULL): row_ptr [ns+1], col, penetration [npairs]. */
+int pg_coupling_get_pairs(pg_coupling* c, unsigned* row_ptr, unsigned* col, float* penetration, void* stream);
+/* computeCouplingForces for the pairs of the last find_pairs, with the current
+ * positions and velocities (same particle counts).  solid_force, fluid_force
+ * [3 npairs] (may be NULL).  sph_force [3 ns] (in/out) receives the fluid
+ * forces when enable_two_way_coupling is set (may be NULL otherwise). */
+int pg_coupling_compute_forces(pg_coupling* c, const float* sph_x, const float* sph_v, const float* mpm_x,
+                               const float* mpm_v, float* sph_force, float* solid_force, float* fluid_force,
+                               void* stream);
 
 #ifdef __cplusplus
 }

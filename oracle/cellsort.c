@@ -3,7 +3,8 @@
  *
  * Restates SpatialHashGrid (src/sparse_data_structures.h):
  *   mortonEncode3D   :121-131  10-bit expansion by multiply-and-mask
- *   getGridCoords    :134-140  truncating float->uint32 conversion of (x-min)/cell
+ *   getGridCoords    :134-140  truncating float->uint32 conversion of (x-min)/cell,
+ *                              saturated (below domain_min: 0)
  *   getCellID        :143-152  clamp to res-1, morton % total_cells
  *   buildHashTable   :155-190  count, exclusive prefix sum, scatter in index order
  */
@@ -29,7 +30,9 @@ void orc_cell_ids(const float* pos, unsigned n, float cell_size, const float* dm
     for (unsigned i = 0; i < n; ++i) {
         unsigned c[3];
         for (int a = 0; a < 3; ++a) {
-            c[a] = (unsigned)((pos[3 * i + a] - dmin[a]) / cell_size);
+            const float t = (pos[3 * i + a] - dmin[a]) / cell_size;
+            /* float -> unsigned saturated where C leaves it undefined (t < 0, NaN, t >= 2^32) */
+            c[a] = !(t > 0.0f) ? 0u : (t < 4294967296.0f ? (unsigned)t : 0xFFFFFFFFu);
             if (c[a] > res[a] - 1) c[a] = res[a] - 1;
         }
         ids[i] = orc_morton3(c[0], c[1], c[2]) % total;

@@ -18,6 +18,8 @@
  *   3. cell keys + stable counting sort: a restatement of
  *      SpatialHashGrid::mortonEncode3D/getCellID/buildHashTable
  *      (src/sparse_data_structures.h:121-190).
+ *   4. neighbour lists of the hash (getNeighbors, :193-228) and the SPH-MPM
+ *      coupling pairs and forces (src/sph_mpm_coupling.h:454-545).
  */
 #ifndef PHYSGRAD_ORACLE_H
 #define PHYSGRAD_ORACLE_H
@@ -117,6 +119,26 @@ void orc_cell_ids(const float* pos, unsigned n, float cell_size, const float* do
 /* stable counting sort (count, exclusive scan, scatter in index order) */
 void orc_counting_sort(const unsigned* ids, unsigned n, unsigned total_cells,
                        unsigned* particle_indices, unsigned* cell_start, unsigned* cell_count);
+
+/* ---------------- neighbour lists and SPH-MPM coupling (coupling.c) ---------------- */
+/* SpatialHashGrid::getNeighbors for nq queries over a built hash: counts [nq]
+ * (may be NULL), out the lists back to back (NULL: count only).  Returns the
+ * total. */
+unsigned long long orc_hash_neighbors(const float* queries, unsigned nq, float radius, float cell_size,
+                                      const float* dmin, const unsigned* res, const unsigned* particle_indices,
+                                      const unsigned* cell_start, const unsigned* cell_count, unsigned* counts,
+                                      unsigned* out);
+/* findCouplingPairs: row_counts [ns] (may be NULL); col / penetration the
+ * pairs in SPH-major, MPM-ascending order (NULL: count only).  Returns the
+ * pair count. */
+unsigned long long orc_coupling_pairs(const float* sph_x, unsigned ns, const float* mpm_x, unsigned nm, float radius,
+                                      unsigned* row_counts, unsigned* col, float* penetration);
+/* computeCouplingForces over CSR pairs (row_ptr [ns+1]): solid_force,
+ * fluid_force [3 nnz]; two_way adds each fluid force onto sph_force [3 ns]. */
+void orc_coupling_forces(const float* sph_x, const float* sph_v, unsigned ns, const float* mpm_x,
+                         const float* mpm_v, const unsigned* row_ptr, const unsigned* col, const float* penetration,
+                         float stiffness, float damping, float friction, int two_way, float* solid_force,
+                         float* fluid_force, float* sph_force);
 
 #ifdef __cplusplus
 }

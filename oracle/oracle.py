@@ -88,6 +88,15 @@ def lib():
         L.orc_morton3.restype = ctypes.c_uint
         L.orc_cell_ids.argtypes = [_f32p, ctypes.c_uint, ctypes.c_float, _f32p, _u32p, _u32p]
         L.orc_counting_sort.argtypes = [_u32p, ctypes.c_uint, ctypes.c_uint, _u32p, _u32p, _u32p]
+        L.orc_hash_neighbors.argtypes = [_f32p, ctypes.c_uint, ctypes.c_float, ctypes.c_float, _f32p, _u32p, _u32p,
+                                         _u32p, _u32p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_hash_neighbors.restype = ctypes.c_ulonglong
+        L.orc_coupling_pairs.argtypes = [_f32p, ctypes.c_uint, _f32p, ctypes.c_uint, ctypes.c_float,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_coupling_pairs.restype = ctypes.c_ulonglong
+        L.orc_coupling_forces.argtypes = [_f32p, _f32p, ctypes.c_uint, _f32p, _f32p, _u32p, _u32p, _f32p,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                          _f32p, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -107,6 +116,13 @@ def ref():
                                    ctypes.POINTER(ctypes.c_double)])
         R.ref_spatial_hash.argtypes = [_f32p, ctypes.c_uint, ctypes.c_float, _f32p, _f32p,
                                        _u32p, _u32p, _u32p, _u32p, ctypes.POINTER(ctypes.c_uint)]
+        R.ref_hash_neighbors.argtypes = [_f32p, ctypes.c_uint, ctypes.c_float, _f32p, _f32p, _f32p, ctypes.c_uint,
+                                         ctypes.c_float, _u32p, _u32p, ctypes.c_ulonglong]
+        R.ref_hash_neighbors.restype = ctypes.c_longlong
+        R.ref_coupling.argtypes = [_f32p, _f32p, ctypes.c_uint, _f32p, _f32p, ctypes.c_uint, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, _u32p, _u32p,
+                                   _f32p, _f32p, _f32p, _f32p, ctypes.c_ulonglong]
+        R.ref_coupling.restype = ctypes.c_longlong
         _ref = R
     return _ref
 
@@ -313,3 +329,100 @@ def reference_spatial_hash(pos, cell_size, dmin, dmax):
     tc = ctypes.c_uint(0)
     R.ref_spatial_hash(pos, n, float(cell_size), dmin, dmax, ids, idx, st, ct, ctypes.byref(tc))
     return ids, idx, st, ct
+
+
+# ----------------------------------------------------------------------------- neighbours and coupling
+
+def _f3(a):
+    return np.ascontiguousarray(a, np.float32).reshape(-1)
+
+
+def hash_neighbors(queries, radius, cell_size, dmin, res, particle_indices, cell_start, cell_count):
+    """SpatialHashGrid::getNeighbors for every query over a built hash.
+    Returns (row_ptr [nq+1] uint32, neighbours uint32)."""
+    q = _f3(queries)
+    nq = q.size // 3
+    dmin = np.asarray(dmin, np.float32)
+    res = np.asarray(res, np.uint32)
+    pi, st, ct = (np.ascontiguousarray(a, np.uint32) for a in (particle_indices, cell_start, cell_count))
+    counts = np.zeros(nq, np.uint32)
+    total = lib().orc_hash_neighbors(q, nq, float(radius), float(cell_size), dmin, res, pi, st, ct,
+                                     counts.ctypes.data_as(ctypes.c_void_p), None)
+    out = np.zeros(max(int(total), 1), np.uint32)
+    lib().orc_hash_neighbors(q, nq, float(radius), float(cell_size), dmin, res, pi, st, ct, None,
+                             out.ctypes.data_as(ctypes.c_void_p))
+    row_ptr = np.zeros(nq + 1, np.uint32)
+    row_ptr[1:] = np.cumsum(counts, dtype=np.uint64).astype(np.uint32)
+    return row_ptr, out[:int(total)]
+
+
+def coupling_pairs(sph_x, mpm_x, radius):
+    """findCouplingPairs: (row_ptr [ns+1], col, penetration), SPH-major, MPM ascending."""
+    xs, xm = _f3(sph_x), _f3(mpm_x)
+    ns, nm = xs.size // 3, xm.size // 3
+    counts = np.zeros(ns, np.uint32)
+    k = lib().orc_coupling_pairs(xs, ns, xm, nm, float(radius), counts.ctypes.data_as(ctypes.c_void_p), None, None)
+    col = np.zeros(max(int(k), 1), np.uint32)
+    pen = np.zeros(max(int(k), 1), np.float32)
+    lib().orc_coupling_pairs(xs, ns, xm, nm, float(radius), None, col.ctypes.data_as(ctypes.c_void_p),
+                             pen.ctypes.data_as(ctypes.c_void_p))
+    row_ptr = np.zeros(ns + 1, np.uint32)
+    row_ptr[1:] = np.cumsum(counts, dtype=np.uint64).astype(np.uint32)
+    return row_ptr, col[:int(k)], pen[:int(k)]
+
+
+def coupling_forces(sph_x, sph_v, mpm_x, mpm_v, row_ptr, col, penetration, stiffness, damping, friction,
+                    two_way=True, sph_force=None):
+    """computeCouplingForces: (solid_force [nnz,3], fluid_force [nnz,3], sph_force [ns,3])."""
+    xs, vs, xm, vm = _f3(sph_x), _f3(sph_v), _f3(mpm_x), _f3(mpm_v)
+    ns = xs.size // 3
+    rp = np.ascontiguousarray(row_ptr, np.uint32)
+    col = np.ascontiguousarray(col, np.uint32)
+    pen = np.ascontiguousarray(penetration, np.float32)
+    nnz = int(rp[-1])
+    fs = np.zeros(3 * max(nnz, 1), np.float32)
+    ff = np.zeros(3 * max(nnz, 1), np.float32)
+    sf = np.zeros(3 * ns, np.float32) if sph_force is None else _f3(sph_force).copy()
+    lib().orc_coupling_forces(xs, vs, ns, xm, vm, rp, np.resize(col, max(nnz, 1)), np.resize(pen, max(nnz, 1)),
+                              float(stiffness), float(damping), float(friction), int(bool(two_way)), fs, ff, sf)
+    return fs[:3 * nnz].reshape(-1, 3), ff[:3 * nnz].reshape(-1, 3), sf.reshape(-1, 3)
+
+
+def reference_hash_neighbors(pos, cell_size, dmin, dmax, queries, radius, cap=1 << 24):
+    """The reference's compiled getNeighbors (oracle/_ref): (row_ptr, neighbours)."""
+    R = ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libref_mpm.so is not built")
+    pos, q = _f3(pos), _f3(queries)
+    nq = q.size // 3
+    counts = np.zeros(nq, np.uint32)
+    out = np.zeros(cap, np.uint32)
+    k = R.ref_hash_neighbors(pos, pos.size // 3, float(cell_size), np.asarray(dmin, np.float32),
+                             np.asarray(dmax, np.float32), q, nq, float(radius), counts, out, cap)
+    if k < 0:
+        raise RuntimeError("neighbour lists exceed the capacity")
+    row_ptr = np.zeros(nq + 1, np.uint32)
+    row_ptr[1:] = np.cumsum(counts, dtype=np.uint64).astype(np.uint32)
+    return row_ptr, out[:k].copy()
+
+
+def reference_coupling(sph_x, sph_v, mpm_x, mpm_v, radius, stiffness, damping, friction, two_way=True,
+                       cap=1 << 22):
+    """The reference's compiled SPHMPMCouplingSystem: dict(sph, mpm, penetration,
+    solid_force, fluid_force, sph_force)."""
+    R = ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libref_mpm.so is not built")
+    xs, vs, xm, vm = _f3(sph_x), _f3(sph_v), _f3(mpm_x), _f3(mpm_v)
+    ns, nm = xs.size // 3, xm.size // 3
+    ps, pm = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+    pen = np.zeros(cap, np.float32)
+    fs, ff = np.zeros(3 * cap, np.float32), np.zeros(3 * cap, np.float32)
+    sf = np.zeros(3 * max(ns, 1), np.float32)
+    k = R.ref_coupling(xs, vs, ns, xm, vm, nm, float(radius), float(stiffness), float(damping), float(friction),
+                       int(bool(two_way)), ps, pm, pen, fs, ff, sf, cap)
+    if k < 0:
+        raise RuntimeError("pairs exceed the capacity")
+    return dict(sph=ps[:k].copy(), mpm=pm[:k].copy(), penetration=pen[:k].copy(),
+                solid_force=fs[:3 * k].reshape(-1, 3).copy(), fluid_force=ff[:3 * k].reshape(-1, 3).copy(),
+                sph_force=sf[:3 * ns].reshape(-1, 3).copy())

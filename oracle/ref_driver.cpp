@@ -1,6 +1,7 @@
-// Driver that instantiates the reference's own CPU MPM solver and spatial hash
-// from their headers where they lie (src/material_point_method.h,
-// src/sparse_data_structures.h) and exposes them through extern "C" for the
+// Driver that instantiates the reference's own CPU MPM solver, spatial hash and
+// SPH-MPM coupling system from their headers where they lie
+// (src/material_point_method.h, src/sparse_data_structures.h,
+// src/sph_mpm_coupling.h) and exposes them through extern "C" for the
 // golden-vector script and the CPU baseline.  Built by oracle/Makefile into
 // oracle/_ref/ only.  TEST INFRASTRUCTURE ONLY.
 #include <cstring>
@@ -9,6 +10,7 @@
 #include <chrono>
 #include "material_point_method.h"
 #include "sparse_data_structures.h"
+#include "sph_mpm_coupling.h"
 
 using physgrad::ConceptVector3D;
 
@@ -84,6 +86,76 @@ int ref_spatial_hash(const float* pos, unsigned n, float cell_size, const float*
         for (unsigned k = 0; k < ct[c]; ++k) cell_ids[idx[st[c] + k]] = c;
     }
     return 0;
+}
+
+// Reference SpatialHashGrid::getNeighbors (sparse_data_structures.h:193-228)
+// for nq queries after buildHashTable; counts [nq], out (capacity cap) the
+// lists back to back.  Returns the total, or -1 when it exceeds cap.
+long long ref_hash_neighbors(const float* pos, unsigned n, float cell_size, const float* dmin, const float* dmax,
+                             const float* queries, unsigned nq, float radius, unsigned* counts, unsigned* out,
+                             unsigned long long cap) {
+    std::array<float, 3> lo{dmin[0], dmin[1], dmin[2]}, hi{dmax[0], dmax[1], dmax[2]};
+    physgrad::sparse::SpatialHashGrid<float, uint32_t> h(cell_size, lo, hi, n);
+    std::vector<float> p(pos, pos + 3 * (size_t)n);
+    h.buildHashTable(p, n);
+    unsigned long long k = 0;
+    for (unsigned q = 0; q < nq; ++q) {
+        auto nb = h.getNeighbors(queries[3 * q], queries[3 * q + 1], queries[3 * q + 2], radius);
+        counts[q] = (unsigned)nb.size();
+        for (auto i : nb) {
+            if (k >= cap) return -1;
+            out[k++] = i;
+        }
+    }
+    return (long long)k;
+}
+
+// Reference SPHMPMCouplingSystem<float> (sph_mpm_coupling.h:413-579):
+// findCouplingPairs then computeCouplingForces on the given SPH and MPM
+// particles.  Outputs per pair (capacity cap): sph id, mpm id, penetration,
+// solid and fluid force [3]; the SPH particles' accumulated force [3 ns].
+// Returns the pair count, or -1 when it exceeds cap.
+long long ref_coupling(const float* sph_x, const float* sph_v, unsigned ns, const float* mpm_x, const float* mpm_v,
+                       unsigned nm, float radius, float stiffness, float damping, float friction, int two_way,
+                       unsigned* pair_sph, unsigned* pair_mpm, float* penetration, float* solid_force,
+                       float* fluid_force, float* sph_force, unsigned long long cap) {
+    using namespace physgrad::coupling::sph_mpm;
+    CouplingConfig<float> cfg;
+    cfg.coupling_radius = radius;
+    cfg.coupling_stiffness = stiffness;
+    cfg.coupling_damping = damping;
+    cfg.friction_coefficient = friction;
+    cfg.enable_two_way_coupling = two_way != 0;
+    SPHMPMCouplingSystem<float> sys(cfg);
+    sys.initializeSPHSystem(ns);
+    sys.initializeMPMSystem(int3(4, 4, 4), ConceptVector3D<float>(1.f, 1.f, 1.f), ConceptVector3D<float>(0.f, 0.f, 0.f),
+                            nm);
+    auto& sph = sys.getSPHSystem().getParticles();
+    for (unsigned s = 0; s < ns; ++s) {
+        sph[s].position = ConceptVector3D<float>(sph_x[3 * s], sph_x[3 * s + 1], sph_x[3 * s + 2]);
+        sph[s].velocity = ConceptVector3D<float>(sph_v[3 * s], sph_v[3 * s + 1], sph_v[3 * s + 2]);
+    }
+    auto& mp = sys.getMPMParticles();
+    for (unsigned m = 0; m < nm; ++m) {
+        mp.setPosition(m, ConceptVector3D<float>(mpm_x[3 * m], mpm_x[3 * m + 1], mpm_x[3 * m + 2]));
+        mp.setVelocity(m, ConceptVector3D<float>(mpm_v[3 * m], mpm_v[3 * m + 1], mpm_v[3 * m + 2]));
+    }
+    sys.findCouplingPairs();
+    sys.computeCouplingForces();
+    const auto& F = sys.getCouplingForces();
+    if (F.size() > cap) return -1;
+    for (size_t k = 0; k < F.size(); ++k) {
+        pair_sph[k] = (unsigned)F[k].fluid_particle_id;
+        pair_mpm[k] = (unsigned)F[k].solid_particle_id;
+        penetration[k] = F[k].penetration_depth;
+        for (int a = 0; a < 3; ++a) {
+            solid_force[3 * k + a] = F[k].solid_force[a];
+            fluid_force[3 * k + a] = F[k].fluid_force[a];
+        }
+    }
+    for (unsigned s = 0; s < ns; ++s)
+        for (int a = 0; a < 3; ++a) sph_force[3 * s + a] = sph[s].force[a];
+    return (long long)F.size();
 }
 
 }  // extern "C"

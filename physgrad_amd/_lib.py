@@ -119,7 +119,28 @@ _SIGNATURES = {
                                              _vp, ctypes.c_size_t, _vp]),
     "pg_spatial_hash_build_noalloc": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, _vp, _vp, _vp, _vp, _vp,
                                                      _vp, _vp]),
+    "pg_hash_neighbors_workspace_size": (ctypes.c_size_t, [ctypes.c_uint]),
+    "pg_hash_neighbors_count": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp,
+                                               ctypes.POINTER(ctypes.c_ulonglong), _vp, ctypes.c_size_t, _vp]),
+    "pg_hash_neighbors_fill": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp,
+                                              _vp, _vp, _vp, _vp]),
+    "pg_coupling_default_config": (None, [_vp]),
+    "pg_coupling_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "pg_coupling_destroy": (None, [_vp]),
+    "pg_coupling_last_error": (ctypes.c_char_p, [_vp]),
+    "pg_coupling_find_pairs": (ctypes.c_int, [_vp, _vp, ctypes.c_uint, _vp, ctypes.c_uint,
+                                              ctypes.POINTER(ctypes.c_ulonglong), _vp]),
+    "pg_coupling_get_pairs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pg_coupling_compute_forces": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
+
+
+class PgCouplingConfig(ctypes.Structure):
+    """Mirror of struct pg_coupling_config."""
+    _fields_ = [("coupling_stiffness", ctypes.c_float), ("coupling_damping", ctypes.c_float),
+                ("friction_coefficient", ctypes.c_float), ("coupling_radius", ctypes.c_float),
+                ("enable_two_way_coupling", ctypes.c_int), ("domain_min", ctypes.c_float * 3),
+                ("domain_max", ctypes.c_float * 3)]
 
 SPATIAL_HASH_NOALLOC_MAX = 1 << 20
 

@@ -34,15 +34,16 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
     return v;
 }
 
-// getGridCoords + getCellID (src/sparse_data_structures.h:134-152)
+// getGridCoords + getCellID (src/sparse_data_structures.h:134-152); the
+// float -> unsigned conversion saturates (below domain_min: 0), see hash_coord
 __global__ void k_cell_ids(const float* __restrict__ pos, unsigned n, float cell, float m0, float m1, float m2,
                            unsigned r0, unsigned r1, unsigned r2, uint32_t* __restrict__ ids,
                            uint32_t* __restrict__ idx) {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    unsigned c0 = (unsigned)((pos[3 * (size_t)i] - m0) / cell);
-    unsigned c1 = (unsigned)((pos[3 * (size_t)i + 1] - m1) / cell);
-    unsigned c2 = (unsigned)((pos[3 * (size_t)i + 2] - m2) / cell);
+    unsigned c0 = hash_coord(pos[3 * (size_t)i], m0, cell);
+    unsigned c1 = hash_coord(pos[3 * (size_t)i + 1], m1, cell);
+    unsigned c2 = hash_coord(pos[3 * (size_t)i + 2], m2, cell);
     c0 = c0 < r0 - 1 ? c0 : r0 - 1;
     c1 = c1 < r1 - 1 ? c1 : r1 - 1;
     c2 = c2 < r2 - 1 ? c2 : r2 - 1;
@@ -228,6 +229,14 @@ int hash_build(const float* pos, unsigned n, float cell_size, const float* domai
 }
 
 }  // namespace
+
+int spatial_hash_build_ws(const float* pos, unsigned n, float cell_size, const float* domain_min, const unsigned* res,
+                          unsigned total, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
+                          unsigned* cell_count, void* workspace, hipStream_t st) {
+    return hash_build(pos, n, cell_size, domain_min, res, total, cell_ids, particle_indices, cell_start, cell_count,
+                      workspace, st);
+}
+
 }  // namespace pg
 
 extern "C" int pg_spatial_hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min,

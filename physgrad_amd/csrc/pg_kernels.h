@@ -32,6 +32,24 @@ struct MlsGeom {
 // message of the last failure on this host thread (pg_mpm_last_error(NULL))
 void set_thread_error(const char* msg);
 
+// Cell coordinate of the spatial hash: the truncating float -> unsigned
+// conversion of (x - lo) / cell (SpatialHashGrid::getGridCoords,
+// src/sparse_data_structures.h:134-140), saturated where C++ leaves it
+// undefined: values below 0 (points below domain_min, NaN) give 0, values of
+// 2^32 and above give 2^32 - 1.
+__host__ __device__ __forceinline__ uint32_t hash_coord(float x, float lo, float cell) {
+    const float t = (x - lo) / cell;
+    if (!(t > 0.f)) return 0u;
+    return t < 4294967296.f ? (uint32_t)t : 0xFFFFFFFFu;
+}
+
+// Spatial hash build (cell ids, stable radix sort, cell ranges) over a device
+// workspace of pg_spatial_hash_workspace_size(n) bytes; domain_min and res are
+// host arrays.  Used by the neighbour-list and coupling builds.
+int spatial_hash_build_ws(const float* pos, unsigned n, float cell_size, const float* domain_min, const unsigned* res,
+                          unsigned total, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
+                          unsigned* cell_count, void* workspace, hipStream_t st);
+
 // Per-particle material table of a context (device memory): recs[ids[u]] is
 // the material of the particle with caller index u.  recs == NULL: every
 // particle has its scene's material, Poisson ratio and volume.
