@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch leg")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-batch-fb", action="store_true", help="skip the 16-scene forward+backward leg")
+    ap.add_argument("--no-coupling", action="store_true", help="skip the SPH-MPM coupling leg")
+    ap.add_argument("--coupling-iters", type=int, default=20)
     return ap.parse_args()
 
 
@@ -324,6 +326,66 @@ def fp64_error_leg(scene, steps):
     return out
 
 
+def coupling_leg(iters, with_cpu):
+    """SPH-MPM coupling (pg_coupling_*): 1,048,576 SPH and 1,048,576 MPM
+    particles, uniform in the unit box, coupling radius 0.012 (about 7.6 MPM
+    partners per SPH particle).  One iteration = findCouplingPairs (MPM
+    binning, count, scan, fill with sorted rows; ends with the host read of
+    the pair count) + computeCouplingForces (forces and the two-way reaction).
+    The CPU baseline is the oracle's restatement of the reference's all-pairs
+    loop on a sample of SPH rows against all MPM particles."""
+    import numpy as np
+    import torch
+    from physgrad_amd.spatial import SPHMPMCoupling
+    rng = np.random.default_rng(11)
+    n, radius = 1 << 20, 0.012
+    xs = torch.as_tensor(rng.random((n, 3)).astype(np.float32)).cuda()
+    xm = torch.as_tensor(rng.random((n, 3)).astype(np.float32)).cuda()
+    vs = torch.as_tensor(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    vm = torch.as_tensor(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    f = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    c = SPHMPMCoupling(coupling_radius=radius)
+    for _ in range(3):
+        c.findCouplingPairs(xs, xm)
+        c.computeCouplingForces(xs, vs, xm, vm, f)
+    torch.cuda.synchronize()
+    t_find = t_force = 0.0
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        npairs = c.findCouplingPairs(xs, xm)          # synchronises (pair count to the host)
+        t1 = time.perf_counter()
+        c.computeCouplingForces(xs, vs, xm, vm, f)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_find += t1 - t0
+        t_force += t2 - t1
+    c.close()
+    t_it = (t_find + t_force) / iters
+    out = {"workload": "1,048,576 SPH x 1,048,576 MPM particles, uniform in [0,1]^3, radius 0.012",
+           "pairs": npairs, "iters": iters, "ms_find_pairs": 1e3 * t_find / iters,
+           "ms_forces": 1e3 * t_force / iters, "sph_rows_per_s": n / t_it, "pairs_per_s": npairs / t_it,
+           "unit": "SPH particles searched + forced per second"}
+    # minimum HBM bytes of one find+forces: positions in (SPH 12 B, MPM 12 B
+    # read by the binning + 16 B gathered), row counts and offsets (12 B per
+    # SPH row), pairs out (col + penetration 8 B), forces (pairs: col, pen
+    # in 8 B, positions/velocities gathered 24 B, 24 B out; rows: 24 B)
+    alg = n * (12 + 12 + 16 + 12 + 24) + npairs * (8 + 8 + 24 + 24)
+    out["effective_GBps"] = alg / t_it / 1e9
+    if with_cpu:
+        from oracle import oracle as O
+        sample = 1024
+        xs_h, xm_h = xs[:sample].cpu().numpy(), xm.cpu().numpy()
+        t0 = time.perf_counter()
+        O.coupling_pairs(xs_h, xm_h, radius)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sample / dt, "unit": "SPH particles searched per second", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{sample} SPH rows x {n} MPM particles with the oracle's restatement "
+                                         f"of findCouplingPairs (all-pairs loop, single thread), {dt:.2f} s"}
+        out["speedup_vs_cpu_rows"] = out["sph_rows_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def cpu_baseline(scene, steps):
     """FP32 port of the same step (oracle) on all host cores (OpenMP) for `steps`
     steps of the full workload, the same port on one core for steps // 2 steps,
@@ -435,6 +497,7 @@ def main():
         if not args.no_batch_fb:
             batch_fb = batch_fb_leg(args.bwd_ckpt)
     grad = None if args.no_bwd or args.no_grad else grad_leg()
+    coupling = None if args.no_coupling else coupling_leg(args.coupling_iters, not args.no_cpu)
     batch = None if args.no_batch else batch_leg(args.batch_steps)
 
     cpu, cpu_ref, fp64 = (None, None, None) if args.no_cpu else cpu_baseline(scene, args.cpu_steps) + \
@@ -466,6 +529,7 @@ def main():
         "batch_fwd_bwd": batch_fb,
         "grad": grad,
         "batch": batch,
+        "coupling": coupling,
         "fp64_error": fp64,
         "plain_port": plain,
         "speedup_vs_plain": (N * K / t_tiled) / plain["value"] if plain else None,

@@ -354,12 +354,13 @@ int pg_hash_neighbors_fill(const float* queries, unsigned nq, float radius, floa
 
 /* ---------------------------------------------------------------- SPH-MPM coupling
  * SPHMPMCouplingSystem (src/sph_mpm_coupling.h:391-579) on the GPU.  The
- * reference tests every (SPH, MPM) pair on the host; here the MPM particles
- * are binned with the spatial hash (cells of coupling_radius (1 + 2^-8) over
- * [domain_min, domain_max], so the 27 cells around an SPH particle's cell hold
- * every MPM particle within the radius, also for particles outside the domain,
- * which the hash clamps into its border cells) and each SPH particle tests the
- * candidates of its 27 cells.  Results are identical to the reference's:
+ * reference tests every (SPH, MPM) pair on the host; here both particle sets
+ * are sorted by cell with the spatial hash's radix sort (cells of
+ * coupling_radius (1 + 2^-8) over [domain_min, domain_max], linear cell ids,
+ * so the 27 cells around an SPH particle's cell hold every MPM particle within
+ * the radius, also for particles outside the domain, which the binning clamps
+ * into its border cells) and each SPH particle tests the candidates of its 27
+ * cells.  Results are identical to the reference's:
  *   pairs     all (s, m) with |x_m - x_s| < coupling_radius, in the
  *             reference's order (SPH index major, MPM index ascending), in CSR
  *             form: row_ptr [ns+1], col [npairs] (MPM index),

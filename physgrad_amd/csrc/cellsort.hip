@@ -17,6 +17,7 @@
 #include <string>
 #include "physgrad_mpm.h"
 #include "pg_kernels.h"
+#include "pg_scan.h"
 
 namespace pg {
 namespace {
@@ -35,9 +36,11 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
 }
 
 // getGridCoords + getCellID (src/sparse_data_structures.h:134-152); the
-// float -> unsigned conversion saturates (below domain_min: 0), see hash_coord
+// float -> unsigned conversion saturates (below domain_min: 0), see hash_coord.
+// linear: cell id (k r1 + j) r0 + i instead of the Morton code mod the cell
+// count (no two cells share an id; used by the coupling's binning).
 __global__ void k_cell_ids(const float* __restrict__ pos, unsigned n, float cell, float m0, float m1, float m2,
-                           unsigned r0, unsigned r1, unsigned r2, uint32_t* __restrict__ ids,
+                           unsigned r0, unsigned r1, unsigned r2, int linear, uint32_t* __restrict__ ids,
                            uint32_t* __restrict__ idx) {
     unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -47,8 +50,12 @@ __global__ void k_cell_ids(const float* __restrict__ pos, unsigned n, float cell
     c0 = c0 < r0 - 1 ? c0 : r0 - 1;
     c1 = c1 < r1 - 1 ? c1 : r1 - 1;
     c2 = c2 < r2 - 1 ? c2 : r2 - 1;
-    uint32_t code = expand_bits(c0) | (expand_bits(c1) << 1) | (expand_bits(c2) << 2);
-    ids[i] = code % (r0 * r1 * r2);
+    if (linear) {
+        ids[i] = (c2 * r1 + c1) * r0 + c0;
+    } else {
+        uint32_t code = expand_bits(c0) | (expand_bits(c1) << 1) | (expand_bits(c2) << 2);
+        ids[i] = code % (r0 * r1 * r2);
+    }
     idx[i] = i;
 }
 
@@ -64,30 +71,6 @@ __global__ void __launch_bounds__(RS_T) k_rs_hist(const uint32_t* __restrict__ k
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
-}
-
-// exclusive scan of `total` entries in place, one workgroup
-__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ a, unsigned total) {
-    __shared__ uint32_t s[1024];
-    const unsigned per = (total + 1023) / 1024;
-    const unsigned lo = threadIdx.x * per;
-    const unsigned hi = min(lo + per, total);
-    uint32_t sum = 0;
-    for (unsigned i = lo; i < hi; ++i) sum += a[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (unsigned i = lo; i < hi; ++i) {
-        uint32_t v = a[i];
-        a[i] = run;
-        run += v;
-    }
 }
 
 __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -160,7 +143,8 @@ inline size_t rs_blocks(unsigned n) { return (n + RS_TILE - 1) / RS_TILE; }
 extern "C" size_t pg_spatial_hash_workspace_size(unsigned n) {
     size_t nb = pg::rs_blocks(n ? n : 1);
     size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
-    return 3 * a + (((size_t)pg::RADIX * nb * 4 + 255) & ~(size_t)255);
+    return 3 * a + (((size_t)pg::RADIX * nb * 4 + 255) & ~(size_t)255) +
+           pg::scan::ws_bytes((unsigned)(pg::RADIX * nb));
 }
 
 namespace pg {
@@ -192,15 +176,18 @@ unsigned check_hash_args(const float* pos, unsigned n, float cell_size, const fl
 // launch of the same build, so the workspace needs no clearing.
 int hash_build(const float* pos, unsigned n, float cell_size, const float* domain_min, const unsigned* res,
                unsigned total, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
-               unsigned* cell_count, void* workspace, hipStream_t st) {
+               unsigned* cell_count, void* workspace, hipStream_t st, bool linear = false) {
     const size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
     uint32_t* kb[2] = {(uint32_t*)workspace, (uint32_t*)((char*)workspace + a)};
     uint32_t* v2 = (uint32_t*)((char*)workspace + 2 * a);
     uint32_t* hist = (uint32_t*)((char*)workspace + 3 * a);
+    const size_t nbh = rs_blocks(n ? n : 1);
+    uint64_t* sws = (uint64_t*)((char*)hist + ((((size_t)RADIX * nbh * 4) + 255) & ~(size_t)255));
     const uint32_t* sorted = cell_ids;
     if (n > 0) {
         k_cell_ids<<<(n + 255) / 256, 256, 0, st>>>(pos, n, cell_size, domain_min[0], domain_min[1], domain_min[2],
-                                                   res[0], res[1], res[2], cell_ids, particle_indices);
+                                                   res[0], res[1], res[2], linear ? 1 : 0, cell_ids,
+                                                   particle_indices);
         int bits = 0;
         while (bits < 32 && ((total - 1) >> bits)) ++bits;
         int passes = (bits + 7) / 8;
@@ -214,12 +201,12 @@ int hash_build(const float* pos, unsigned n, float cell_size, const float* domai
             uint32_t* vo = (p & 1) ? particle_indices : v2;
             const int shift = 8 * p;  // passes <= 4, digits above the key width are zero
             k_rs_hist<<<nb, RS_T, 0, st>>>(ki, n, shift, hist, nb);
-            k_rs_scan<<<1, 1024, 0, st>>>(hist, RADIX * nb);
+            scan::exclusive(hist, RADIX * nb, hist, false, sws, st);
             k_rs_scatter<<<nb, RS_T, 0, st>>>(ki, vi, ko, vo, n, shift, hist, nb);
             sorted = ko;
         }
     }
-    k_cell_ranges<<<(total + 255) / 256, 256, 0, st>>>(sorted, n, total, cell_start, cell_count);
+    if (cell_start) k_cell_ranges<<<(total + 255) / 256, 256, 0, st>>>(sorted, n, total, cell_start, cell_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         set_thread_error((std::string("spatial hash: ") + hipGetErrorString(e)).c_str());
@@ -232,9 +219,9 @@ int hash_build(const float* pos, unsigned n, float cell_size, const float* domai
 
 int spatial_hash_build_ws(const float* pos, unsigned n, float cell_size, const float* domain_min, const unsigned* res,
                           unsigned total, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
-                          unsigned* cell_count, void* workspace, hipStream_t st) {
+                          unsigned* cell_count, void* workspace, hipStream_t st, bool linear) {
     return hash_build(pos, n, cell_size, domain_min, res, total, cell_ids, particle_indices, cell_start, cell_count,
-                      workspace, st);
+                      workspace, st, linear);
 }
 
 }  // namespace pg
@@ -266,9 +253,11 @@ extern "C" int pg_spatial_hash_build(const float* pos, unsigned n, float cell_si
 namespace pg {
 namespace {
 
+constexpr size_t NOALLOC_BLOCKS = (PG_SPATIAL_HASH_NOALLOC_MAX + RS_TILE - 1) / RS_TILE;
 constexpr size_t NOALLOC_BYTES = 3 * (((size_t)PG_SPATIAL_HASH_NOALLOC_MAX * 4 + 255) & ~(size_t)255) +
-                                 (((size_t)RADIX * ((PG_SPATIAL_HASH_NOALLOC_MAX + RS_TILE - 1) / RS_TILE) * 4 +
-                                   255) & ~(size_t)255);
+                                 (((size_t)RADIX * NOALLOC_BLOCKS * 4 + 255) & ~(size_t)255) +
+                                 (((((size_t)RADIX * NOALLOC_BLOCKS + scan::TILE - 1) / scan::TILE + 1) * 8 + 255) &
+                                  ~(size_t)255);
 
 __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 

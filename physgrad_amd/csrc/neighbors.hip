@@ -9,23 +9,25 @@
 //   getNeighbors   count: one thread per query sums the counts of its
 //                  (2R+1)^3 cells; fill: one wave per query copies each
 //                  visited slot's particle range (coalesced).
-//   coupling       the MPM particles are binned with the spatial hash (cells
-//                  of radius (1 + 2^-8)), then gathered into cell order as
-//                  float4 (x, y, z, index bits).  One wave per SPH particle:
-//                  lanes 0..26 own the 27 neighbour cells (a slot that an
-//                  earlier lane already holds is dropped: distinct cells can
-//                  share a hash slot), a wave prefix sum flattens their
-//                  candidate ranges and the 64 lanes walk the flattened list,
-//                  so each load of 64 candidates is contiguous within a cell.
-//                  The fill sorts each row's MPM indices ascending (the
-//                  reference's order): rows of at most ROW_CAP pairs in the
-//                  wave's LDS (bitonic), longer rows afterwards by one
+//   coupling       both particle sets are binned with the spatial hash's
+//                  radix sort on linear cell ids (cells of radius (1 + 2^-8),
+//                  so distinct cells never share an id and the three cells of
+//                  an x-row are one contiguous range); the MPM particles are
+//                  gathered into cell order as float4 (x, y, z, index bits).
+//                  One thread per SPH particle, SPH particles taken in cell
+//                  order so that the threads of a wave read the same few
+//                  candidate ranges: 9 ranges cover the 27 neighbour cells.
+//                  The fill keeps rows of up to SMALL_ROW pairs sorted in
+//                  registers (insertion by rank) with their penetrations;
+//                  longer rows are written unsorted and sorted afterwards, by
+//                  one wave each in LDS (bitonic, up to ROW_CAP pairs) or one
 //                  workgroup each (LDS-sorted blocks, then merge passes in
 //                  global memory through the penetration array as the second
-//                  buffer).  Penetrations are computed from the sorted indices.
-//   forces         one thread per pair; the two-way reaction is summed per SPH
-//                  particle by one thread per row in pair order, so the float
-//                  additions happen in the reference's order.
+//                  buffer), penetrations computed from the sorted indices.
+//   forces         one thread per pair (coalesced writes); the two-way
+//                  reaction is summed per SPH particle by one thread per row
+//                  in pair order, so the float additions happen in the
+//                  reference's order.
 //
 // The distance, force and friction arithmetic is written in the reference's
 // operation order with contraction off, so results are bit-identical to it.
@@ -38,15 +40,15 @@
 #include "physgrad_mpm.h"
 #include "pg_kernels.h"
 #include "pg_tile.h"
+#include "pg_scan.h"
 
 namespace pg {
 namespace {
 
-constexpr int SCAN_T = 256;
-constexpr int SCAN_PER = 16;
-constexpr unsigned SCAN_TILE = SCAN_T * SCAN_PER;  // elements per scan workgroup
 constexpr int CPL_T = 256;                         // coupling workgroup (4 waves)
+constexpr unsigned SMALL_ROW = 16;                 // pairs per row sorted in registers
 constexpr unsigned ROW_CAP = 2048;                 // pairs per row sorted in a wave's LDS
+constexpr unsigned MID_WG = 1024;                  // workgroups of the in-LDS row sort
 constexpr unsigned LONG_BLOCK = 8192;              // block of a long row sorted in LDS
 constexpr unsigned LONG_WG = 256;                  // workgroups of the long-row sort
 constexpr uint64_t MAX_CPL_CELLS = 1ull << 28;
@@ -58,111 +60,6 @@ __device__ __forceinline__ uint32_t morton_slot(uint32_t x, uint32_t y, uint32_t
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float uniform_f(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
-}
-
-// inclusive prefix sum over the 64 lanes of a wave
-__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, 64);
-        if (lane >= o) v += u;
-    }
-    return v;
-}
-
-// ------------------------------------------------------------------ exclusive scan
-// out[i] = sum in[0..i), out[n] = total; total64 (device) = the total in 64 bits.
-
-__global__ void __launch_bounds__(SCAN_T) k_scan_tiles(const uint32_t* __restrict__ in, unsigned n,
-                                                       uint64_t* __restrict__ tsum) {
-    __shared__ uint64_t ws[SCAN_T / 64];
-    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
-    uint64_t s = 0;
-#pragma unroll
-    for (int r = 0; r < SCAN_PER; ++r) {
-        const size_t i = base + (size_t)r * SCAN_T + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < SCAN_T / 64; ++w) t += ws[w];
-        tsum[blockIdx.x] = t;
-    }
-}
-
-// exclusive scan of nt tile sums in place (one workgroup); tsum[nt] = total
-__global__ void __launch_bounds__(1024) k_scan_top(uint64_t* __restrict__ tsum, unsigned nt) {
-    __shared__ uint64_t ws[16];
-    __shared__ uint64_t carry_s;
-    if (threadIdx.x == 0) carry_s = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (unsigned b = 0; b < nt; b += 1024) {
-        const unsigned i = b + threadIdx.x;
-        const uint64_t v = i < nt ? tsum[i] : 0;
-        uint64_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint64_t u = __shfl_up(x, o, 64);
-            if (lane >= o) x += u;
-        }
-        if (lane == 63) ws[w] = x;
-        __syncthreads();
-        uint64_t before = carry_s;
-        for (int k = 0; k < w; ++k) before += ws[k];
-        if (i < nt) tsum[i] = before + x - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry_s = before + x;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) tsum[nt] = carry_s;
-}
-
-__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const uint32_t* __restrict__ in, unsigned n,
-                                                       const uint64_t* __restrict__ tsum, unsigned nt,
-                                                       uint32_t* __restrict__ out) {
-    __shared__ uint32_t ws[SCAN_T / 64];
-    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_PER;
-    uint32_t v[SCAN_PER];
-    uint32_t s = 0;
-#pragma unroll
-    for (int r = 0; r < SCAN_PER; ++r) {
-        v[r] = base + r < n ? in[base + r] : 0u;
-        s += v[r];
-    }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t incl = wave_inclusive(s);
-    if (lane == 63) ws[w] = incl;
-    __syncthreads();
-    uint32_t run = (uint32_t)tsum[blockIdx.x] + incl - s;
-    for (int k = 0; k < w; ++k) run += ws[k];
-#pragma unroll
-    for (int r = 0; r < SCAN_PER; ++r) {
-        if (base + r < n) out[base + r] = run;
-        run += v[r];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = (uint32_t)tsum[nt];
-}
-
-size_t scan_ws_bytes(unsigned n) {
-    const size_t nt = n ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
-    return ((nt + 1) * sizeof(uint64_t) + 255) & ~(size_t)255;
-}
-
-// row_ptr[0..n] from counts[0..n); the 64-bit total lands at ws[nt] for the host
-hipError_t exclusive_scan(const uint32_t* counts, unsigned n, uint32_t* row_ptr, uint64_t* ws, uint64_t** total,
-                          hipStream_t st) {
-    const unsigned nt = n ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
-    k_scan_tiles<<<nt, SCAN_T, 0, st>>>(counts, n, ws);
-    k_scan_top<<<1, 1024, 0, st>>>(ws, nt);
-    k_scan_apply<<<nt, SCAN_T, 0, st>>>(counts, n, ws, nt, row_ptr);
-    *total = ws + nt;
-    return hipGetLastError();
 }
 
 hipError_t read_total(const uint64_t* d, unsigned long long* h, hipStream_t st) {
@@ -255,7 +152,7 @@ __device__ __forceinline__ float pair_distance(float mx, float my, float mz, flo
     return sqrtf(rx * rx + ry * ry + rz * rz);
 }
 
-// MPM particles in hash order: (x, y, z, index bits)
+// MPM particles in cell order: (x, y, z, index bits)
 __global__ void k_cpl_gather(const float* __restrict__ x, const uint32_t* __restrict__ pidx, unsigned nm,
                              float4* __restrict__ spos) {
     const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,13 +161,13 @@ __global__ void k_cpl_gather(const float* __restrict__ x, const uint32_t* __rest
     spos[k] = make_float4(x[3 * (size_t)i], x[3 * (size_t)i + 1], x[3 * (size_t)i + 2], __uint_as_float(i));
 }
 
-// Walks the MPM candidates of SPH particle s (wave-uniform): for every chunk of
-// up to 64 candidates calls on_chunk(hit, mpm index), hit = distance < radius.
-template <class OnChunk>
-__device__ __forceinline__ void cpl_walk(const CplGeom& g, float sx, float sy, float sz,
-                                         const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ ccount,
-                                         const float4* __restrict__ spos, OnChunk on_chunk) {
-    const int lane = threadIdx.x & 63;
+// Visits the MPM particles within the radius of the point (sx, sy, sz):
+// on_hit(mpm index, distance) for each, candidates taken from the 3 x 3 rows
+// of 3 cells around the point's (clamped) cell.
+template <class OnHit>
+__device__ __forceinline__ void cpl_visit(const CplGeom& g, float sx, float sy, float sz,
+                                          const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ ccount,
+                                          const float4* __restrict__ spos, OnHit on_hit) {
     const float sp[3] = {sx, sy, sz};
     int c[3];
 #pragma unroll
@@ -278,117 +175,142 @@ __device__ __forceinline__ void cpl_walk(const CplGeom& g, float sx, float sy, f
         const uint32_t t = hash_coord(sp[a], g.lo[a], g.cell);
         c[a] = (int)(t < g.res[a] - 1 ? t : g.res[a] - 1);
     }
-    uint32_t slot = 0;
-    int valid = 0;
-    if (lane < 27) {
-        const int gx = c[0] + lane / 9 - 1, gy = c[1] + (lane / 3) % 3 - 1, gz = c[2] + lane % 3 - 1;
-        valid = gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)g.res[0] && gy < (int)g.res[1] && gz < (int)g.res[2];
-        if (valid) slot = morton_slot((uint32_t)gx, (uint32_t)gy, (uint32_t)gz, g.total);
-    }
-    // distinct cells can share a hash slot: keep its first lane only
-    for (int j = 0; j < 26; ++j) {
-        const uint32_t sj = __shfl(slot, j, 64);
-        const int vj = __shfl(valid, j, 64);
-        if (lane > j && vj && valid && sj == slot) valid = 0;
-    }
-    uint32_t cnt = 0, st = 0;
-    if (valid) {
-        cnt = ccount[slot];
-        st = cstart[slot];
-    }
-    const uint32_t incl = wave_inclusive(cnt);
-    const uint32_t pre = incl - cnt;
-    const uint32_t T = uniform(__shfl(incl, 63, 64));
-    for (uint32_t base = 0; base < T; base += 64) {
-        const uint32_t f = base + lane;
-        // the lane l of the largest exclusive prefix <= f owns candidate f
-        int l = 0;
-#pragma unroll
-        for (int step = 16; step > 0; step >>= 1) {
-            const int m = l + step;
-            const uint32_t pm = __shfl(pre, m < 27 ? m : 26, 64);
-            if (m < 27 && pm <= f) l = m;
+    const int i0 = c[0] > 0 ? c[0] - 1 : 0;
+    const int i1 = c[0] + 1 < (int)g.res[0] ? c[0] + 1 : c[0];
+    for (int z = c[2] - 1; z <= c[2] + 1; ++z) {
+        if (z < 0 || z >= (int)g.res[2]) continue;
+        for (int y = c[1] - 1; y <= c[1] + 1; ++y) {
+            if (y < 0 || y >= (int)g.res[1]) continue;
+            const uint32_t row = ((uint32_t)z * g.res[1] + (uint32_t)y) * g.res[0];
+            const uint32_t q0 = cstart[row + i0], q1 = cstart[row + i1] + ccount[row + i1];
+            for (uint32_t q = q0; q < q1; ++q) {
+                const float4 p = spos[q];
+                const float d = pair_distance(p.x, p.y, p.z, sx, sy, sz);
+                if (d < g.radius) on_hit(__float_as_uint(p.w), d);
+            }
         }
-        const uint32_t gst = __shfl(st, l, 64), gpre = __shfl(pre, l, 64);
-        bool hit = false;
-        uint32_t id = 0;
-        if (f < T) {
-            const float4 p = spos[gst + (f - gpre)];
-            id = __float_as_uint(p.w);
-            hit = pair_distance(p.x, p.y, p.z, sx, sy, sz) < g.radius;
-        }
-        on_chunk(hit, id);
     }
 }
 
-__global__ void __launch_bounds__(CPL_T) k_cpl_count(const float* __restrict__ sx, unsigned ns, CplGeom g,
-                                                     const uint32_t* __restrict__ cstart,
+// one thread per SPH particle, in cell order (sidx)
+__global__ void __launch_bounds__(CPL_T) k_cpl_count(const float* __restrict__ sx, const uint32_t* __restrict__ sidx,
+                                                     unsigned ns, CplGeom g, const uint32_t* __restrict__ cstart,
                                                      const uint32_t* __restrict__ ccount,
                                                      const float4* __restrict__ spos, uint32_t* __restrict__ counts) {
-    const unsigned s = uniform((blockIdx.x * (unsigned)CPL_T + threadIdx.x) >> 6);
-    if (s >= ns) return;
-    const float x = uniform_f(sx[3 * (size_t)s]), y = uniform_f(sx[3 * (size_t)s + 1]),
-                z = uniform_f(sx[3 * (size_t)s + 2]);
+    const unsigned t = blockIdx.x * CPL_T + threadIdx.x;
+    if (t >= ns) return;
+    const uint32_t s = sidx[t];
     uint32_t n = 0;
-    cpl_walk(g, x, y, z, cstart, ccount, spos, [&](bool hit, uint32_t) { n += __popcll(__ballot(hit)); });
-    if ((threadIdx.x & 63) == 0) counts[s] = n;
+    cpl_visit(g, sx[3 * (size_t)s], sx[3 * (size_t)s + 1], sx[3 * (size_t)s + 2], cstart, ccount, spos,
+              [&](uint32_t, float) { ++n; });
+    counts[s] = n;
 }
 
-__global__ void __launch_bounds__(CPL_T) k_cpl_fill(const float* __restrict__ sx, unsigned ns, const float* __restrict__ mx,
-                                                    CplGeom g, const uint32_t* __restrict__ cstart,
+__global__ void __launch_bounds__(CPL_T) k_cpl_fill(const float* __restrict__ sx, const uint32_t* __restrict__ sidx,
+                                                    unsigned ns, CplGeom g, const uint32_t* __restrict__ cstart,
                                                     const uint32_t* __restrict__ ccount,
                                                     const float4* __restrict__ spos,
                                                     const uint32_t* __restrict__ row_ptr, uint32_t* __restrict__ col,
-                                                    uint32_t* __restrict__ penbits, uint32_t* __restrict__ long_rows,
-                                                    uint32_t* __restrict__ nlong) {
-    __shared__ uint32_t buf[CPL_T / 64][ROW_CAP];
-    const unsigned s = uniform((blockIdx.x * (unsigned)CPL_T + threadIdx.x) >> 6);
-    const unsigned lane = threadIdx.x & 63;
-    if (s >= ns) return;
-    const uint32_t o = uniform(row_ptr[s]);
-    const uint32_t k = uniform(row_ptr[s + 1]) - o;
+                                                    uint32_t* __restrict__ penbits, uint32_t* __restrict__ prow,
+                                                    uint32_t* __restrict__ long_rows, uint32_t* __restrict__ nlong) {
+    const unsigned t = blockIdx.x * CPL_T + threadIdx.x;
+    if (t >= ns) return;
+    const uint32_t s = sidx[t];
+    const uint32_t o = row_ptr[s], k = row_ptr[s + 1] - o;
     if (k == 0) return;
-    const float x = uniform_f(sx[3 * (size_t)s]), y = uniform_f(sx[3 * (size_t)s + 1]),
-                z = uniform_f(sx[3 * (size_t)s + 2]);
-    const bool small = k <= ROW_CAP;
-    uint32_t* lb = buf[threadIdx.x >> 6];
-    uint32_t n = 0;
-    cpl_walk(g, x, y, z, cstart, ccount, spos, [&](bool hit, uint32_t id) {
-        const uint64_t m = __ballot(hit);
-        const uint32_t r = n + lanes_below(m);
-        // r < k: the count pass found exactly k hits on the same candidates
-        if (hit && r < k) {
-            if (small) lb[r] = id;
-            else col[o + r] = id;
+    const float x = sx[3 * (size_t)s], y = sx[3 * (size_t)s + 1], z = sx[3 * (size_t)s + 2];
+    if (k <= SMALL_ROW) {
+        // insertion by rank into a sorted register array (constant indices only)
+        uint32_t ids[SMALL_ROW];
+        float dist[SMALL_ROW];
+#pragma unroll
+        for (int i = 0; i < (int)SMALL_ROW; ++i) {
+            ids[i] = 0xFFFFFFFFu;
+            dist[i] = 0.f;
         }
-        n += __popcll(m);
-    });
-    if (!small) {
-        if (lane == 0) long_rows[atomicAdd(nlong, 1u)] = s;
-        return;
-    }
-    // bitonic sort of lb[0..P), P = 2^ceil(log2 k), padded with the largest key
-    uint32_t P = 1;
-    while (P < k) P <<= 1;
-    for (uint32_t i = k + lane; i < P; i += 64) lb[i] = 0xFFFFFFFFu;
-    lds_order();
-    for (uint32_t size = 2; size <= P; size <<= 1)
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t t = lane; t < (P >> 1); t += 64) {
-                const uint32_t i = 2 * t - (t & (stride - 1)), j = i + stride;
-                const uint32_t a = lb[i], b = lb[j];
-                if ((a > b) == ((i & size) == 0)) {
-                    lb[i] = b;
-                    lb[j] = a;
+        uint32_t n = 0;
+        cpl_visit(g, x, y, z, cstart, ccount, spos, [&](uint32_t id, float d) {
+            uint32_t pos = 0;
+#pragma unroll
+            for (int i = 0; i < (int)SMALL_ROW; ++i) pos += ids[i] < id ? 1u : 0u;
+#pragma unroll
+            for (int j = (int)SMALL_ROW - 1; j > 0; --j) {
+                if ((uint32_t)j > pos) {
+                    ids[j] = ids[j - 1];
+                    dist[j] = dist[j - 1];
+                } else if ((uint32_t)j == pos) {
+                    ids[j] = id;
+                    dist[j] = d;
                 }
             }
-            lds_order();
+            if (pos == 0) {
+                ids[0] = id;
+                dist[0] = d;
+            }
+            ++n;
+        });
+        // n == k: the count pass found the same hits
+#pragma unroll
+        for (int i = 0; i < (int)SMALL_ROW; ++i)
+            if ((uint32_t)i < k) {
+                col[o + i] = ids[i];
+                penbits[o + i] = __float_as_uint(g.radius - dist[i]);
+                prow[o + i] = s;
+            }
+        return;
+    }
+    uint32_t n = 0;
+    cpl_visit(g, x, y, z, cstart, ccount, spos, [&](uint32_t id, float) {
+        if (n < k) {
+            col[o + n] = id;
+            prow[o + n] = s;
         }
-    for (uint32_t i = lane; i < k; i += 64) {
-        const uint32_t id = lb[i];
-        col[o + i] = id;
-        const float d = pair_distance(mx[3 * (size_t)id], mx[3 * (size_t)id + 1], mx[3 * (size_t)id + 2], x, y, z);
-        penbits[o + i] = __float_as_uint(g.radius - d);
+        ++n;
+    });
+    long_rows[atomicAdd(nlong, 1u)] = s;
+}
+
+// Rows of SMALL_ROW < k <= ROW_CAP pairs: one wave each sorts the row in LDS
+// (bitonic) and writes the penetrations.
+__global__ void __launch_bounds__(CPL_T) k_cpl_sort_mid(const float* __restrict__ sx, const float* __restrict__ mx,
+                                                        float radius, const uint32_t* __restrict__ long_rows,
+                                                        const uint32_t* __restrict__ nlong,
+                                                        const uint32_t* __restrict__ row_ptr, uint32_t* col,
+                                                        uint32_t* penbits) {
+    __shared__ uint32_t buf[CPL_T / 64][ROW_CAP];
+    uint32_t* lb = buf[threadIdx.x >> 6];
+    const unsigned lane = threadIdx.x & 63;
+    const uint32_t nl = *nlong;
+    const unsigned nw = gridDim.x * (CPL_T / 64);
+    for (uint32_t e = uniform(blockIdx.x * (CPL_T / 64) + (threadIdx.x >> 6)); e < nl; e += nw) {
+        const uint32_t s = uniform(long_rows[e]);
+        const uint32_t o = uniform(row_ptr[s]);
+        const uint32_t k = uniform(row_ptr[s + 1]) - o;
+        if (k > ROW_CAP) continue;
+        uint32_t P = 1;
+        while (P < k) P <<= 1;
+        for (uint32_t i = lane; i < P; i += 64) lb[i] = i < k ? col[o + i] : 0xFFFFFFFFu;
+        lds_order();
+        for (uint32_t size = 2; size <= P; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = lane; t < (P >> 1); t += 64) {
+                    const uint32_t i = 2 * t - (t & (stride - 1)), j = i + stride;
+                    const uint32_t a = lb[i], b = lb[j];
+                    if ((a > b) == ((i & size) == 0)) {
+                        lb[i] = b;
+                        lb[j] = a;
+                    }
+                }
+                lds_order();
+            }
+        const float x = sx[3 * (size_t)s], y = sx[3 * (size_t)s + 1], z = sx[3 * (size_t)s + 2];
+        for (uint32_t i = lane; i < k; i += 64) {
+            const uint32_t id = lb[i];
+            col[o + i] = id;
+            const float d = pair_distance(mx[3 * (size_t)id], mx[3 * (size_t)id + 1], mx[3 * (size_t)id + 2], x, y, z);
+            penbits[o + i] = __float_as_uint(radius - d);
+        }
+        lds_order();
     }
 }
 
@@ -416,6 +338,7 @@ __global__ void __launch_bounds__(256) k_cpl_sort_long(const float* __restrict__
     for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
         const uint32_t s = long_rows[li];
         const uint32_t o = row_ptr[s], k = row_ptr[s + 1] - o;
+        if (k <= ROW_CAP) continue;   // sorted by k_cpl_sort_mid
         uint32_t* a = col + o;
         uint32_t* b = penbits + o;
         for (uint32_t blk = 0; blk < k; blk += LONG_BLOCK) {
@@ -478,27 +401,24 @@ struct ForceCfg {
     float stiffness, damping, friction;
 };
 
-__global__ void k_cpl_forces(const float* __restrict__ sx, const float* __restrict__ sv, unsigned ns,
+// One thread per pair (prow: its SPH particle): the reference's contact force.
+// skip[k] = 1 for a pair closer than 1e-6 (zero forces, no reaction), as
+// the reference skips it.
+__global__ void k_cpl_forces(const float* __restrict__ sx, const float* __restrict__ sv,
                              const float* __restrict__ mx, const float* __restrict__ mv,
-                             const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                             const uint32_t* __restrict__ prow, const uint32_t* __restrict__ col,
                              const float* __restrict__ pen, unsigned npairs, ForceCfg fc, float* __restrict__ solid,
-                             float* __restrict__ fluid) {
+                             float* __restrict__ fluid, uint8_t* __restrict__ skip) {
 #pragma clang fp contract(off)
     const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npairs) return;
-    // row of pair k: the last s with row_ptr[s] <= k
-    unsigned lo = 0, hi = ns;
-    while (hi - lo > 1) {
-        const unsigned mid = (lo + hi) >> 1;
-        if (row_ptr[mid] <= k) lo = mid;
-        else hi = mid;
-    }
-    const unsigned s = lo, m = col[k];
+    const uint32_t s = prow[k], m = col[k];
     float fs[3] = {0.f, 0.f, 0.f}, ff[3] = {0.f, 0.f, 0.f};
     const float r[3] = {mx[3 * (size_t)m] - sx[3 * (size_t)s], mx[3 * (size_t)m + 1] - sx[3 * (size_t)s + 1],
                         mx[3 * (size_t)m + 2] - sx[3 * (size_t)s + 2]};
     const float d = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    if (!(d < 1e-6f)) {
+    const bool sk = d < 1e-6f;
+    if (!sk) {
         const float inv = 1.0f / d;
         const float nrm[3] = {r[0] * inv, r[1] * inv, r[2] * inv};
         float vd[3];
@@ -531,23 +451,21 @@ __global__ void k_cpl_forces(const float* __restrict__ sx, const float* __restri
         solid[3 * (size_t)k + a] = fs[a];
         fluid[3 * (size_t)k + a] = ff[a];
     }
+    skip[k] = sk ? 1 : 0;
 }
 
-// two-way reaction: sph_force[s] += fluid force of each of its pairs, in pair
-// order, skipping the pairs the reference skips (distance < 1e-6)
-__global__ void k_cpl_accum(const float* __restrict__ sx, unsigned ns, const float* __restrict__ mx,
-                            const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                            const float* __restrict__ fluid, float* __restrict__ sph_force) {
+// Two-way reaction, one thread per SPH particle: its force plus the fluid
+// forces of its pairs, added in pair order (the reference's order).
+__global__ void k_cpl_accum(unsigned ns, const uint32_t* __restrict__ row_ptr, const float* __restrict__ fluid,
+                            const uint8_t* __restrict__ skip, float* __restrict__ sph_force) {
 #pragma clang fp contract(off)
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const uint32_t b = row_ptr[s], e = row_ptr[s + 1];
     if (b == e) return;
-    const float x = sx[3 * (size_t)s], y = sx[3 * (size_t)s + 1], z = sx[3 * (size_t)s + 2];
     float f0 = sph_force[3 * (size_t)s], f1 = sph_force[3 * (size_t)s + 1], f2 = sph_force[3 * (size_t)s + 2];
     for (uint32_t k = b; k < e; ++k) {
-        const uint32_t m = col[k];
-        if (pair_distance(mx[3 * (size_t)m], mx[3 * (size_t)m + 1], mx[3 * (size_t)m + 2], x, y, z) < 1e-6f) continue;
+        if (skip[k]) continue;
         f0 = f0 + fluid[3 * (size_t)k];
         f1 = f1 + fluid[3 * (size_t)k + 1];
         f2 = f2 + fluid[3 * (size_t)k + 2];
@@ -569,7 +487,7 @@ unsigned nblocks(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per
 // ====================================================================== C-ABI
 
 extern "C" size_t pg_hash_neighbors_workspace_size(unsigned nq) {
-    return (((size_t)nq * 4 + 255) & ~(size_t)255) + pg::scan_ws_bytes(nq);
+    return (((size_t)nq * 4 + 255) & ~(size_t)255) + pg::scan::ws_bytes(nq);
 }
 
 namespace {
@@ -620,7 +538,7 @@ extern "C" int pg_hash_neighbors_count(const float* queries, unsigned nq, float 
     uint64_t* sws = (uint64_t*)((char*)workspace + (((size_t)nq * 4 + 255) & ~(size_t)255));
     if (nq) pg::k_nbr_count<<<pg::nblocks(nq, 256), 256, 0, st>>>(queries, nq, g, cell_count, counts);
     uint64_t* dtot = nullptr;
-    hipError_t e = pg::exclusive_scan(counts, nq, row_ptr, sws, &dtot, st);
+    hipError_t e = pg::scan::exclusive(counts, nq, row_ptr, true, sws, st, &dtot);
     if (e == hipSuccess) e = pg::read_total(dtot, total, st);
     if (e != hipSuccess) {
         pg::set_thread_error((std::string("hash neighbours: ") + hipGetErrorString(e)).c_str());
@@ -661,18 +579,19 @@ struct pg_coupling {
     pg_coupling_config cfg;
     pg::CplGeom g;
     std::string err;
-    // binning of the MPM particles
+    // binning: MPM cell ranges (linear cell ids), sorted MPM and SPH orders
     uint32_t *cstart = nullptr, *ccount = nullptr;
-    uint32_t *cell_ids = nullptr, *pidx = nullptr;
+    uint32_t *cell_ids = nullptr, *pidx = nullptr, *sidx = nullptr;
     void* hws = nullptr;
     float4* spos = nullptr;
-    unsigned cap_m = 0;
+    unsigned cap_m = 0, cap_bin = 0;
     // rows
     uint32_t *counts = nullptr, *row_ptr = nullptr, *long_rows = nullptr, *nlong = nullptr;
     uint64_t* sws = nullptr;
     unsigned cap_s = 0;
-    // pairs and forces
-    uint32_t *col = nullptr, *pen = nullptr;
+    // pairs and scratch forces
+    uint32_t *col = nullptr, *pen = nullptr, *prow = nullptr;
+    uint8_t* skip = nullptr;
     float *solid = nullptr, *fluid = nullptr;
     size_t cap_p = 0;
     unsigned ns = 0, nm = 0;
@@ -762,8 +681,8 @@ extern "C" int pg_coupling_create(const pg_coupling_config* cfg, pg_coupling** o
 
 extern "C" void pg_coupling_destroy(pg_coupling* c) {
     if (!c) return;
-    void* ps[] = {c->cstart, c->ccount, c->cell_ids, c->pidx, c->hws, c->spos, c->counts, c->row_ptr,
-                  c->long_rows, c->nlong, c->sws, c->col, c->pen, c->solid, c->fluid};
+    void* ps[] = {c->cstart, c->ccount, c->cell_ids, c->pidx, c->sidx, c->hws, c->spos, c->counts, c->row_ptr,
+                  c->long_rows, c->nlong, c->sws, c->col, c->pen, c->prow, c->skip, c->solid, c->fluid};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete c;
@@ -778,45 +697,56 @@ extern "C" int pg_coupling_find_pairs(pg_coupling* c, const float* sph_x, unsign
     hipStream_t st = (hipStream_t)stream;
     c->have_pairs = false;
     int rc;
+    const unsigned nbin = std::max(nm, ns);
     if (nm > c->cap_m) {
-        if ((rc = grow(c, &c->cell_ids, nm)) || (rc = grow(c, &c->pidx, nm)) || (rc = grow(c, &c->spos, nm)) ||
-            (rc = grow(c, (unsigned char**)&c->hws, pg_spatial_hash_workspace_size(nm))))
-            return rc;
+        if ((rc = grow(c, &c->pidx, nm)) || (rc = grow(c, &c->spos, nm))) return rc;
         c->cap_m = nm;
+    }
+    if (nbin > c->cap_bin || !c->hws) {
+        if ((rc = grow(c, &c->cell_ids, nbin)) ||
+            (rc = grow(c, (unsigned char**)&c->hws, pg_spatial_hash_workspace_size(nbin))))
+            return rc;
+        c->cap_bin = nbin;
     }
     if (ns + 1 > c->cap_s || !c->row_ptr) {
         if ((rc = grow(c, &c->counts, ns + 1)) || (rc = grow(c, &c->row_ptr, ns + 1)) ||
-            (rc = grow(c, &c->long_rows, ns + 1)) ||
-            (rc = grow(c, (unsigned char**)&c->sws, pg::scan_ws_bytes(ns))))
+            (rc = grow(c, &c->long_rows, ns + 1)) || (rc = grow(c, &c->sidx, ns + 1)) ||
+            (rc = grow(c, (unsigned char**)&c->sws, pg::scan::ws_bytes(ns))))
             return rc;
         c->cap_s = ns + 1;
     }
     const unsigned res[3] = {c->g.res[0], c->g.res[1], c->g.res[2]};
+    // MPM particles: cell ranges and cell order; SPH particles: cell order
     rc = pg::spatial_hash_build_ws(mpm_x, nm, c->g.cell, c->g.lo, res, c->g.total, c->cell_ids, c->pidx, c->cstart,
-                                   c->ccount, c->hws, st);
+                                   c->ccount, c->hws, st, true);
+    if (rc == PG_OK)
+        rc = pg::spatial_hash_build_ws(sph_x, ns, c->g.cell, c->g.lo, res, c->g.total, c->cell_ids, c->sidx, nullptr,
+                                       nullptr, c->hws, st, true);
     if (rc != PG_OK) return cfail(c, rc, std::string("coupling: ") + pg_mpm_last_error(nullptr));
     if (nm) pg::k_cpl_gather<<<pg::nblocks(nm, 256), 256, 0, st>>>(mpm_x, c->pidx, nm, c->spos);
     if (ns)
-        pg::k_cpl_count<<<pg::nblocks((size_t)ns * 64, pg::CPL_T), pg::CPL_T, 0, st>>>(sph_x, ns, c->g, c->cstart,
-                                                                                     c->ccount, c->spos, c->counts);
+        pg::k_cpl_count<<<pg::nblocks(ns, pg::CPL_T), pg::CPL_T, 0, st>>>(sph_x, c->sidx, ns, c->g, c->cstart,
+                                                                        c->ccount, c->spos, c->counts);
     uint64_t* dtot = nullptr;
     unsigned long long tot = 0;
-    hipError_t e = pg::exclusive_scan(c->counts, ns, c->row_ptr, c->sws, &dtot, st);
+    hipError_t e = pg::scan::exclusive(c->counts, ns, c->row_ptr, true, c->sws, st, &dtot);
     if (e == hipSuccess) e = pg::read_total(dtot, &tot, st);
     if (e != hipSuccess) return cfail(c, PG_ERR_HIP, std::string("coupling: ") + hipGetErrorString(e));
     if (tot > 0xFFFFFFFFull) return cfail(c, PG_ERR_INVALID, "coupling: 2^32 or more pairs");
     if (tot > c->cap_p) {
         const size_t cap = std::max<size_t>(tot, c->cap_p + c->cap_p / 2);
-        if ((rc = grow(c, &c->col, cap)) || (rc = grow(c, &c->pen, cap)) || (rc = grow(c, &c->solid, 3 * cap)) ||
-            (rc = grow(c, &c->fluid, 3 * cap)))
+        if ((rc = grow(c, &c->col, cap)) || (rc = grow(c, &c->pen, cap)) || (rc = grow(c, &c->prow, cap)) ||
+            (rc = grow(c, &c->skip, cap)) || (rc = grow(c, &c->solid, 3 * cap)) || (rc = grow(c, &c->fluid, 3 * cap)))
             return rc;
         c->cap_p = cap;
     }
     if (tot) {
         pg::k_set_u32<<<1, 64, 0, st>>>(c->nlong, 0u);
-        pg::k_cpl_fill<<<pg::nblocks((size_t)ns * 64, pg::CPL_T), pg::CPL_T, 0, st>>>(
-            sph_x, ns, mpm_x, c->g, c->cstart, c->ccount, c->spos, c->row_ptr, c->col, c->pen, c->long_rows,
-            c->nlong);
+        pg::k_cpl_fill<<<pg::nblocks(ns, pg::CPL_T), pg::CPL_T, 0, st>>>(sph_x, c->sidx, ns, c->g, c->cstart,
+                                                                       c->ccount, c->spos, c->row_ptr, c->col, c->pen,
+                                                                       c->prow, c->long_rows, c->nlong);
+        pg::k_cpl_sort_mid<<<pg::MID_WG, pg::CPL_T, 0, st>>>(sph_x, mpm_x, c->g.radius, c->long_rows, c->nlong,
+                                                             c->row_ptr, c->col, c->pen);
         pg::k_cpl_sort_long<<<pg::LONG_WG, 256, 0, st>>>(sph_x, mpm_x, c->g.radius, c->long_rows, c->nlong,
                                                          c->row_ptr, c->col, c->pen);
         e = hipGetLastError();
@@ -858,12 +788,11 @@ extern "C" int pg_coupling_compute_forces(pg_coupling* c, const float* sph_x, co
     float* ff = fluid_force ? fluid_force : c->fluid;
     if (c->npairs) {
         const pg::ForceCfg fc{c->cfg.coupling_stiffness, c->cfg.coupling_damping, c->cfg.friction_coefficient};
-        pg::k_cpl_forces<<<pg::nblocks(c->npairs, 256), 256, 0, st>>>(sph_x, sph_v, c->ns, mpm_x, mpm_v, c->row_ptr,
-                                                                       c->col, (const float*)c->pen,
-                                                                       (unsigned)c->npairs, fc, fs, ff);
+        pg::k_cpl_forces<<<pg::nblocks(c->npairs, 256), 256, 0, st>>>(sph_x, sph_v, mpm_x, mpm_v, c->prow, c->col,
+                                                                       (const float*)c->pen, (unsigned)c->npairs, fc,
+                                                                       fs, ff, c->skip);
         if (c->cfg.enable_two_way_coupling)
-            pg::k_cpl_accum<<<pg::nblocks(c->ns, 256), 256, 0, st>>>(sph_x, c->ns, mpm_x, c->row_ptr, c->col, ff,
-                                                                     sph_force);
+            pg::k_cpl_accum<<<pg::nblocks(c->ns, 256), 256, 0, st>>>(c->ns, c->row_ptr, ff, c->skip, sph_force);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return cfail(c, PG_ERR_HIP, std::string("coupling: ") + hipGetErrorString(e));

@@ -45,10 +45,12 @@ __host__ __device__ __forceinline__ uint32_t hash_coord(float x, float lo, float
 
 // Spatial hash build (cell ids, stable radix sort, cell ranges) over a device
 // workspace of pg_spatial_hash_workspace_size(n) bytes; domain_min and res are
-// host arrays.  Used by the neighbour-list and coupling builds.
+// host arrays.  linear: cell id (k res1 + j) res0 + i instead of the
+// reference's Morton code mod the cell count.  cell_start NULL: no cell ranges
+// (only the sorted order).  Used by the coupling build.
 int spatial_hash_build_ws(const float* pos, unsigned n, float cell_size, const float* domain_min, const unsigned* res,
                           unsigned total, unsigned* cell_ids, unsigned* particle_indices, unsigned* cell_start,
-                          unsigned* cell_count, void* workspace, hipStream_t st);
+                          unsigned* cell_count, void* workspace, hipStream_t st, bool linear);
 
 // Per-particle material table of a context (device memory): recs[ids[u]] is
 // the material of the particle with caller index u.  recs == NULL: every
