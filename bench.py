@@ -231,7 +231,7 @@ def batch_leg(steps):
     b.close()
     return {"value": n * steps / t_graph, "unit": "particle-steps/s", "scenes": len(scenes), "particles": n,
             "steps": steps, "ms_per_step": 1e3 * t_graph / steps, "ms_per_step_eager": 1e3 * t_eager / steps,
-            "tile_workgroups": tuned, "state_finite": ok,
+            "tile_workgroups": tuned, "tuning": b.tuning(), "state_finite": ok,
             "kernel_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
             "workload": "batch_64: 64 scenes, side 32/48/64, 4,096..65,536 particles each (seed 4), "
                         "fixed-corotated / neo-Hookean, dt 1e-4, hipGraph replay of steps [10 + steps, 10 + 2 steps)"}
@@ -436,6 +436,10 @@ def main():
 
     ctx = MPMContext(scene.params(impl=_lib.IMPL_TILED))
     ctx.set_particles(scene.x, scene.v)
+    # launch shape (tile workgroups x particles per work item) picked by the
+    # autotuner on this particle set; the state is restored after the trials
+    ctx.autotune(trial_steps=10, use_cache=False)
+    tuning = ctx.tuning()
     t_tiled, wall = time_steps(ctx, K, W)
     grid = ctx.get_grid()
     active = int((grid["mass"] > 0).sum().item())
@@ -518,7 +522,8 @@ def main():
         "data": "synthetic (seeded two-box scene, generated in-process)",
         "config": {"workload": "two_boxes_128: 128^3 grid, dx 1/128, 1,048,576 particles in two boxes at +-2 m/s, "
                                "neo-Hookean E 5e4 nu 0.3 rho 1000, dt 5e-5, sticky walls 3 nodes",
-                   "particles": N, "grid": list(scene.dims), "impl": "tiled", "active_nodes": active},
+                   "particles": N, "grid": list(scene.dims), "impl": "tiled", "active_nodes": active,
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": "k_tile<3> (fused G2P+P2G)", "avg_launch_us": avg_kernel_s * 1e6,

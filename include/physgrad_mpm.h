@@ -254,15 +254,23 @@ int pg_mpm_num_scenes(const pg_mpm_ctx* ctx);
  * (src/mpm_solver.h:236-284). */
 int pg_mpm_set_graph(pg_mpm_ctx* ctx, int enable);
 
-/* Workgroups of the persistent tile kernels, and the autotuner that picks them
- * by timing trial_steps steps per candidate on the current particle set (the
- * state is restored afterwards).  Results are cached in a text file
+/* Launch shape of the tiled path: the workgroups of the persistent tile
+ * kernels, and the work-item size (a tile's particles are split into work
+ * items of at most that many particles, [640, 5120], default 1280: smaller
+ * spreads dense tiles over more workgroups, larger costs less per item).  The
+ * autotuner picks both by timing trial_steps steps per candidate
+ * (workgroups 256..8192 x work items 640, 1280, 2560) on the current particle
+ * set (the state is restored afterwards).  Results are cached in a text file
  * ($PG_AUTOTUNE_CACHE, else ~/.cache/physgrad_amd_autotune.txt) keyed by
  * tile slots, grid blocks, particle count bucket and scene count; use_cache
- * reads that file first.  The reference fixes its launch shapes at compile
- * time (KernelConfig, src/mpm_g2p2g_kernels.h:68-75). */
+ * reads that file first.  The tile edge (4 cells) and the tile workgroup (256
+ * threads) are compile-time; pg_mpm_get_tuning reports all four.  The
+ * reference fixes its launch shapes at compile time (KernelConfig,
+ * src/mpm_g2p2g_kernels.h:68-75). */
 int pg_mpm_get_launch(const pg_mpm_ctx* ctx, int* tile_workgroups);
 int pg_mpm_set_launch(pg_mpm_ctx* ctx, int tile_workgroups);
+int pg_mpm_set_work_item(pg_mpm_ctx* ctx, int particles);
+int pg_mpm_get_tuning(const pg_mpm_ctx* ctx, int* tile_workgroups, int* work_item, int* tile_edge, int* block_threads);
 int pg_mpm_autotune(pg_mpm_ctx* ctx, int trial_steps, int use_cache, void* stream);
 
 /* The reference's standalone transfer launchers, same arguments and meaning

@@ -14,6 +14,7 @@
 #include "physgrad_mpm.h"
 #include "pg_device.h"
 #include "pg_kernels.h"
+#include "pg_tile.h"
 
 namespace {
 
@@ -223,12 +224,14 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
     // faster than 2,048 in back-to-back runs on the 1M-particle scene and the
     // 64-scene batch, and a workgroup past the active list exits at once
     w.nblocks_launch = 4096;
-    // active-list entries: one per chunk of at most CHUNK particles of a non-empty
-    // tile.  A tile with c particles has ceil(c / CHUNK) = 1 + floor((c - 1) / CHUNK)
-    // entries, so a binning has at most min(tiles, particles) + particles / CHUNK.
+    w.chunk = pg::CHUNK;
+    // active-list entries: one per chunk of at most `chunk` particles of a
+    // non-empty tile.  A tile with c particles has ceil(c / chunk) =
+    // 1 + floor((c - 1) / chunk) entries, so a binning has at most
+    // min(tiles, particles) + particles / chunk; sized for the smallest chunk.
     uint64_t tiles = 0;
     for (int s = 0; s < ns; ++s) tiles += (uint64_t)S[s].ntx * S[s].nty * S[s].ntz;
-    const uint64_t max_entries = std::min<uint64_t>(tiles, c->cap) + c->cap / pg::CHUNK + 1;
+    const uint64_t max_entries = std::min<uint64_t>(tiles, c->cap) + c->cap / pg::MIN_CHUNK + 1;
     if (max_entries >= (1ull << 31)) return fail(c, PG_ERR_INVALID, "too many tiles and particles");
     w.max_entries = (int)max_entries;
     w.prof = &c->prof;
@@ -1196,26 +1199,32 @@ std::string tune_key(const pg_mpm_ctx* c) {
            ";n=" + std::to_string(nb) + ";scenes=" + std::to_string(c->nscenes);
 }
 
-// cached launch size for this workload shape, or 0
-int tune_lookup(const pg_mpm_ctx* c) {
+// cached (launch size, work-item size) for this workload shape; false if none
+// (a line without the work-item size keeps the default)
+bool tune_lookup(const pg_mpm_ctx* c, int* wg, int* chunk) {
     FILE* f = std::fopen(tune_cache_path().c_str(), "r");
-    if (!f) return 0;
+    if (!f) return false;
     const std::string key = tune_key(c);
     char line[512];
-    int val = 0;
+    bool found = false;
     while (std::fgets(line, sizeof(line), f)) {
         char k[400];
-        int v = 0;
-        if (std::sscanf(line, "%399s %d", k, &v) == 2 && key == k && v > 0) val = v;
+        int v = 0, ch = pg::CHUNK;
+        const int got = std::sscanf(line, "%399s %d %d", k, &v, &ch);
+        if (got >= 2 && key == k && v >= 8 && v <= (1 << 20) && ch >= pg::MIN_CHUNK && ch <= pg::MAX_CHUNK) {
+            *wg = v;
+            *chunk = got == 3 ? ch : pg::CHUNK;
+            found = true;
+        }
     }
     std::fclose(f);
-    return val;
+    return found;
 }
 
-void tune_store(const pg_mpm_ctx* c, int val) {
+void tune_store(const pg_mpm_ctx* c, int wg, int chunk) {
     FILE* f = std::fopen(tune_cache_path().c_str(), "a");
     if (!f) return;
-    std::fprintf(f, "%s %d\n", tune_key(c).c_str(), val);
+    std::fprintf(f, "%s %d %d\n", tune_key(c).c_str(), wg, chunk);
     std::fclose(f);
 }
 
@@ -1294,6 +1303,32 @@ int pg_mpm_set_launch(pg_mpm_ctx* c, int tile_workgroups) {
     return PG_OK;
 }
 
+int pg_mpm_set_work_item(pg_mpm_ctx* c, int particles) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
+        return fail(c, PG_ERR_INVALID, "the work-item size applies to the tiled MLS-MPM path");
+    if (particles < pg::MIN_CHUNK || particles > pg::MAX_CHUNK)
+        return fail(c, PG_ERR_INVALID, "particles per work item must be in [" + std::to_string(pg::MIN_CHUNK) + ", " +
+                                           std::to_string(pg::MAX_CHUNK) + "]");
+    if (particles == c->tw.chunk) return PG_OK;
+    // both scan passes of a binning run in the same call, so the new size takes
+    // effect at the next binning; captured graphs hold the old one
+    drop_graphs(c);
+    c->tw.chunk = particles;
+    c->version++;
+    return PG_OK;
+}
+
+int pg_mpm_get_tuning(const pg_mpm_ctx* c, int* tile_workgroups, int* work_item, int* tile_edge,
+                      int* block_threads) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (tile_workgroups) *tile_workgroups = c->tw.nblocks_launch;
+    if (work_item) *work_item = c->tw.chunk;
+    if (tile_edge) *tile_edge = pg::TT;
+    if (block_threads) *block_threads = pg::TB;
+    return PG_OK;
+}
+
 int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream) {
     if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
     if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
@@ -1301,8 +1336,11 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
     if (trial_steps < 1) return fail(c, PG_ERR_INVALID, "trial_steps must be >= 1");
     if (c->n == 0) return PG_OK;
     if (use_cache) {
-        const int v = tune_lookup(c);
-        if (v > 0) return pg_mpm_set_launch(c, v);
+        int wg = 0, ch = 0;
+        if (tune_lookup(c, &wg, &ch)) {
+            const int rc = pg_mpm_set_launch(c, wg);
+            return rc != PG_OK ? rc : pg_mpm_set_work_item(c, ch);
+        }
     }
     hipStream_t st = (hipStream_t)stream;
     drop_graphs(c);
@@ -1350,41 +1388,48 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
     hipEvent_t a = nullptr, b = nullptr;
     hipEventCreate(&a);
     hipEventCreate(&b);
-    const int cand[] = {256, 512, 1024, 2048, 4096, 8192};
-    int best = c->tw.nblocks_launch;
+    const int cand_wg[] = {256, 512, 1024, 2048, 4096, 8192};
+    const int cand_chunk[] = {640, 1280, 2560};
+    int best = c->tw.nblocks_launch, best_chunk = c->tw.chunk;
     float best_ms = 1e30f;
     hipError_t e = hipSuccess;
-    for (int v : cand) {
-        c->tw.nblocks_launch = v;
-        float ms_sum = 0.f;
-        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-            save(true);
-            float* E = c->have_E ? c->E : nullptr;
-            float* E2 = c->have_E ? c->E2 : nullptr;
-            float* m = c->have_mass ? c->mass : nullptr;
-            float* m2 = c->have_mass ? c->mass2 : nullptr;
-            hipEventRecord(a, st);
-            e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
-                                trial_steps, st);
-            hipEventRecord(b, st);
-            hipEventSynchronize(b);
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, a, b);
-            if (rep == 1) ms_sum += ms;  // the first repetition warms up
+    for (int ch : cand_chunk) {
+        for (int v : cand_wg) {
+            c->tw.nblocks_launch = v;
+            c->tw.chunk = ch;
+            float ms_sum = 0.f;
+            for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+                save(true);
+                float* E = c->have_E ? c->E : nullptr;
+                float* E2 = c->have_E ? c->E2 : nullptr;
+                float* m = c->have_mass ? c->mass : nullptr;
+                float* m2 = c->have_mass ? c->mass2 : nullptr;
+                hipEventRecord(a, st);
+                e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
+                                    trial_steps, st);
+                hipEventRecord(b, st);
+                hipEventSynchronize(b);
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, a, b);
+                if (rep == 1) ms_sum += ms;  // the first repetition warms up
+            }
+            if (e != hipSuccess) break;
+            if (ms_sum < best_ms) {
+                best_ms = ms_sum;
+                best = v;
+                best_chunk = ch;
+            }
         }
         if (e != hipSuccess) break;
-        if (ms_sum < best_ms) {
-            best_ms = ms_sum;
-            best = v;
-        }
     }
     c->tw.nblocks_launch = best;
+    c->tw.chunk = best_chunk;
     save(true);
     hipEventDestroy(a);
     hipEventDestroy(b);
     release();
     if (e != hipSuccess) return hip_fail(c, e, "autotune");
-    tune_store(c, best);
+    tune_store(c, best, best_chunk);
     return PG_OK;
 }
 

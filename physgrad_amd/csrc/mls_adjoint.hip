@@ -87,8 +87,7 @@ PG_DEV void weight_grads(const Stencil& st, float dw[3][3]) {
     }
 }
 
-constexpr int APASS = 1280;   // particles per LDS pass of a work item
-static_assert(CHUNK % APASS == 0, "a work item is a whole number of passes");
+constexpr int APASS = 1280;   // particles per LDS pass of a work item (larger work items take several)
 
 // The reverse of one particle's G2P and state update, given its stencil, F and
 // the adjoints of its new state (ax, aF, av, aC):

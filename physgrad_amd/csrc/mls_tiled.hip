@@ -10,7 +10,7 @@
 // perm = sorted position -> storage slot.
 //
 // Transfers.  One workgroup processes one work item at a time: a chunk of at
-// most CHUNK particles of one tile (dense tiles have several).  Per pass of up
+// most `chunk` particles of one tile (dense tiles have several).  Per pass of up
 // to PASS particles, x and the storage slots are loaded in bulk and ordered by
 // stencil-base cell with an LDS counting sort (64 cells) and dealt over
 // groups of up to 64 (one per lane) that hold at most two particles of a cell.
@@ -44,8 +44,8 @@
 namespace pg {
 namespace {
 
-constexpr int PASS = 1280;       // particles staged per LDS pass of a work item
-static_assert(CHUNK % PASS == 0, "a work item is a whole number of passes");
+constexpr int PASS = 1280;       // particles staged per LDS pass of a work item (a larger work item takes
+                                 // several passes)
 constexpr uint16_t FAR = 0xFFFF; // direction code of a particle that jumped more than one tile
 constexpr int QCAP = 48;         // P2G values of cell changers kept in LDS per pass (more are reloaded)
 
@@ -822,7 +822,7 @@ __global__ void k_init_keys(int n, const float4* __restrict__ S, const uint32_t*
 // Scan of the tile counts in two kernels over blocks of SCAN_B slots:
 // block sums and active-list entry counts, then per-block exclusive scan with
 // the preceding blocks' totals; writes starts, the active list (one entry per
-// chunk of at most CHUNK particles of a tile), the tile -> entries map, and
+// chunk of at most `chunk` particles of a tile), the tile -> entries map, and
 // clears the counts.
 constexpr int SCAN_T = 256;
 constexpr int SCAN_B = 1024;
@@ -843,7 +843,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, ui
     return incl - v;
 }
 
-__device__ __forceinline__ void scan_a_block(int blk, const uint32_t* __restrict__ count, int slots,
+__device__ __forceinline__ void scan_a_block(int blk, const uint32_t* __restrict__ count, int slots, uint32_t chunk,
                                              uint32_t* __restrict__ bsum, uint32_t* sh) {
     const int base = blk * SCAN_B + threadIdx.x * 4;
     uint32_t s = 0, act = 0;
@@ -851,7 +851,7 @@ __device__ __forceinline__ void scan_a_block(int blk, const uint32_t* __restrict
     for (int k = 0; k < 4; ++k) {
         const uint32_t v = base + k < slots ? count[base + k] : 0u;
         s += v;
-        act += (v + CHUNK - 1) / CHUNK;
+        act += (v + chunk - 1) / chunk;
     }
     uint32_t ts, ta;
     block_excl_scan(s, sh, &ts);
@@ -862,10 +862,10 @@ __device__ __forceinline__ void scan_a_block(int blk, const uint32_t* __restrict
     }
 }
 
-__global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ count, int slots,
+__global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ count, int slots, uint32_t chunk,
                                                    uint32_t* __restrict__ bsum) {
     __shared__ uint32_t sh[SCAN_T];
-    scan_a_block(blockIdx.x, count, slots, bsum, sh);
+    scan_a_block(blockIdx.x, count, slots, chunk, bsum, sh);
 }
 
 // The grid update of step n and the first scan pass over the tile counts of
@@ -881,7 +881,7 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
                                                         const float4* __restrict__ wbuf,
                                                         float4* __restrict__ mp_all, float4* __restrict__ gv_all,
                                                         uint32_t* __restrict__ flags8, int ngrid,
-                                                        const uint32_t* __restrict__ count,
+                                                        const uint32_t* __restrict__ count, uint32_t chunk,
                                                         uint32_t* __restrict__ bsum,
                                                         const uint32_t* __restrict__ node_bc) {
     static_assert(SCAN_T == 256, "both parts run 256 threads");
@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
         grid_list<TAPE>((uint32_t)ngrid, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8,
                         node_bc);
     else
-        scan_a_block(blockIdx.x - ngrid, count, slots, bsum, sh);
+        scan_a_block(blockIdx.x - ngrid, count, slots, chunk, bsum, sh);
 }
 
 __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count, int slots,
@@ -898,7 +898,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
                                                    uint4* __restrict__ alist, uint32_t* __restrict__ nactive,
                                                    uint2* __restrict__ tslot, const uint32_t* __restrict__ tscene,
                                                    uint32_t* __restrict__ nblist, uint32_t max_entries,
-                                                   uint32_t* __restrict__ err) {
+                                                   uint32_t chunk, uint32_t* __restrict__ err) {
     __shared__ uint32_t sh[SCAN_T];
     // the grid update of the last binning has read the block list: empty it
     if (blockIdx.x == 0 && threadIdx.x == 0) *nblist = 0u;
@@ -917,7 +917,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
     for (int k = 0; k < 4; ++k) {
         v[k] = base + k < slots ? count[base + k] : 0u;
         s += v[k];
-        act += (v[k] + CHUNK - 1) / CHUNK;
+        act += (v[k] + chunk - 1) / chunk;
     }
     uint32_t ts, ta;
     uint32_t run = block_excl_scan(s, sh, &ts) + pre_s;
@@ -926,14 +926,14 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
     for (int k = 0; k < 4; ++k) {
         if (base + k < slots) {
             start[base + k] = run;
-            const uint32_t ne = (v[k] + CHUNK - 1) / CHUNK;
+            const uint32_t ne = (v[k] + chunk - 1) / chunk;
             tslot[base + k] = make_uint2(v[k] ? pos : 0xFFFFFFFFu, ne);
             if (v[k]) {
                 const uint32_t sc = tscene[base + k];
                 for (uint32_t c = 0; c < ne; ++c, ++pos) {
                     if (pos < max_entries)
-                        alist[pos] = make_uint4((uint32_t)(base + k), run + c * CHUNK,
-                                                min((uint32_t)CHUNK, v[k] - c * CHUNK), (uint32_t)sc);
+                        alist[pos] = make_uint4((uint32_t)(base + k), run + c * chunk,
+                                                min(chunk, v[k] - c * chunk), (uint32_t)sc);
                     else
                         atomicOr(err, ERR_ALIST);
                 }
@@ -967,7 +967,7 @@ hipError_t bin_rest(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
                                     nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist,
-                                    (uint32_t)w.max_entries, w.err);
+                                    (uint32_t)w.max_entries, (uint32_t)w.chunk, w.err);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm, w.tile_slots, w.err);
     return hipGetLastError();
 }
@@ -976,10 +976,10 @@ hipError_t bin_into(TiledWork& w, int n, uint32_t* perm, uint32_t* alist, uint32
                     hipStream_t st) {
     ProfScope ps(w.prof, PK_BIN, st);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
-    k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum);
+    k_scan_a<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, (uint32_t)w.chunk, w.bsum);
     k_scan_b<<<nb, SCAN_T, 0, st>>>(w.count, w.tile_slots, w.bsum, w.start, reinterpret_cast<uint4*>(alist),
                                     nactive, reinterpret_cast<uint2*>(tslot), w.tscene, w.nblist,
-                                    (uint32_t)w.max_entries, w.err);
+                                    (uint32_t)w.max_entries, (uint32_t)w.chunk, w.err);
     if (n > 0) k_scatter<<<(n + 255) / 256, 256, 0, st>>>(n, w.key, w.rank, w.start, perm, w.tile_slots, w.err);
     return hipGetLastError();
 }
@@ -1028,8 +1028,8 @@ void grid_update_and_count(const TiledWork& w, const uint32_t* tslot, float4* mp
     const int ngrid = grid_wgs(w);
     const int nb = (w.tile_slots + SCAN_B - 1) / SCAN_B;
     k_grid_and_count<TAPE><<<ngrid + nb, 256, 0, st>>>(w.scenes, w.tscene, w.tile_slots, tslot, w.bflag, w.blist,
-                                                        w.nblist, w.wbuf, mp, gv, flags8, ngrid, w.count, w.bsum,
-                                                        w.node_bc);
+                                                        w.nblist, w.wbuf, mp, gv, flags8, ngrid, w.count,
+                                                        (uint32_t)w.chunk, w.bsum, w.node_bc);
 }
 
 }  // namespace

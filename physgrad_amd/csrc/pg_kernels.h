@@ -149,9 +149,12 @@ struct SceneInfo {
 };
 
 // Work items of the tile kernels: the particles of a tile in chunks of at
-// most CHUNK, so that a dense tile is spread over several workgroups instead
-// of serialising the step on one.
+// most `chunk` particles (TiledWork::chunk, default CHUNK, a runtime knob of
+// the autotuner in [MIN_CHUNK, MAX_CHUNK]), so that a dense tile is spread over
+// several workgroups instead of serialising the step on one.
 constexpr int CHUNK = 1280;
+constexpr int MIN_CHUNK = 640;
+constexpr int MAX_CHUNK = 5120;
 
 // Largest grid side of the tiled path (64 tiles of 4 cells per axis).  One
 // step on a 264^3 grid (66 tiles per axis, 2^21 tile slots) raised an illegal
@@ -183,8 +186,9 @@ struct TiledWork {
     uint32_t* blist;       // [tile_slots] the flagged blocks, in the order they were first flagged
     uint32_t* nblist;      // [1] their count; emptied after every grid update
     float4* wbuf;          // [max_entries][512] P2G window sums by active-list entry
-    int max_entries;       // capacity of the active list
+    int max_entries;       // capacity of the active list (sized for MIN_CHUNK)
     int nblocks_launch;    // persistent grid size of the tile kernels
+    int chunk;             // particles per work item (CHUNK by default)
     uint32_t* err;         // [1] binning error word: ERR_* bits set by guarded writes, never cleared by a step
     MatTable mt;           // per-particle materials (recs NULL: the scenes')
     const uint32_t* node_bc;  // [total grid nodes] boundary flags per node, or NULL

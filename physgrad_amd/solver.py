@@ -130,8 +130,27 @@ class MPMContext:
     def launch(self, workgroups: int):
         check(self._L.pg_mpm_set_launch(self._h, int(workgroups)), self._h)
 
+    @property
+    def work_item(self) -> int:
+        """Particles per work item of the tile kernels (a tile's particles are split into
+        work items of at most this many)."""
+        return self.tuning()["work_item"]
+
+    @work_item.setter
+    def work_item(self, particles: int):
+        check(self._L.pg_mpm_set_work_item(self._h, int(particles)), self._h)
+
+    def tuning(self) -> dict:
+        """Launch shape of the tiled path: tile workgroups, work-item size, tile edge
+        (cells), threads per tile workgroup."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(self._L.pg_mpm_get_tuning(self._h, *[ctypes.byref(t) for t in v]), self._h)
+        return {"tile_workgroups": v[0].value, "work_item": v[1].value, "tile_edge": v[2].value,
+                "block_threads": v[3].value}
+
     def autotune(self, trial_steps: int = 10, use_cache: bool = True) -> int:
-        """Pick the tile-kernel launch size by timing; returns the choice."""
+        """Pick the tile-kernel launch size and work-item size by timing; returns the
+        workgroup count (tuning() gives both)."""
         check(self._L.pg_mpm_autotune(self._h, int(trial_steps), 1 if use_cache else 0, _stream_ptr()), self._h)
         return self.launch
 

@@ -241,12 +241,17 @@ def test_autotune_restores_state_and_caches(gpu, tmp_path, monkeypatch):
     for key in before:
         assert np.array_equal(before[key], after[key]), key
     assert c.time()[1] == 2
-    assert cache.exists() and cache.read_text().strip().endswith(str(choice))
-    # a second context of the same shape takes the cached value without timing
+    tuned = c.tuning()
+    assert tuned["tile_workgroups"] == choice and tuned["work_item"] in (640, 1280, 2560)
+    last = cache.read_text().strip().splitlines()[-1].split()
+    assert last[1:] == [str(choice), str(tuned["work_item"])]
+    # a second context of the same shape takes the cached values without timing
     d = MPMContext(sc.params(impl=1))
     d.set_particles(sc.x, sc.v)
     d.launch = 256 if choice != 256 else 512
+    d.work_item = 5120
     assert d.autotune(trial_steps=3, use_cache=True) == choice
+    assert d.tuning() == tuned
     # the tuned launch gives the same trajectory
     d.step(2)
     c2 = {k: t.cpu().numpy() for k, t in d.get_particles().items()}
