@@ -257,19 +257,23 @@ int pg_mpm_set_graph(pg_mpm_ctx* ctx, int enable);
 /* Launch shape of the tiled path: the workgroups of the persistent tile
  * kernels, and the work-item size (a tile's particles are split into work
  * items of at most that many particles, [640, 5120], default 1280: smaller
- * spreads dense tiles over more workgroups, larger costs less per item).  The
- * autotuner picks both by timing trial_steps steps per candidate
- * (workgroups 256..8192 x work items 640, 1280, 2560) on the current particle
- * set (the state is restored afterwards).  Results are cached in a text file
- * ($PG_AUTOTUNE_CACHE, else ~/.cache/physgrad_amd_autotune.txt) keyed by
- * tile slots, grid blocks, particle count bucket and scene count; use_cache
- * reads that file first.  The tile edge (4 cells) and the tile workgroup (256
- * threads) are compile-time; pg_mpm_get_tuning reports all four.  The
+ * spreads dense tiles over more workgroups, larger costs less per item), and
+ * the threads of a tile workgroup (256: 4 waves, 3 waves per SIMD; or 512: 8
+ * waves, 2 per SIMD).  The autotuner picks all three by timing trial_steps
+ * steps per candidate (workgroups 256..8192 x work items 640, 1280, 2560 x
+ * 256 or 512 threads) on the current particle set (the state is restored
+ * afterwards).  Results are cached in a text file ($PG_AUTOTUNE_CACHE, else
+ * ~/.cache/physgrad_amd_autotune.txt) keyed by tile slots, grid blocks,
+ * particle count bucket and scene count; use_cache reads that file first.  The
+ * tile edge (4 cells) is compile-time: an 8-cell tile's P2G windows (10^3
+ * nodes, 16 KB each, two per wave) do not fit the 160 KB of LDS of a CU at
+ * more than one workgroup.  pg_mpm_get_tuning reports all four.  The
  * reference fixes its launch shapes at compile time (KernelConfig,
  * src/mpm_g2p2g_kernels.h:68-75). */
 int pg_mpm_get_launch(const pg_mpm_ctx* ctx, int* tile_workgroups);
 int pg_mpm_set_launch(pg_mpm_ctx* ctx, int tile_workgroups);
 int pg_mpm_set_work_item(pg_mpm_ctx* ctx, int particles);
+int pg_mpm_set_block(pg_mpm_ctx* ctx, int threads);
 int pg_mpm_get_tuning(const pg_mpm_ctx* ctx, int* tile_workgroups, int* work_item, int* tile_edge, int* block_threads);
 int pg_mpm_autotune(pg_mpm_ctx* ctx, int trial_steps, int use_cache, void* stream);
 

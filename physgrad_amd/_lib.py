@@ -108,6 +108,7 @@ _SIGNATURES = {
     "pg_mpm_set_launch": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_mpm_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pg_mpm_set_work_item": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pg_mpm_set_block": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pg_mpm_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "pg_launch_optimized_p2g": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),

@@ -225,6 +225,7 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
     // 64-scene batch, and a workgroup past the active list exits at once
     w.nblocks_launch = 4096;
     w.chunk = pg::CHUNK;
+    w.block = 256;
     // active-list entries: one per chunk of at most `chunk` particles of a
     // non-empty tile.  A tile with c particles has ceil(c / chunk) =
     // 1 + floor((c - 1) / chunk) entries, so a binning has at most
@@ -1199,9 +1200,9 @@ std::string tune_key(const pg_mpm_ctx* c) {
            ";n=" + std::to_string(nb) + ";scenes=" + std::to_string(c->nscenes);
 }
 
-// cached (launch size, work-item size) for this workload shape; false if none
-// (a line without the work-item size keeps the default)
-bool tune_lookup(const pg_mpm_ctx* c, int* wg, int* chunk) {
+// cached (launch size, work-item size, workgroup threads) for this workload
+// shape; false if none (missing trailing values keep their defaults)
+bool tune_lookup(const pg_mpm_ctx* c, int* wg, int* chunk, int* block) {
     FILE* f = std::fopen(tune_cache_path().c_str(), "r");
     if (!f) return false;
     const std::string key = tune_key(c);
@@ -1209,11 +1210,13 @@ bool tune_lookup(const pg_mpm_ctx* c, int* wg, int* chunk) {
     bool found = false;
     while (std::fgets(line, sizeof(line), f)) {
         char k[400];
-        int v = 0, ch = pg::CHUNK;
-        const int got = std::sscanf(line, "%399s %d %d", k, &v, &ch);
-        if (got >= 2 && key == k && v >= 8 && v <= (1 << 20) && ch >= pg::MIN_CHUNK && ch <= pg::MAX_CHUNK) {
+        int v = 0, ch = pg::CHUNK, bt = 256;
+        const int got = std::sscanf(line, "%399s %d %d %d", k, &v, &ch, &bt);
+        if (got >= 2 && key == k && v >= 8 && v <= (1 << 20) && ch >= pg::MIN_CHUNK && ch <= pg::MAX_CHUNK &&
+            (bt == 256 || bt == 512)) {
             *wg = v;
-            *chunk = got == 3 ? ch : pg::CHUNK;
+            *chunk = got >= 3 ? ch : pg::CHUNK;
+            *block = got >= 4 ? bt : 256;
             found = true;
         }
     }
@@ -1221,10 +1224,10 @@ bool tune_lookup(const pg_mpm_ctx* c, int* wg, int* chunk) {
     return found;
 }
 
-void tune_store(const pg_mpm_ctx* c, int wg, int chunk) {
+void tune_store(const pg_mpm_ctx* c, int wg, int chunk, int block) {
     FILE* f = std::fopen(tune_cache_path().c_str(), "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d\n", tune_key(c).c_str(), wg, chunk);
+    std::fprintf(f, "%s %d %d %d\n", tune_key(c).c_str(), wg, chunk, block);
     std::fclose(f);
 }
 
@@ -1319,13 +1322,25 @@ int pg_mpm_set_work_item(pg_mpm_ctx* c, int particles) {
     return PG_OK;
 }
 
+int pg_mpm_set_block(pg_mpm_ctx* c, int threads) {
+    if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
+    if (c->p.mode != PG_MODE_MLS_MPM || c->p.impl != PG_IMPL_TILED)
+        return fail(c, PG_ERR_INVALID, "the tile workgroup size applies to the tiled MLS-MPM path");
+    if (threads != 256 && threads != 512) return fail(c, PG_ERR_INVALID, "tile workgroup threads must be 256 or 512");
+    if (threads == c->tw.block) return PG_OK;
+    drop_graphs(c);
+    c->tw.block = threads;
+    c->version++;
+    return PG_OK;
+}
+
 int pg_mpm_get_tuning(const pg_mpm_ctx* c, int* tile_workgroups, int* work_item, int* tile_edge,
                       int* block_threads) {
     if (!c) return fail(nullptr, PG_ERR_INVALID, "NULL context");
     if (tile_workgroups) *tile_workgroups = c->tw.nblocks_launch;
     if (work_item) *work_item = c->tw.chunk;
     if (tile_edge) *tile_edge = pg::TT;
-    if (block_threads) *block_threads = pg::TB;
+    if (block_threads) *block_threads = c->tw.block;
     return PG_OK;
 }
 
@@ -1336,10 +1351,11 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
     if (trial_steps < 1) return fail(c, PG_ERR_INVALID, "trial_steps must be >= 1");
     if (c->n == 0) return PG_OK;
     if (use_cache) {
-        int wg = 0, ch = 0;
-        if (tune_lookup(c, &wg, &ch)) {
-            const int rc = pg_mpm_set_launch(c, wg);
-            return rc != PG_OK ? rc : pg_mpm_set_work_item(c, ch);
+        int wg = 0, ch = 0, bt = 0;
+        if (tune_lookup(c, &wg, &ch, &bt)) {
+            int rc = pg_mpm_set_launch(c, wg);
+            if (rc == PG_OK) rc = pg_mpm_set_work_item(c, ch);
+            return rc != PG_OK ? rc : pg_mpm_set_block(c, bt);
         }
     }
     hipStream_t st = (hipStream_t)stream;
@@ -1390,13 +1406,16 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
     hipEventCreate(&b);
     const int cand_wg[] = {256, 512, 1024, 2048, 4096, 8192};
     const int cand_chunk[] = {640, 1280, 2560};
-    int best = c->tw.nblocks_launch, best_chunk = c->tw.chunk;
+    const int cand_block[] = {256, 512};
+    int best = c->tw.nblocks_launch, best_chunk = c->tw.chunk, best_block = c->tw.block;
     float best_ms = 1e30f;
     hipError_t e = hipSuccess;
+    for (int bt : cand_block)
     for (int ch : cand_chunk) {
         for (int v : cand_wg) {
             c->tw.nblocks_launch = v;
             c->tw.chunk = ch;
+            c->tw.block = bt;
             float ms_sum = 0.f;
             for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
                 save(true);
@@ -1418,18 +1437,20 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
                 best_ms = ms_sum;
                 best = v;
                 best_chunk = ch;
+                best_block = bt;
             }
         }
         if (e != hipSuccess) break;
     }
     c->tw.nblocks_launch = best;
     c->tw.chunk = best_chunk;
+    c->tw.block = best_block;
     save(true);
     hipEventDestroy(a);
     hipEventDestroy(b);
     release();
     if (e != hipSuccess) return hip_fail(c, e, "autotune");
-    tune_store(c, best, best_chunk);
+    tune_store(c, best, best_chunk, best_block);
     return PG_OK;
 }
 

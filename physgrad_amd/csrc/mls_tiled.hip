@@ -255,12 +255,15 @@ __device__ __forceinline__ void p2g_global(float4* __restrict__ mps, uint32_t* b
 // global atomics.  Outputs are written
 // group by group, so stores coalesce.  At the end of the tile the windows are
 // summed into the tile's buffer.
-template <int MODE>
-__global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
+// BT: threads per workgroup (256: 4 waves, 3 waves per SIMD; 512: 8 waves,
+// 2 waves per SIMD), a launch-shape knob of the autotuner.
+template <int MODE, int BT>
+__global__ void __launch_bounds__(BT, BT == 256 ? 3 : 2) k_tile(TileArgs a) {
+    constexpr int TB = BT;        // threads of this workgroup shape
+    constexpr int NW = BT / 64;   // its waves
     constexpr bool G2P = (MODE & 1) != 0;
     constexpr bool P2G = (MODE & 2) != 0;
-    constexpr int PT = PASS / TB;                 // particles per thread in the bulk phases
-    static_assert(PASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
+    constexpr int PT = (PASS + TB - 1) / TB;      // particles per thread in the bulk phases
     __shared__ float4 sv[G2P ? WV : 1];          // G2P window: grid (v, m) on nodes [t0, t0 + WN)
     __shared__ float4 pw[P2G ? NW : 1][P2G ? 2 : 1][WV];  // per-wave P2G windows (m, p) on nodes [t0, t0 + WN):
                                                         // rank in cell < G, >= G
@@ -671,6 +674,13 @@ __global__ void __launch_bounds__(TB, 3) k_tile(TileArgs a) {
     }
 }
 
+// the tile kernel in the workgroup shape the context chose
+template <int MODE>
+void launch_tile(const TiledWork& w, const TileArgs& a, hipStream_t st) {
+    if (w.block == 512) k_tile<MODE, 512><<<w.nblocks_launch, 512, 0, st>>>(a);
+    else k_tile<MODE, 256><<<w.nblocks_launch, 256, 0, st>>>(a);
+}
+
 // Grid update of one flagged 4^3 node block (one wave, lane = node).  The
 // P2G window of tile T covers nodes [4T - 1, 4T + 7) per axis, so node block
 // B is covered by tiles B + {-1, 0, 1}^3, each node by 8 of them; the node's
@@ -1070,7 +1080,7 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
         set_io();
         {
             ProfScope ps(w.prof, PK_TILE_P2G, st);
-            k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
+            launch_tile<2>(w, a, st);
         }
         // the grid update reads the window buffers by this binning's active list,
         // so it runs before the next binning
@@ -1080,7 +1090,7 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
         set_io();
         {
             ProfScope ps(w.prof, PK_TILE_G2P2G, st);
-            k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
+            launch_tile<3>(w, a, st);
         }
         grid_update_and_count<false>(w, w.tslot, mp, gv, nullptr, st);
         e = bin_rest(w, n, w.perm, w.active, w.nactive, w.tslot, st);
@@ -1090,10 +1100,10 @@ hipError_t tiled_steps(TiledWork& w, int n, size_t cap, float4*& S, float4*& S2,
     set_io();
     if (leave_primed) {
         ProfScope ps(w.prof, PK_TILE_G2P2G, st);
-        k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
+        launch_tile<3>(w, a, st);
     } else {
         ProfScope ps(w.prof, PK_TILE_G2P, st);
-        k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(a);
+        launch_tile<1>(w, a, st);
     }
     std::swap(S, S2); std::swap(pid, pid2); std::swap(E, E2); std::swap(mass, mass2);
     return hipGetLastError();
@@ -1150,7 +1160,7 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
     a.mp = tape[0].mp;
     {
         ProfScope ps(w.prof, PK_TILE_P2G, st);
-        k_tile<2><<<w.nblocks_launch, TB, 0, st>>>(a);
+        launch_tile<2>(w, a, st);
     }
     grid_update<true>(w, tape[0].tslot, tape[0].mp, tape[0].gv, tape[0].flags, st);
     for (int s = 0; s < L; ++s) {
@@ -1175,7 +1185,7 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
             a.mp = nxt.mp;
             {
                 ProfScope ps(w.prof, PK_TILE_G2P2G, st);
-                k_tile<3><<<w.nblocks_launch, TB, 0, st>>>(a);
+                launch_tile<3>(w, a, st);
             }
             grid_update_and_count<true>(w, cur.tslot, nxt.mp, nxt.gv, nxt.flags, st);
             e = bin_rest(w, n, nxt.perm, nxt.alist, nxt.nactive, nxt.tslot, st);
@@ -1183,7 +1193,7 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
         } else {
             // last step of the segment: G2P only, to fix the end state's storage order
             ProfScope ps(w.prof, PK_TILE_G2P, st);
-            k_tile<1><<<w.nblocks_launch, TB, 0, st>>>(a);
+            launch_tile<1>(w, a, st);
         }
     }
     return hipGetLastError();

@@ -189,6 +189,7 @@ struct TiledWork {
     int max_entries;       // capacity of the active list (sized for MIN_CHUNK)
     int nblocks_launch;    // persistent grid size of the tile kernels
     int chunk;             // particles per work item (CHUNK by default)
+    int block;             // threads per tile workgroup: 256 (default) or 512
     uint32_t* err;         // [1] binning error word: ERR_* bits set by guarded writes, never cleared by a step
     MatTable mt;           // per-particle materials (recs NULL: the scenes')
     const uint32_t* node_bc;  // [total grid nodes] boundary flags per node, or NULL

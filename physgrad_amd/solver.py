@@ -140,6 +140,15 @@ class MPMContext:
     def work_item(self, particles: int):
         check(self._L.pg_mpm_set_work_item(self._h, int(particles)), self._h)
 
+    @property
+    def block_threads(self) -> int:
+        """Threads per tile workgroup (256 or 512)."""
+        return self.tuning()["block_threads"]
+
+    @block_threads.setter
+    def block_threads(self, threads: int):
+        check(self._L.pg_mpm_set_block(self._h, int(threads)), self._h)
+
     def tuning(self) -> dict:
         """Launch shape of the tiled path: tile workgroups, work-item size, tile edge
         (cells), threads per tile workgroup."""
@@ -149,8 +158,8 @@ class MPMContext:
                 "block_threads": v[3].value}
 
     def autotune(self, trial_steps: int = 10, use_cache: bool = True) -> int:
-        """Pick the tile-kernel launch size and work-item size by timing; returns the
-        workgroup count (tuning() gives both)."""
+        """Pick the tile-kernel launch size, work-item size and workgroup threads by
+        timing; returns the workgroup count (tuning() gives all)."""
         check(self._L.pg_mpm_autotune(self._h, int(trial_steps), 1 if use_cache else 0, _stream_ptr()), self._h)
         return self.launch
 

@@ -243,13 +243,15 @@ def test_autotune_restores_state_and_caches(gpu, tmp_path, monkeypatch):
     assert c.time()[1] == 2
     tuned = c.tuning()
     assert tuned["tile_workgroups"] == choice and tuned["work_item"] in (640, 1280, 2560)
+    assert tuned["block_threads"] in (256, 512)
     last = cache.read_text().strip().splitlines()[-1].split()
-    assert last[1:] == [str(choice), str(tuned["work_item"])]
+    assert last[1:] == [str(choice), str(tuned["work_item"]), str(tuned["block_threads"])]
     # a second context of the same shape takes the cached values without timing
     d = MPMContext(sc.params(impl=1))
     d.set_particles(sc.x, sc.v)
     d.launch = 256 if choice != 256 else 512
     d.work_item = 5120
+    d.block_threads = 512 if tuned["block_threads"] == 256 else 256
     assert d.autotune(trial_steps=3, use_cache=True) == choice
     assert d.tuning() == tuned
     # the tuned launch gives the same trajectory

@@ -1,7 +1,8 @@
-"""GPU: the tiled path's work-item size (pg_mpm_set_work_item, a knob of the
-autotuner): a dense tile split into work items of 640 .. 5120 particles (one
-to several LDS passes each) against the FP64 oracle, forward and reverse; the
-autotuner's choice of (workgroups, work-item size) and its cache."""
+"""GPU: the tiled path's launch-shape knobs of the autotuner: the work-item
+size (pg_mpm_set_work_item: a dense tile split into work items of 640 .. 5120
+particles, one to several LDS passes each) and the tile workgroup size
+(pg_mpm_set_block: 256 or 512 threads), against the FP64 oracle, forward and
+reverse."""
 import numpy as np
 import pytest
 import torch
@@ -15,8 +16,8 @@ def dense(rng, n=6000):
     return scene(rng, n, 1, span=0.2)   # 6,000 particles in about one 4^3-cell tile
 
 
-@pytest.mark.parametrize("work_item", [640, 2560, 5120])
-def test_work_item_forward_vs_fp64(gpu, oracle_lib, work_item):
+@pytest.mark.parametrize("work_item,block", [(640, 256), (2560, 256), (5120, 256), (1280, 512), (2560, 512)])
+def test_work_item_forward_vs_fp64(gpu, oracle_lib, work_item, block):
     from physgrad_amd.solver import MPMContext
     O = oracle_lib
     rng = np.random.default_rng(60)
@@ -24,7 +25,8 @@ def test_work_item_forward_vs_fp64(gpu, oracle_lib, work_item):
     n = sc.n
     ctx = MPMContext(sc.params(impl=1))
     ctx.work_item = work_item
-    assert ctx.tuning()["work_item"] == work_item
+    ctx.block_threads = block
+    assert ctx.tuning()["work_item"] == work_item and ctx.tuning()["block_threads"] == block
     ctx.set_particles(sc.x, sc.v, E=sc.E_particles)
     prm = O.scene_params(sc)
     x64, v64 = sc.x.astype(np.float64), sc.v.astype(np.float64)
@@ -41,8 +43,8 @@ def test_work_item_forward_vs_fp64(gpu, oracle_lib, work_item):
     ctx.close()
 
 
-@pytest.mark.parametrize("work_item", [640, 2560])
-def test_work_item_backward_vs_oracle_reverse(gpu, oracle_lib, work_item):
+@pytest.mark.parametrize("work_item,block", [(640, 256), (2560, 256), (1280, 512)])
+def test_work_item_backward_vs_oracle_reverse(gpu, oracle_lib, work_item, block):
     from physgrad_amd.solver import MPMContext
     O = oracle_lib
     rng = np.random.default_rng(61)
@@ -54,6 +56,7 @@ def test_work_item_backward_vs_oracle_reverse(gpu, oracle_lib, work_item):
     print("work item backward", work_item, flush=True)
     ctx = MPMContext(sc.params(impl=1))
     ctx.work_item = work_item
+    ctx.block_threads = block
     ctx.set_particles(sc.x, sc.v, C, F, E=sc.E_particles)
     ctx.forward_checkpointed(nsteps, K)
     got = ctx.backward(*[torch.as_tensor(s, dtype=torch.float32) for s in seeds])
@@ -72,13 +75,14 @@ def test_work_item_change_between_calls(gpu, oracle_lib):
     rng = np.random.default_rng(62)
     sc = dense(rng)
     out = []
-    for sizes in ((1280, 1280), (1280, 640)):
+    for sizes in ((1280, 1280, 256), (1280, 640, 512)):
         ctx = MPMContext(sc.params(impl=1))
         ctx.set_graph(True)
         ctx.set_particles(sc.x, sc.v, E=sc.E_particles)
         ctx.work_item = sizes[0]
         ctx.step(2)
         ctx.work_item = sizes[1]
+        ctx.block_threads = sizes[2]
         ctx.step(2)
         out.append({k: t.cpu().numpy() for k, t in ctx.get_particles().items()})
         ctx.close()
@@ -96,6 +100,9 @@ def test_work_item_bounds(gpu):
     for bad in (639, 5121, 0):
         with pytest.raises(PgError, match="work item"):
             ctx.work_item = bad
+    for bad in (128, 384, 1024):
+        with pytest.raises(PgError, match="256 or 512"):
+            ctx.block_threads = bad
     ctx.close()
     plain = MPMContext(dense(rng, 100).params(impl=0))
     with pytest.raises(PgError):
