@@ -6,10 +6,15 @@ over all particles with the particle state resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
-Prints one JSON line: particle-steps/s of the tiled path (value), the plain
-GPU port measured in the same run (forward, and forward+backward), the roofline of the dominant kernel (the
-fused G2P+P2G tile kernel, timed with HIP events on its launch stream) and the
-CPU baseline (FP32 port of the same step on one host core, bounded sample).
+Prints one JSON line: particle-steps/s of the tiled path (value, launch shape
+autotuned), the roofline of the dominant kernel (the fused G2P+P2G tile
+kernel, timed with HIP events on its launch stream), forward+backward with
+checkpoint recompute (one 1M scene and the 16-scene batch, K = 16), rollout
+gradients vs closed form and FP64 finite differences, the 64-scene batch, the
+SPH-MPM coupling pair search and forces (1M x 1M), the plain GPU port measured
+in the same run, the FP64 error of the forward, and the CPU baselines (FP32
+port of the same step on the host cores, the reference's CPU solver, the
+all-pairs coupling loop; bounded samples).
 """
 from __future__ import annotations
 
