@@ -126,7 +126,12 @@ def run(sc, launch, steps, backward=False):
 @pytest.mark.parametrize("launch", [8, 100, 500, 1001])
 def test_launch_sizes_not_multiple_of_8(gpu, launch):
     """Workgroup counts that do not divide over the 8 XCDs (and one per XCD)
-    take every work item: forward and backward equal the default launch."""
+    take every work item: forward and backward equal the default launch to
+    float rounding.  Two float32 runs whose summation orders differ (the
+    binning's atomics depend on the launch and on timing) agree to about 1e-6
+    (1e-5 for dL/dE, a sum with cancellation over the steps), measured here
+    between two runs of the default launch too; a work item left out would
+    change its particles' results by O(1)."""
     from physgrad_amd.scenes import Scene
     rng = np.random.default_rng(9)
     n = 20000
@@ -135,13 +140,19 @@ def test_launch_sizes_not_multiple_of_8(gpu, launch):
     E = np.exp(rng.uniform(np.log(5e3), np.log(5e4), n)).astype(np.float32)
     sc = Scene("launch", (48, 48, 48), 1 / 48, 1e-4, 5, 0, 2e4, 0.3, 1000.0, 0.6 ** 3 / n, x, v, E_particles=E)
     print("launch", launch, flush=True)
+
+    def errs(got, want, keys):
+        return {k: float(np.abs(got[k].astype(np.float64) - want[k]).max() / np.abs(want[k]).max()) for k in keys}
+
     want = run(sc, 0, 5)
-    got = run(sc, launch, 5)
-    for k in ("x", "v", "C", "F"):
-        a, b = got[k].reshape(-1).astype(np.float64), want[k].reshape(-1).astype(np.float64)
-        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max(), k
+    again = errs(run(sc, 0, 5), want, ("x", "v", "C", "F"))
+    got = errs(run(sc, launch, 5), want, ("x", "v", "C", "F"))
+    print("launch", launch, "forward", got, "default run to run", again, flush=True)
+    for k, err in got.items():
+        assert err <= 1e-5, (k, err)
     want = run(sc, 0, 4, backward=True)
-    got = run(sc, launch, 4, backward=True)
-    for k in ("x", "v", "C", "F", "E"):
-        a, b = got[k].reshape(-1).astype(np.float64), want[k].reshape(-1).astype(np.float64)
-        assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max(), k
+    again = errs(run(sc, 0, 4, backward=True), want, ("x", "v", "C", "F", "E"))
+    got = errs(run(sc, launch, 4, backward=True), want, ("x", "v", "C", "F", "E"))
+    print("launch", launch, "backward", got, "default run to run", again, flush=True)
+    for k, err in got.items():
+        assert err <= (1e-4 if k == "E" else 1e-5), (k, err)
