@@ -40,6 +40,8 @@
  *       SPHMPMCouplingSystem<T>::findCouplingPairs    src/sph_mpm_coupling.h:454-486
  *   pg_coupling_compute_forces
  *       SPHMPMCouplingSystem<T>::computeCouplingForces src/sph_mpm_coupling.h:488-545
+ *   pg_csr_spmv
+ *       SparseMatrix<T,IndexType>::multiply src/sparse_data_structures.h:392-410
  */
 #ifndef PHYSGRAD_MPM_H
 #define PHYSGRAD_MPM_H
@@ -363,6 +365,14 @@ int pg_hash_neighbors_count(const float* queries, unsigned nq, float radius, flo
 int pg_hash_neighbors_fill(const float* queries, unsigned nq, float radius, float cell_size, const float* domain_min,
                            const unsigned* res, const unsigned* particle_indices, const unsigned* cell_start,
                            const unsigned* cell_count, const unsigned* row_ptr, unsigned* neighbors, void* stream);
+
+/* CSR sparse matrix-vector product y = A x (SparseMatrix::multiply,
+ * src/sparse_data_structures.h:392-410): row_ptr [nrows+1], col and val
+ * [row_ptr[nrows]], x [ncols], y [nrows], all device arrays; each row's
+ * products are summed in its stored order, in float, as the reference does.
+ * A column index >= ncols makes its row NaN (no read outside x). */
+int pg_csr_spmv(const unsigned* row_ptr, const unsigned* col, const float* val, const float* x, unsigned ncols,
+                float* y, unsigned nrows, void* stream);
 
 /* ---------------------------------------------------------------- SPH-MPM coupling
  * SPHMPMCouplingSystem (src/sph_mpm_coupling.h:391-579) on the GPU.  The

@@ -12,6 +12,8 @@
  *       all (sph, mpm) with |x_mpm - x_sph| < radius, SPH-major, MPM ascending,
  *       penetration = radius - distance
  *   SPHMPMCouplingSystem::computeCouplingForces  src/sph_mpm_coupling.h:488-545
+ *   SparseMatrix::multiply                       src/sparse_data_structures.h:392-410
+ *       per row, sum = sum + value * x[column] in stored order
  *       spring-damper normal force, Coulomb-style friction, two-way reaction
  *       accumulated onto the SPH particle in pair order
  * Float arithmetic in the order the reference writes it (built with
@@ -124,4 +126,13 @@ void orc_coupling_forces(const float* sph_x, const float* sph_v, unsigned ns, co
             if (two_way)
                 for (int a = 0; a < 3; ++a) sph_force[3 * (size_t)s + a] = sph_force[3 * (size_t)s + a] + ff[a];
         }
+}
+
+void orc_csr_spmv(const unsigned* row_ptr, const unsigned* col, const float* val, const float* x, unsigned nrows,
+                  float* y) {
+    for (unsigned r = 0; r < nrows; ++r) {
+        float sum = 0.0f;
+        for (unsigned k = row_ptr[r]; k < row_ptr[r + 1]; ++k) sum += val[k] * x[col[k]];
+        y[r] = sum;
+    }
 }

@@ -140,6 +140,10 @@ void orc_coupling_forces(const float* sph_x, const float* sph_v, unsigned ns, co
                          float stiffness, float damping, float friction, int two_way, float* solid_force,
                          float* fluid_force, float* sph_force);
 
+/* SparseMatrix::multiply over a CSR matrix: y [nrows] */
+void orc_csr_spmv(const unsigned* row_ptr, const unsigned* col, const float* val, const float* x, unsigned nrows,
+                  float* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,7 @@ def lib():
         L.orc_coupling_forces.argtypes = [_f32p, _f32p, ctypes.c_uint, _f32p, _f32p, _u32p, _u32p, _f32p,
                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                           _f32p, _f32p, _f32p]
+        L.orc_csr_spmv.argtypes = [_u32p, _u32p, _f32p, _f32p, ctypes.c_uint, _f32p]
         _lib = L
     return _lib
 
@@ -123,6 +124,8 @@ def ref():
                                    ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, _u32p, _u32p,
                                    _f32p, _f32p, _f32p, _f32p, ctypes.c_ulonglong]
         R.ref_coupling.restype = ctypes.c_longlong
+        R.ref_sparse_matrix.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, _u32p, _u32p, _f32p, _f32p,
+                                        _f32p, _u32p]
         _ref = R
     return _ref
 
@@ -426,3 +429,31 @@ def reference_coupling(sph_x, sph_v, mpm_x, mpm_v, radius, stiffness, damping, f
     return dict(sph=ps[:k].copy(), mpm=pm[:k].copy(), penetration=pen[:k].copy(),
                 solid_force=fs[:3 * k].reshape(-1, 3).copy(), fluid_force=ff[:3 * k].reshape(-1, 3).copy(),
                 sph_force=sf[:3 * ns].reshape(-1, 3).copy())
+
+
+def csr_spmv(row_ptr, col, val, x):
+    """SparseMatrix::multiply: y = A x, each row summed in stored order (float32)."""
+    rp = np.ascontiguousarray(row_ptr, np.uint32)
+    nrows = rp.size - 1
+    col = np.ascontiguousarray(col, np.uint32)
+    val = np.ascontiguousarray(val, np.float32)
+    y = np.zeros(max(nrows, 1), np.float32)
+    lib().orc_csr_spmv(rp, np.resize(col, max(col.size, 1)), np.resize(val, max(val.size, 1)),
+                       np.ascontiguousarray(x, np.float32), nrows, y)
+    return y[:nrows]
+
+
+def reference_sparse_matrix(nrows, ncols, rows, cols, vals, x):
+    """The reference's compiled SparseMatrix<float,uint32_t> fed with
+    addElement(rows[k], cols[k], vals[k]) in the given order, finalize(),
+    multiply(x): (row_pointers [nrows+1], y [nrows])."""
+    R = ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libref_mpm.so is not built")
+    rows = np.ascontiguousarray(rows, np.uint32)
+    cols = np.ascontiguousarray(cols, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    y = np.zeros(max(nrows, 1), np.float32)
+    rp = np.zeros(nrows + 1, np.uint32)
+    R.ref_sparse_matrix(nrows, ncols, rows.size, rows, cols, vals, np.ascontiguousarray(x, np.float32), y, rp)
+    return rp, y[:nrows]

@@ -158,4 +158,19 @@ long long ref_coupling(const float* sph_x, const float* sph_v, unsigned ns, cons
     return (long long)F.size();
 }
 
+// Reference SparseMatrix<float,uint32_t> (sparse_data_structures.h:304-443):
+// addElement in the given order, finalize, multiply; row pointers and y out.
+int ref_sparse_matrix(unsigned nrows, unsigned ncols, unsigned nnz, const unsigned* rows, const unsigned* cols,
+                      const float* vals, const float* x, float* y, unsigned* row_ptr) {
+    physgrad::sparse::SparseMatrix<float, uint32_t> A(nrows, ncols, nnz);
+    for (unsigned k = 0; k < nnz; ++k) A.addElement(rows[k], cols[k], vals[k]);
+    A.finalize();
+    std::vector<float> xv(x, x + ncols);
+    std::vector<float> yv = A.multiply(xv);
+    for (unsigned r = 0; r < nrows; ++r) y[r] = yv[r];
+    const auto& rp = A.getRowPointers();
+    for (unsigned r = 0; r <= nrows; ++r) row_ptr[r] = rp[r];
+    return 0;
+}
+
 }  // extern "C"

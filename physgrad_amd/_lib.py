@@ -128,6 +128,7 @@ _SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_ulonglong), _vp, ctypes.c_size_t, _vp]),
     "pg_hash_neighbors_fill": (ctypes.c_int, [_vp, ctypes.c_uint, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp, _vp,
                                               _vp, _vp, _vp, _vp]),
+    "pg_csr_spmv": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint, _vp, ctypes.c_uint, _vp]),
     "pg_coupling_default_config": (None, [_vp]),
     "pg_coupling_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "pg_coupling_destroy": (None, [_vp]),

@@ -475,6 +475,28 @@ __global__ void k_cpl_accum(unsigned ns, const uint32_t* __restrict__ row_ptr, c
     sph_force[3 * (size_t)s + 2] = f2;
 }
 
+// CSR sparse matrix-vector product, one thread per row, the row's products
+// summed in column-index order (SparseMatrix::multiply,
+// src/sparse_data_structures.h:392-410).  A column index >= ncols makes its
+// row NaN instead of reading outside x.
+__global__ void k_csr_spmv(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                           const float* __restrict__ val, const float* __restrict__ x, uint32_t ncols,
+                           float* __restrict__ y, uint32_t nrows) {
+#pragma clang fp contract(off)
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    float sum = 0.f;
+    for (uint32_t k = row_ptr[r], e = row_ptr[r + 1]; k < e; ++k) {
+        const uint32_t c = col[k];
+        if (c >= ncols) {
+            sum = __builtin_nanf("");
+            break;
+        }
+        sum += val[k] * x[c];
+    }
+    y[r] = sum;
+}
+
 __global__ void k_set_u32(uint32_t* p, uint32_t v) {
     if (threadIdx.x == 0) *p = v;
 }
@@ -568,6 +590,22 @@ extern "C" int pg_hash_neighbors_fill(const float* queries, unsigned nq, float r
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         pg::set_thread_error((std::string("hash neighbours: ") + hipGetErrorString(e)).c_str());
+        return PG_ERR_HIP;
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_csr_spmv(const unsigned* row_ptr, const unsigned* col, const float* val, const float* x,
+                           unsigned ncols, float* y, unsigned nrows, void* stream) {
+    if (nrows > 0 && (!row_ptr || !y || (ncols > 0 && !x))) {
+        pg::set_thread_error("csr spmv: NULL argument");
+        return PG_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (nrows) pg::k_csr_spmv<<<pg::nblocks(nrows, 256), 256, 0, st>>>(row_ptr, col, val, x, ncols, y, nrows);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        pg::set_thread_error((std::string("csr spmv: ") + hipGetErrorString(e)).c_str());
         return PG_ERR_HIP;
     }
     return PG_OK;

@@ -63,15 +63,15 @@ def test_dropin_matches_reference_fixture(gpu, tmp_path):
 def build_driver(tmp_path, name):
     exe = os.path.join(str(tmp_path), name)
     lib_dir = os.path.join(ROOT, "physgrad_amd")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
-           os.path.join(ROOT, "tests", "cpp", name + ".cpp"), f"-L{lib_dir}", "-lphysgrad_mpm",
-           f"-Wl,-rpath,{lib_dir}", "-o", exe]
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           f"-I{os.path.join(ROOT, 'include')}", os.path.join(ROOT, "tests", "cpp", name + ".cpp"), f"-L{lib_dir}",
+           "-lphysgrad_mpm", "-L/opt/rocm/lib", "-lamdhip64", f"-Wl,-rpath,{lib_dir}", "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
 
 
-@pytest.mark.parametrize("name", ["dropin_gpu_solver", "dropin_differentiable"])
+@pytest.mark.parametrize("name", ["dropin_gpu_solver", "dropin_differentiable", "dropin_sparse"])
 def test_gpu_dropins_compile_and_link(tmp_path, name):
     assert os.path.exists(build_driver(tmp_path, name))
 
@@ -304,3 +304,74 @@ def test_differentiable_dropin_vs_oracle_reverse(gpu, oracle_lib, tmp_path):
         print(key, err)
         assert err <= 1e-3, (key, err)
     assert "verifyGradients passed" in r.stdout, r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cell,dmax,radius,n,nq", [(0.1, (1.0, 1.0, 1.0), 0.1, 5000, 300),
+                                                   (0.13, (1.0, 0.7, 0.9), 0.3, 3000, 100)])
+def test_sparse_dropin_hash_vs_oracle(gpu, oracle_lib, tmp_path, cell, dmax, radius, n, nq):
+    """SpatialHashGrid<float> of the drop-in: buildHashTable (cell ids, order,
+    starts, counts) and getNeighbors / getNeighborsBatch against the oracle,
+    which is bit-identical to the reference's class."""
+    O = oracle_lib
+    exe = build_driver(tmp_path, "dropin_sparse")
+    rng = np.random.default_rng(n)
+    dmax = np.array(dmax, np.float32)
+    pos = (rng.random((n, 3)) * dmax).astype(np.float32)
+    q = (rng.random((nq, 3)) * dmax).astype(np.float32)
+    inp, out = os.path.join(str(tmp_path), "in.bin"), os.path.join(str(tmp_path), "out.bin")
+    with open(inp, "wb") as fh:
+        np.array([cell, 0, 0, 0, *dmax, radius], np.float32).tofile(fh)
+        np.array([n, nq], np.uint32).tofile(fh)
+        pos.tofile(fh)
+        q.tofile(fh)
+    print("sparse drop-in hash", n, nq, flush=True)
+    r = subprocess.run([exe, "hash", inp, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "threw 1" in r.stdout, r.stdout
+    res = np.ceil(dmax / np.float32(cell)).astype(np.uint32)
+    C = int(np.prod(res.astype(np.uint64)))
+    ids = O.cell_ids(pos, cell, (0, 0, 0), res)
+    idx, st, ct = O.counting_sort(ids, C)
+    orp, onb = O.hash_neighbors(q, radius, cell, (0, 0, 0), res, idx, st, ct)
+    d = np.fromfile(out, np.uint32)
+    parts = np.split(d, np.cumsum([n, n, C, C, nq + 1, len(onb)]))
+    for got, want in zip(parts[:6], (ids, idx, st, ct, orp, onb)):
+        np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(parts[6], onb[orp[0]:orp[1]])
+
+
+@pytest.mark.gpu
+def test_sparse_dropin_matrix_vs_oracle(gpu, oracle_lib, tmp_path):
+    """SparseMatrix<float> of the drop-in: addElement in row-major order,
+    finalize, multiply on the GPU against the oracle's row-ordered sums, bit for
+    bit (the reference's own class loses the rows and returns zeros)."""
+    O = oracle_lib
+    exe = build_driver(tmp_path, "dropin_sparse")
+    rng = np.random.default_rng(70)
+    nr, nc, nnz = 3000, 2500, 40000
+    rows = np.sort(rng.integers(0, nr, nnz)).astype(np.uint32)
+    rows[rows == 7] = 8            # an empty row
+    rows = np.sort(rows)
+    cols = rng.integers(0, nc, nnz).astype(np.uint32)
+    vals = rng.normal(size=nnz).astype(np.float32)
+    x = rng.normal(size=nc).astype(np.float32)
+    inp, out = os.path.join(str(tmp_path), "in.bin"), os.path.join(str(tmp_path), "out.bin")
+    with open(inp, "wb") as fh:
+        np.array([nr, nc, nnz], np.uint32).tofile(fh)
+        rows.tofile(fh)
+        cols.tofile(fh)
+        vals.tofile(fh)
+        x.tofile(fh)
+    print("sparse drop-in matrix", nr, nc, nnz, flush=True)
+    r = subprocess.run([exe, "matrix", inp, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    d = np.fromfile(out, np.uint32)
+    rp = d[:nr + 1]
+    y = d[nr + 1:].view(np.float32)
+    want_rp = np.zeros(nr + 1, np.uint32)
+    want_rp[1:] = np.cumsum(np.bincount(rows, minlength=nr))
+    np.testing.assert_array_equal(rp, want_rp)
+    assert rp[8] == rp[7]
+    want = O.csr_spmv(want_rp, cols, vals, x)
+    np.testing.assert_array_equal(y.view(np.uint32), want.view(np.uint32))

@@ -178,3 +178,36 @@ def test_million_particles_properties(gpu, oracle_lib):
     for i, s in enumerate(sample):
         np.testing.assert_array_equal(col[rp[s]:rp[s + 1]], ocol[orp[i]:orp[i + 1]])
         np.testing.assert_array_equal(_bits(pen[rp[s]:rp[s + 1]]), _bits(opn[orp[i]:orp[i + 1]]))
+
+
+def test_csr_spmv_vs_oracle_and_column_guard(gpu, oracle_lib):
+    """pg_csr_spmv over the coupling pairs as a matrix (rows: SPH particles,
+    columns: MPM particles, values: penetrations) against the oracle's
+    row-ordered sums, bit for bit; a column index past x makes its row NaN."""
+    import ctypes
+    from physgrad_amd import _lib
+    from physgrad_amd.spatial import SPHMPMCoupling
+    rng = np.random.default_rng(46)
+    ns, nm = 4000, 5000
+    xs = torch.rand(ns, 3)
+    xm = torch.rand(nm, 3)
+    c = SPHMPMCoupling(coupling_radius=0.05)
+    c.findCouplingPairs(xs, xm)
+    rp, col, pen = c.getCouplingPairs()
+    c.close()
+    x = torch.as_tensor(rng.normal(size=nm).astype(np.float32)).cuda()
+    rp32, col32 = rp.to(torch.int32).cuda(), col.to(torch.int32).cuda()
+    y = torch.empty(ns, dtype=torch.float32, device="cuda")
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = _lib.load()
+    assert L.pg_csr_spmv(p(rp32), p(col32), p(pen), p(x), nm, p(y), ns, st) == 0
+    want = oracle_lib.csr_spmv(_np(rp), _np(col), _np(pen), _np(x))
+    np.testing.assert_array_equal(_np(y).view(np.uint32), want.view(np.uint32))
+    # the same with x one column short: rows that reach the last column are NaN
+    assert L.pg_csr_spmv(p(rp32), p(col32), p(pen), p(x), nm - 1, p(y), ns, st) == 0
+    rows_last = np.repeat(np.arange(ns), np.diff(_np(rp)))[_np(col) == nm - 1]
+    got = _np(y)
+    assert np.isnan(got[rows_last]).all()
+    others = np.setdiff1d(np.arange(ns), rows_last)
+    np.testing.assert_array_equal(got[others].view(np.uint32), want[others].view(np.uint32))

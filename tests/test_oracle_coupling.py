@@ -142,3 +142,34 @@ def test_empty_inputs():
     assert rp.tolist() == [0] and len(col) == 0
     rp, col, pen = O.coupling_pairs(np.ones((4, 3), np.float32), np.zeros((0, 3), np.float32), 0.1)
     assert rp.tolist() == [0] * 5 and len(col) == 0
+
+
+def test_csr_spmv_matches_scipy():
+    import scipy.sparse as sp
+    rng = np.random.default_rng(8)
+    nr, nc = 200, 150
+    A = sp.random(nr, nc, density=0.05, random_state=1, format="csr", dtype=np.float64)
+    A.data = A.data.astype(np.float32).astype(np.float64)
+    x = rng.normal(size=nc).astype(np.float32)
+    y = O.csr_spmv(A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data.astype(np.float32), x)
+    np.testing.assert_allclose(y, A @ x.astype(np.float64), rtol=1e-5, atol=1e-5 * np.abs(A @ x).max())
+    # float sums in stored order: bit-equal to a numpy float32 restatement
+    want = np.zeros(nr, np.float32)
+    for r in range(nr):
+        s = np.float32(0)
+        for k in range(A.indptr[r], A.indptr[r + 1]):
+            s = np.float32(s + np.float32(np.float32(A.data[k]) * x[A.indices[k]]))
+        want[r] = s
+    np.testing.assert_array_equal(y.view(np.uint32), want.view(np.uint32))
+
+
+@needs_ref
+def test_reference_sparse_matrix_loses_its_rows():
+    """The reference SparseMatrix (addElement drops the row, finalize leaves the
+    row pointers at zero): every product is zero.  The drop-in records rows."""
+    rng = np.random.default_rng(9)
+    rows = np.sort(rng.integers(0, 20, 100)).astype(np.uint32)
+    cols = rng.integers(0, 10, 100).astype(np.uint32)
+    vals = rng.normal(size=100).astype(np.float32) + 3
+    rp, y = O.reference_sparse_matrix(20, 10, rows, cols, vals, np.ones(10, np.float32))
+    assert not rp.any() and not y.any()
