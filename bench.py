@@ -233,10 +233,11 @@ def batch_leg(steps):
     b.step(10)
     t_graph, _ = time_steps(b, steps, steps)
     ok = bool(torch.isfinite(b.get_particles()["x"]).all().item())
+    shape = b.tuning()
     b.close()
     return {"value": n * steps / t_graph, "unit": "particle-steps/s", "scenes": len(scenes), "particles": n,
             "steps": steps, "ms_per_step": 1e3 * t_graph / steps, "ms_per_step_eager": 1e3 * t_eager / steps,
-            "tile_workgroups": tuned, "tuning": b.tuning(), "state_finite": ok,
+            "tile_workgroups": tuned, "tuning": shape, "state_finite": ok,
             "kernel_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
             "workload": "batch_64: 64 scenes, side 32/48/64, 4,096..65,536 particles each (seed 4), "
                         "fixed-corotated / neo-Hookean, dt 1e-4, hipGraph replay of steps [10 + steps, 10 + 2 steps)"}
