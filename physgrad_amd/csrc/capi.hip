@@ -244,7 +244,7 @@ int init_mls(pg_mpm_ctx* c, const pg_scene_desc* d, int ns) {
                    (rc = dalloc(c, &w.active, 4 * max_entries)) || (rc = dalloc(c, &w.nactive, 1)) ||
                    (rc = dalloc(c, &w.key, c->cap)) || (rc = dalloc(c, &w.rank, c->cap)) ||
                    (rc = dalloc(c, &w.perm, c->cap)) || (rc = dalloc(c, &w.flags, boff)) ||
-                   (rc = dalloc(c, &w.bsum, 2 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, 2 * toff)) ||
+                   (rc = dalloc(c, &w.bsum, 5 * ((toff + 1023) / 1024))) || (rc = dalloc(c, &w.tslot, 2 * toff)) ||
                    (rc = dalloc(c, &w.tscene, toff)) || (rc = dalloc(c, &w.bscene, boff)) ||
                    (rc = dalloc(c, &w.bflag, toff)) || (rc = dalloc(c, &w.blist, toff)) ||
                    (rc = dalloc(c, &w.nblist, 1)) || (rc = dalloc(c, &w.err, 1)))))

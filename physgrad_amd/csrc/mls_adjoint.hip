@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
     __shared__ uint8_t lcell[APASS];          // stencil-base cell by output offset (64: outside)
     __shared__ uint32_t lcnt[65];
     __shared__ uint32_t s_max, s_tot;
-    const XcdChunk ch(*a.nactive);
+    const ItemStride ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];
@@ -626,7 +626,7 @@ __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, GetF getF
 
 __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per SIMD: latency-bound gather
     __shared__ float4 sp[WV];
-    const XcdChunk ch(*a.nactive);
+    const ItemStride ch(*a.nactive);
     const int tid = threadIdx.x;
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
         const uint4 te = a.alist[ti];

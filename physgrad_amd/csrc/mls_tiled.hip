@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 3 : 2) k_tile(TileArgs a) {
     static_assert(PV <= 4 * PASS, "the changers' cell tags fit in lslot");
     static_assert(PASS / 32 <= 64, "one wave compacts the P2G queue");
 
-    const XcdChunk ch(*a.nactive);
+    const ItemStride ch(*a.nactive);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
@@ -472,13 +472,12 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 3 : 2) k_tile(TileArgs a) {
 
             // one particle: j = its cell-order index, oo = its output offset, own_ok = it
             // is in a round (its base is a cell of the tile: G2P from the window),
-            // win = which of the wave's two P2G windows it adds into
-            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok, int win) {
+            // win = which of the wave's two P2G windows it adds into; s = its storage
+            // slot and q0 = its (x, F00), loaded by the caller
+            auto particle_q = [&](uint32_t j, uint32_t oo, bool own_ok, int win, uint32_t s, float4 q0) {
+                const float4 q1 = a.Sin[a.cap + s], q2 = a.Sin[2 * a.cap + s];
                 float x[3], F[9], v[3], C[9];
-                const uint32_t s = lslot[j];
                 const uint32_t o = beg + pb + oo;
-                // x, F from the state planes (Q0 was read by the bulk phase and is in cache)
-                const float4 q0 = a.Sin[s], q1 = a.Sin[a.cap + s], q2 = a.Sin[2 * a.cap + s];
                 x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
                 F[0] = q0.w;
                 int ob = -1;
@@ -530,6 +529,11 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 3 : 2) k_tile(TileArgs a) {
                 }
                 if (P2G) scatter(stn, F, C, v, particle_mat(g, a.mt, uid, a.E_in, s, mq), mq, own_ok, oo, ob, win);
             };
+            // the same with its slot and (x, F00) loaded here
+            auto particle = [&](uint32_t j, uint32_t oo, bool own_ok, int win) {
+                const uint32_t s = lslot[j];
+                particle_q(j, oo, own_ok, win, s, a.Sin[s]);
+            };
 
             // ---- rounds: cell-order index j = grp + lane G of group grp (grp < G); the
             // groups' output ranges follow each other (group grp holds ceil((tot - grp) / G)
@@ -538,9 +542,22 @@ __global__ void __launch_bounds__(BT, BT == 256 ? 3 : 2) k_tile(TileArgs a) {
             const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
             if (G > 0) {
                 const uint32_t gq = tot / G, gr = tot % G;
+                // a lane loads the slot and (x, F00) of its particle in the wave's next
+                // group while it works on the current one (the G2P addresses need x)
+                uint32_t jn = wave + lane * G;
+                bool vn = wave < G && jn < tot;
+                uint32_t sn = vn ? lslot[jn] : 0u;
+                float4 q0n = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (vn) q0n = a.Sin[sn];
                 for (uint32_t grp = wave; grp < G; grp += NW) {
-                    const uint32_t j = grp + lane * G;
-                    if (j < tot) particle(j, grp * gq + min(grp, gr) + lane, true, lrank[j] >= G ? 1 : 0);
+                    const uint32_t j = jn;
+                    const bool val = vn;
+                    const uint32_t s0 = sn;
+                    const float4 q0 = q0n;
+                    jn = grp + NW + lane * G;
+                    vn = grp + NW < G && jn < tot;
+                    if (vn) { sn = lslot[jn]; q0n = a.Sin[sn]; }
+                    if (val) particle_q(j, grp * gq + min(grp, gr) + lane, true, lrank[j] >= G ? 1 : 0, s0, q0);
                 }
             }
             // ---- particles whose stencil base is outside the tile (clamped keys)
@@ -705,19 +722,21 @@ __device__ __forceinline__ void grid_block(uint32_t B, const SceneInfo* __restri
     const uint32_t lb = B - sc.toff;
     const int bc[3] = {(int)compact3(lb), (int)compact3(lb >> 1), (int)compact3(lb >> 2)};
     const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
-    // first active-list entry of a tile and its number of entries (chunks)
-    auto slot_of = [&](int tx, int ty, int tz, uint32_t& ne) -> uint32_t {
-        ne = 0u;
-        if (tx < 0 || ty < 0 || tz < 0 || tx >= nt[0] || ty >= nt[1] || tz >= nt[2]) return 0xFFFFFFFFu;
-        const uint2 f = reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(tx, ty, tz)];
-        ne = f.y;
-        return f.x;
+    // first active-list entry of a tile and its number of entries (chunks): (first, 0) outside the grid
+    auto slot_of = [&](int tx, int ty, int tz) -> uint2 {
+        if (tx < 0 || ty < 0 || tz < 0 || tx >= nt[0] || ty >= nt[1] || tz >= nt[2]) return make_uint2(0u, 0u);
+        return reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(tx, ty, tz)];
     };
     const MlsGeom g = sc.g;
     const int lc[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
     const int i = bc[0] * TT + lc[0], j = bc[1] * TT + lc[1], k = bc[2] * TT + lc[2];
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     if (wb) {
+        // the entry ranges of the 8 covering tiles, then their first windows, as
+        // independent loads (two round trips); the further windows of a tile split
+        // into several work items after that
+        uint2 f[8];
+        int wo[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             int dT[3], w[3];
@@ -727,16 +746,29 @@ __device__ __forceinline__ void grid_block(uint32_t B, const SceneInfo* __restri
                 dT[ax] = lc[ax] < 3 ? bit - 1 : bit;
                 w[ax] = lc[ax] + 1 - TT * dT[ax];
             }
-            uint32_t ne;
-            const uint32_t sl = slot_of(bc[0] + dT[0], bc[1] + dT[1], bc[2] + dT[2], ne);
-            for (uint32_t c = 0; c < ne; ++c) {
-                const float4 u = wbuf[(size_t)(sl + c) * PV + (w[2] * PN + w[1]) * PN + w[0]];
-                q.x += u.x;
-                q.y += u.y;
-                q.z += u.z;
-                q.w += u.w;
-            }
+            f[e] = slot_of(bc[0] + dT[0], bc[1] + dT[1], bc[2] + dT[2]);
+            wo[e] = (w[2] * PN + w[1]) * PN + w[0];
         }
+        float4 u[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            u[e] = f[e].y ? wbuf[(size_t)f[e].x * PV + wo[e]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            q.x += u[e].x;
+            q.y += u[e].y;
+            q.z += u[e].z;
+            q.w += u[e].w;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            for (uint32_t c = 1; c < f[e].y; ++c) {
+                const float4 t = wbuf[(size_t)(f[e].x + c) * PV + wo[e]];
+                q.x += t.x;
+                q.y += t.y;
+                q.z += t.z;
+                q.w += t.w;
+            }
     }
     float4* __restrict__ mp = mp_all + sc.goff;
     float4* __restrict__ gv = gv_all + sc.goff;
@@ -830,51 +862,102 @@ __global__ void k_init_keys(int n, const float4* __restrict__ S, const uint32_t*
 }
 
 // Scan of the tile counts in two kernels over blocks of SCAN_B slots:
-// block sums and active-list entry counts, then per-block exclusive scan with
-// the preceding blocks' totals; writes starts, the active list (one entry per
-// chunk of at most `chunk` particles of a tile), the tile -> entries map, and
-// clears the counts.
+// block sums, then per-block exclusive scans with the preceding blocks'
+// totals; writes starts, the active list (one entry per chunk of at most
+// `chunk` particles of a tile), the tile -> entries map, and clears the counts.
+//
+// Order of the active list: longest work first.  A tile's entries stay
+// together; tiles are grouped by the size class of their first entry
+// (quarters of `chunk`), the largest class first, tile order within a class.
+// Dealt round-robin to workgroups that start in index order (ItemStride), the
+// long items start first and the short ones fill the end of the launch.
 constexpr int SCAN_T = 256;
 constexpr int SCAN_B = 1024;
+constexpr int NCLS = 4;            // size classes of work items
+constexpr int BSUM_W = 1 + NCLS;   // words per scan block in bsum: particles, entries of each class
 
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < SCAN_T; off <<= 1) {
-        uint32_t t = tid >= off ? sh[tid - off] : 0u;
-        __syncthreads();
-        sh[tid] += t;
-        __syncthreads();
+// size class of a tile with v > 0 particles: its first entry holds min(v, chunk)
+__device__ __forceinline__ uint32_t item_class(uint32_t v, uint32_t chunk) {
+    return min((4u * min(v, chunk) - 1u) / chunk, (uint32_t)NCLS - 1u);
+}
+
+// The entries per class of a thread's tiles, packed two classes to a 64-bit
+// word (class 3 | class 2, class 1 | class 0) so that one 64-bit scan sums both
+// halves; no half overflows (fewer than 2^32 entries).
+__device__ __forceinline__ void add_class(uint64_t* e, uint32_t cls, uint32_t ne) {
+    const uint64_t d = (uint64_t)ne << (32 * (cls & 1u));
+    if (cls >= 2u) e[0] += d;
+    else e[1] += d;
+}
+__device__ __forceinline__ uint32_t class_part(const uint64_t* e, uint32_t cls) {
+    return (uint32_t)(e[cls >= 2u ? 0 : 1] >> (32 * (cls & 1u)));
+}
+
+// Exclusive scans over the workgroup (SCAN_T threads) of K sums at once: wave
+// scans by shuffles, then the wave totals through LDS (sh: K * SCAN_T/64).
+// v[k] becomes the exclusive prefix, tot[k] the workgroup total.
+template <int K>
+__device__ __forceinline__ void block_scan(uint64_t (&v)[K], uint64_t (&tot)[K], uint64_t* sh) {
+    constexpr int NWV = SCAN_T / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t inc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        inc[k] = v[k];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t t = __shfl_up(inc[k], off);
+            if (lane >= off) inc[k] += t;
+        }
+        if (lane == 63) sh[k * NWV + wave] = inc[k];
     }
-    const uint32_t incl = sh[tid];
-    *total = sh[SCAN_T - 1];
     __syncthreads();
-    return incl - v;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uint64_t pre = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < NWV; ++w) {
+            const uint64_t t = sh[k * NWV + w];
+            pre += w < wave ? t : 0;
+            all += t;
+        }
+        v[k] = inc[k] - v[k] + pre;
+        tot[k] = all;
+    }
+    __syncthreads();
+}
+
+// particles and per-class entries of this thread's 4 tile slots of block blk
+__device__ __forceinline__ void thread_tiles(int blk, const uint32_t* __restrict__ count, int slots, uint32_t chunk,
+                                             uint32_t* v, uint64_t* e, uint64_t& s) {
+    const int base = blk * SCAN_B + threadIdx.x * 4;
+    s = 0;
+    e[0] = e[1] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < slots ? count[base + k] : 0u;
+        s += v[k];
+        if (v[k]) add_class(e, item_class(v[k], chunk), (v[k] + chunk - 1) / chunk);
+    }
 }
 
 __device__ __forceinline__ void scan_a_block(int blk, const uint32_t* __restrict__ count, int slots, uint32_t chunk,
-                                             uint32_t* __restrict__ bsum, uint32_t* sh) {
-    const int base = blk * SCAN_B + threadIdx.x * 4;
-    uint32_t s = 0, act = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = base + k < slots ? count[base + k] : 0u;
-        s += v;
-        act += (v + chunk - 1) / chunk;
-    }
-    uint32_t ts, ta;
-    block_excl_scan(s, sh, &ts);
-    block_excl_scan(act, sh, &ta);
+                                             uint32_t* __restrict__ bsum, uint64_t* sh) {
+    uint32_t v[4];
+    uint64_t x[3], tot[3];
+    thread_tiles(blk, count, slots, chunk, v, x + 1, x[0]);
+    block_scan<3>(x, tot, sh);
     if (threadIdx.x == 0) {
-        bsum[2 * blk] = ts;
-        bsum[2 * blk + 1] = ta;
+        uint32_t* b = bsum + BSUM_W * blk;
+        b[0] = (uint32_t)tot[0];
+#pragma unroll
+        for (uint32_t c = 0; c < NCLS; ++c) b[1 + c] = class_part(tot + 1, c);
     }
 }
 
 __global__ void __launch_bounds__(SCAN_T) k_scan_a(const uint32_t* __restrict__ count, int slots, uint32_t chunk,
                                                    uint32_t* __restrict__ bsum) {
-    __shared__ uint32_t sh[SCAN_T];
+    __shared__ uint64_t sh[3 * SCAN_T / 64];
     scan_a_block(blockIdx.x, count, slots, chunk, bsum, sh);
 }
 
@@ -895,7 +978,7 @@ __global__ void __launch_bounds__(256) k_grid_and_count(const SceneInfo* __restr
                                                         uint32_t* __restrict__ bsum,
                                                         const uint32_t* __restrict__ node_bc) {
     static_assert(SCAN_T == 256, "both parts run 256 threads");
-    __shared__ uint32_t sh[SCAN_T];
+    __shared__ uint64_t sh[3 * SCAN_T / 64];
     if ((int)blockIdx.x < ngrid)
         grid_list<TAPE>((uint32_t)ngrid, scenes, tscene, tslot, bflag, blist, nblist, wbuf, mp_all, gv_all, flags8,
                         node_bc);
@@ -909,53 +992,72 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_b(uint32_t* __restrict__ count,
                                                    uint2* __restrict__ tslot, const uint32_t* __restrict__ tscene,
                                                    uint32_t* __restrict__ nblist, uint32_t max_entries,
                                                    uint32_t chunk, uint32_t* __restrict__ err) {
-    __shared__ uint32_t sh[SCAN_T];
+    __shared__ uint64_t sh[5 * SCAN_T / 64];
     // the grid update of the last binning has read the block list: empty it
     if (blockIdx.x == 0 && threadIdx.x == 0) *nblist = 0u;
-    // totals of the preceding blocks
-    uint32_t ps = 0, pa = 0;
-    for (int b = threadIdx.x; b < (int)blockIdx.x; b += SCAN_T) {
-        ps += bsum[2 * b];
-        pa += bsum[2 * b + 1];
-    }
-    uint32_t pre_s, pre_a;
-    block_excl_scan(ps, sh, &pre_s);
-    block_excl_scan(pa, sh, &pre_a);
-    const int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
-    uint32_t v[4], s = 0, act = 0;
+    // particles and class entries of the preceding blocks; class entries of all blocks
+    uint64_t g[5] = {0, 0, 0, 0, 0}, gt[5];
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += SCAN_T) {
+        const uint32_t* s = bsum + BSUM_W * b;
+        uint64_t e[2] = {0, 0};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        v[k] = base + k < slots ? count[base + k] : 0u;
-        s += v[k];
-        act += (v[k] + chunk - 1) / chunk;
+        for (uint32_t c = 0; c < NCLS; ++c) add_class(e, c, s[1 + c]);
+        if (b < (int)blockIdx.x) {
+            g[0] += s[0];
+            g[1] += e[0];
+            g[2] += e[1];
+        }
+        g[3] += e[0];
+        g[4] += e[1];
     }
-    uint32_t ts, ta;
-    uint32_t run = block_excl_scan(s, sh, &ts) + pre_s;
-    uint32_t pos = block_excl_scan(act, sh, &ta) + pre_a;
+    block_scan<5>(g, gt, sh);
+    // first list position of each class: the larger classes come first
+    uint32_t cbase[NCLS];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int c = NCLS - 1; c >= 0; --c) {
+        cbase[c] = acc;
+        acc += class_part(gt + 3, (uint32_t)c);
+    }
+    uint32_t v[4];
+    uint64_t x[3], xt[3];
+    thread_tiles(blockIdx.x, count, slots, chunk, v, x + 1, x[0]);
+    block_scan<3>(x, xt, sh);
+    uint32_t run = (uint32_t)(x[0] + gt[0]);
+    uint32_t pos[NCLS];
+#pragma unroll
+    for (uint32_t c = 0; c < NCLS; ++c) pos[c] = cbase[c] + class_part(gt + 1, c) + class_part(x + 1, c);
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (base + k < slots) {
             start[base + k] = run;
             const uint32_t ne = (v[k] + chunk - 1) / chunk;
-            tslot[base + k] = make_uint2(v[k] ? pos : 0xFFFFFFFFu, ne);
+            uint32_t p = 0xFFFFFFFFu;
             if (v[k]) {
+                const uint32_t cls = item_class(v[k], chunk);
+                // pick pos[cls] without indexing the array by a runtime value
+                p = cls == 0 ? pos[0] : cls == 1 ? pos[1] : cls == 2 ? pos[2] : pos[3];
                 const uint32_t sc = tscene[base + k];
-                for (uint32_t c = 0; c < ne; ++c, ++pos) {
-                    if (pos < max_entries)
-                        alist[pos] = make_uint4((uint32_t)(base + k), run + c * chunk,
-                                                min(chunk, v[k] - c * chunk), (uint32_t)sc);
+                for (uint32_t c = 0; c < ne; ++c) {
+                    if (p + c < max_entries)
+                        alist[p + c] = make_uint4((uint32_t)(base + k), run + c * chunk,
+                                                  min(chunk, v[k] - c * chunk), (uint32_t)sc);
                     else
                         atomicOr(err, ERR_ALIST);
                 }
+#pragma unroll
+                for (uint32_t c = 0; c < NCLS; ++c) pos[c] += c == cls ? ne : 0u;
             }
+            tslot[base + k] = make_uint2(p, ne);
             run += v[k];
             count[base + k] = 0u;
         }
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         // an overflowing list is cut at its capacity (and reported), so that no
         // kernel reads past it
-        *nactive = min(pre_a + ta, max_entries);
+        *nactive = min(acc, max_entries);
     }
 }
 

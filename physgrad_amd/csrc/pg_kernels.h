@@ -177,7 +177,7 @@ struct TiledWork {
     uint32_t* rank;        // [cap] rank within tile
     uint32_t* perm;        // [cap] sorted position -> storage slot
     uint32_t* flags;       // [total_blocks] 8^3 block flags (unused by the forward step)
-    uint32_t* bsum;        // [2 * ceil(tile_slots/1024)] scan block totals
+    uint32_t* bsum;        // [5 * ceil(tile_slots/1024)] scan block totals (particles, entries per size class)
     uint32_t* tslot;       // [tile_slots] uint2 (first active-list entry of the tile or ~0, entry count)
     uint32_t* tscene;      // [tile_slots] scene of each tile slot
     uint32_t* bscene;      // [total_blocks] scene of each 8^3 grid block

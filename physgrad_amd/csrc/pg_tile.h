@@ -59,28 +59,14 @@ __device__ __forceinline__ void tile_origin(uint32_t tile, int* t0) {
     t0[2] = (int)compact3(tile >> 2) * TT;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; each XCD takes a
-// contiguous chunk of the (Morton-ordered) tile list so that tiles sharing
-// window nodes share an L2.  Workgroup b runs on XCD b mod 8, which holds
-// ceil((gridDim - xcd) / 8) workgroups; they stride over the XCD's chunk by
-// that count, so every entry is taken for any grid size (with fewer than 8
-// workgroups, all of them stride over the whole list instead).
-struct XcdChunk {
+// Work items of the active list, dealt round-robin: workgroup b takes items
+// b, b + gridDim, ...  The list holds the longest items first (the binning
+// orders it by size class), and the hardware starts workgroups in index order
+// as slots free up and spreads consecutive ones over the 8 XCDs, so long
+// items start first on every XCD and short ones fill the end of the launch.
+struct ItemStride {
     uint32_t first, end, stride;
-    __device__ __forceinline__ XcdChunk(uint32_t nact) {
-        if (gridDim.x < 8u) {
-            first = blockIdx.x;
-            end = nact;
-            stride = gridDim.x;
-            return;
-        }
-        const uint32_t xcd = blockIdx.x % 8u;
-        stride = (gridDim.x - xcd + 7u) / 8u;
-        const uint32_t lane = blockIdx.x / 8u;
-        const uint32_t c0 = (uint32_t)(((uint64_t)nact * xcd) / 8u);
-        end = (uint32_t)(((uint64_t)nact * (xcd + 1)) / 8u);
-        first = c0 + lane;
-    }
+    __device__ __forceinline__ explicit ItemStride(uint32_t nact) : first(blockIdx.x), end(nact), stride(gridDim.x) {}
 };
 
 // number of set bits of a wave mask below this lane
