@@ -78,8 +78,47 @@ struct DifferentiableMPMConfig {
     T poisson_ratio = T{0.3};
     T particle_volume = T{0};                // 0: the particles' volume entries (must agree)
     int boundary_nodes = 3;                  // sticky wall thickness in nodes
-    int checkpoint_every = 32;               // steps between checkpoints of a recorded rollout
+    int checkpoint_every = 32;               // steps between checkpoints (UNIFORM)
 };
+
+// Steps between checkpoints of a recorded rollout of `steps` steps, as the
+// config's strategy asks (the reverse pass recomputes every segment but the
+// last from its checkpoint, and holds one segment's tape):
+//   NONE, or use_checkpointing off: one segment (no recompute, the whole tape);
+//   UNIFORM: checkpoint_every;
+//   ADAPTIVE: the longest segment whose device memory (checkpoints plus one
+//     segment's tape, estimated from ckpt_bytes and step_bytes) fits
+//     memory_budget_gb, with at most max_checkpoints checkpoints when that also
+//     fits; the least memory when nothing fits;
+//   LOGARITHMIC: ceil(sqrt(steps)), the single-level spacing with the least
+//     memory (the reference declares the strategy without implementing it; a
+//     multi-level schedule is not implemented here either).
+// ckpt_bytes: device bytes of one checkpoint; step_bytes: of one tape step.
+template <typename T>
+int checkpointInterval(const DifferentiableMPMConfig<T>& cfg, int steps, double ckpt_bytes, double step_bytes) {
+    if (steps <= 1) return 1;
+    if (!cfg.use_checkpointing || cfg.checkpoint_strategy == CheckpointStrategy::NONE) return steps;
+    switch (cfg.checkpoint_strategy) {
+        case CheckpointStrategy::UNIFORM:
+            return std::min(steps, std::max(1, cfg.checkpoint_every));
+        case CheckpointStrategy::LOGARITHMIC:
+            return std::max(1, static_cast<int>(std::ceil(std::sqrt(static_cast<double>(steps)))));
+        default: {
+            auto bytes = [&](int K) { return ((steps + K - 1) / K) * ckpt_bytes + (K + 1.0) * step_bytes; };
+            const double budget = static_cast<double>(cfg.memory_budget_gb) * 1e9;
+            const size_t maxc = std::max<size_t>(1, cfg.max_checkpoints);
+            int best = -1, least = 1;
+            for (int K = steps; K >= 1; --K) {
+                if (bytes(K) < bytes(least)) least = K;
+                if (best < 0 && bytes(K) <= budget && static_cast<size_t>((steps + K - 1) / K) <= maxc) best = K;
+            }
+            if (best < 0)   // the checkpoint cap cannot be met within the budget: the budget rules
+                for (int K = steps; K >= 1 && best < 0; --K)
+                    if (bytes(K) <= budget) best = K;
+            return best > 0 ? best : least;
+        }
+    }
+}
 
 template <typename T>
 class ParticleGradients {
@@ -371,15 +410,21 @@ public:
             const State final_state = captureState();
             restoreState(initial_);
             push();
-            const int K = std::max(1, config_.checkpoint_every);
+            // checkpoint: state, caller index, E and mass per particle; tape step:
+            // the same plus the binning's permutation and source maps, and the
+            // step's grid (m, p) and (v, m)
+            const double ckpt_bytes = 108.0 * n;
+            const double nodes = static_cast<double>(grid_->dims.x) * grid_->dims.y * grid_->dims.z;
+            const double step_bytes = 116.0 * n + 32.0 * nodes;
+            const int K = checkpointInterval(config_, static_cast<int>(recorded_steps_), ckpt_bytes, step_bytes);
             check(pg_mpm_forward_checkpointed(ctx_, static_cast<int>(recorded_steps_), K, nullptr));
             check(pg_mpm_backward(ctx_, ax.data(), av.data(), nullptr, aF.data(), gx.data(), gv.data(), gC.data(),
                                   gF.data(), gE.data(), nullptr));
             // keep the recorded end state (the replay may differ from it in the
             // last bits: P2G sums are accumulated in a different order)
             restoreState(final_state);
-            gradient_tape_.addDeviceBytes(n * sizeof(T) * 24 *
-                                          ((recorded_steps_ + K - 1) / K + static_cast<size_t>(K) + 1));
+            gradient_tape_.addDeviceBytes(static_cast<size_t>(((recorded_steps_ + K - 1) / K) * ckpt_bytes +
+                                                              (K + 1.0) * step_bytes));
         }
         for (size_t p = 0; p < n; ++p) {
             particle_gradients_->setPositionGradient(p, {gx[3 * p], gx[3 * p + 1], gx[3 * p + 2]});

@@ -81,6 +81,7 @@ static int diff(const char* in, const char* out) {
     cfg.youngs_modulus = fl[1];
     cfg.poisson_ratio = fl[2];
     cfg.particle_volume = fl[3];
+    cfg.checkpoint_strategy = CheckpointStrategy::UNIFORM;   // recompute every K steps
     cfg.checkpoint_every = K;
     DifferentiableMPMSystem<float> sys(cfg);
     sys.initializeSystem(int3{nx, nx, nx}, {1.0f / nx, 1.0f / nx, 1.0f / nx}, {0.f, 0.f, 0.f}, n);

@@ -375,3 +375,49 @@ def test_sparse_dropin_matrix_vs_oracle(gpu, oracle_lib, tmp_path):
     assert rp[8] == rp[7]
     want = O.csr_spmv(want_rp, cols, vals, x)
     np.testing.assert_array_equal(y.view(np.uint32), want.view(np.uint32))
+
+
+# ---- checkpoint spacing of the differentiable drop-in (host only)
+
+def test_checkpoint_interval_strategies(tmp_path):
+    """checkpointInterval: NONE / no checkpointing keep one segment, UNIFORM takes
+    checkpoint_every, LOGARITHMIC ceil(sqrt(steps)), ADAPTIVE the longest segment
+    whose checkpoints + one segment's tape fit the budget (checked against a brute
+    force over every spacing), the least memory when nothing fits."""
+    import math
+    exe = os.path.join(str(tmp_path), "checkpoint_interval")
+    src = os.path.join(ROOT, "tests", "cpp", "checkpoint_interval.cpp")
+    lib_dir = os.path.join(ROOT, "physgrad_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", f"-I{os.path.join(ROOT, 'include')}", src, f"-L{lib_dir}",
+                    "-lphysgrad_mpm", f"-Wl,-rpath,{lib_dir}", "-o", exe], check=True, capture_output=True, text=True)
+    NONE, UNIFORM, ADAPTIVE, LOG = 0, 1, 2, 3
+    ck, st = 108.0 * 2**20, 116.0 * 2**20 + 32.0 * 128**3   # 1M particles on 128^3
+
+    def mem(steps, K):
+        return math.ceil(steps / K) * ck + (K + 1) * st
+
+    def adaptive(steps, budget, maxc):
+        fits = [K for K in range(steps, 0, -1) if mem(steps, K) <= budget * 1e9]
+        capped = [K for K in fits if math.ceil(steps / K) <= maxc]
+        if capped:
+            return capped[0]
+        if fits:
+            return fits[0]
+        return min(range(1, steps + 1), key=lambda K: (mem(steps, K), -K))
+
+    cases = [(NONE, 256, 4.0, 100, 1, 32), (UNIFORM, 256, 4.0, 100, 0, 32), (UNIFORM, 256, 4.0, 100, 1, 32),
+             (UNIFORM, 10, 4.0, 100, 1, 32), (LOG, 256, 4.0, 100, 1, 32), (LOG, 1000, 4.0, 100, 1, 32),
+             (ADAPTIVE, 256, 4.0, 100, 1, 32), (ADAPTIVE, 256, 60.0, 100, 1, 32), (ADAPTIVE, 512, 2.0, 100, 1, 32),
+             (ADAPTIVE, 512, 2.0, 4, 1, 32), (ADAPTIVE, 512, 0.01, 100, 1, 32), (ADAPTIVE, 1, 4.0, 100, 1, 32), (ADAPTIVE, 256, 10.0, 100, 1, 32)]
+    args = []
+    for c in cases:
+        args += [str(a) for a in c] + [repr(ck), repr(st)]
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    got = [int(t) for t in r.stdout.split()]
+    want = [256, 256, 32, 10, 16, 32, adaptive(256, 4.0, 100), adaptive(256, 60.0, 100), adaptive(512, 2.0, 100),
+            adaptive(512, 2.0, 4), adaptive(512, 0.01, 100), 1, adaptive(256, 10.0, 100)]
+    assert got == want
+    assert want[7] == 256                         # a generous budget keeps one segment: no recompute
+    assert mem(256, want[6]) == min(mem(256, K) for K in range(1, 257)) > 4e9   # nothing fits 4 GB
+    assert 1 < want[12] < 256 and mem(256, want[12]) <= 10e9 < mem(256, want[12] + 1)
