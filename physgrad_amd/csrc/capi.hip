@@ -1416,21 +1416,27 @@ int pg_mpm_autotune(pg_mpm_ctx* c, int trial_steps, int use_cache, void* stream)
             c->tw.nblocks_launch = v;
             c->tw.chunk = ch;
             c->tw.block = bt;
+            // steady-state steps only: an untimed call of trial_steps steps that hands
+            // its pending work over, then a timed call that takes it and hands over
+            // in turn (the per-call start, binning from scratch, is not timed), then
+            // the pending work completed before the state is restored
             float ms_sum = 0.f;
-            for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+            if (e == hipSuccess) {
                 save(true);
                 float* E = c->have_E ? c->E : nullptr;
                 float* E2 = c->have_E ? c->E2 : nullptr;
                 float* m = c->have_mass ? c->mass : nullptr;
                 float* m2 = c->have_mass ? c->mass2 : nullptr;
-                hipEventRecord(a, st);
                 e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
-                                    trial_steps, st);
+                                    trial_steps, st, false, true);
+                hipEventRecord(a, st);
+                if (e == hipSuccess)
+                    e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp,
+                                        c->gv, trial_steps, st, true, true);
                 hipEventRecord(b, st);
+                if (e == hipSuccess) e = pg::tiled_settle(c->tw, c->n, c->mp, c->gv, st);
                 hipEventSynchronize(b);
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, a, b);
-                if (rep == 1) ms_sum += ms;  // the first repetition warms up
+                hipEventElapsedTime(&ms_sum, a, b);
             }
             if (e != hipSuccess) break;
             if (ms_sum < best_ms) {

@@ -643,8 +643,11 @@ __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per
             sp[l] = u;
         }
         __syncthreads();
+        // each thread loads its next particle's slot while it works on the current one
+        uint32_t sn = (uint32_t)tid < cnt ? a.perm[beg + tid] : 0u;
         for (uint32_t r = tid; r < cnt; r += TB) {
-            const uint32_t s = a.perm[beg + r];
+            const uint32_t s = sn;
+            if (r + TB < cnt) sn = a.perm[beg + r + TB];
             float x[3], v[3], C[9];
             load_xvC(a.S, a.cap, s, x, v, C);
             auto getF = [&](float* F) { load_F(a.S, a.cap, s, F); };
