@@ -4,10 +4,10 @@ Workload (two_boxes, 128^3 grid, dx 1/128, 1,048,576 particles, neo-Hookean
 E 5e4 nu 0.3 rho 1000, dt 5e-5): a step is one P2G -> grid update -> G2P pass
 over all particles with the particle state resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--shape W,I,T]
 
 Prints one JSON line: particle-steps/s of the tiled path (value, launch shape
-autotuned), the roofline of the dominant kernel (the fused G2P+P2G tile
+autotuned, or pinned with --shape workgroups,work-item,threads), the roofline of the dominant kernel (the fused G2P+P2G tile
 kernel, timed with HIP events on its launch stream), forward+backward with
 checkpoint recompute (one 1M scene and the 16-scene batch, K = 16), rollout
 gradients vs closed form and FP64 finite differences, the 64-scene batch, the
@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--no-batch-fb", action="store_true", help="skip the 16-scene forward+backward leg")
     ap.add_argument("--no-coupling", action="store_true", help="skip the SPH-MPM coupling leg")
     ap.add_argument("--coupling-iters", type=int, default=20)
+    ap.add_argument("--shape", type=str, default=None,
+                    help="pin the forward leg's launch shape WORKGROUPS,WORK_ITEM,THREADS instead of autotuning "
+                         "it (reproducible profiler runs)")
     return ap.parse_args()
 
 
@@ -442,9 +445,16 @@ def main():
 
     ctx = MPMContext(scene.params(impl=_lib.IMPL_TILED))
     ctx.set_particles(scene.x, scene.v)
-    # launch shape (tile workgroups x particles per work item) picked by the
-    # autotuner on this particle set; the state is restored after the trials
-    ctx.autotune(trial_steps=10, use_cache=False)
+    # launch shape (tile workgroups x particles per work item x threads) picked
+    # by the autotuner on this particle set (the state is restored after the
+    # trials), or pinned by --shape
+    if args.shape:
+        wg, item, threads = (int(v) for v in args.shape.split(","))
+        ctx.launch = wg
+        ctx.work_item = item
+        ctx.block_threads = threads
+    else:
+        ctx.autotune(trial_steps=10, use_cache=False)
     tuning = ctx.tuning()
     t_tiled, wall = time_steps(ctx, K, W)
     grid = ctx.get_grid()
