@@ -449,6 +449,11 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
     if (n3[0] < g.nx && n3[1] < g.ny && n3[2] < g.nz) {
         const size_t id = node_id(g, n3[0], n3[1], n3[2]);
         float4 av = agv[id];
+        // the entry ranges of the 8 tiles whose windows may cover the node, then
+        // their first windows, as independent loads (two round trips); the
+        // further windows of a tile split into several work items after that
+        uint2 ts[8];
+        int wo[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             int T[3], wl[3];
@@ -459,17 +464,28 @@ __global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ 
                 wl[ax] = n3[ax] - TT * T[ax];
                 ok = ok && T[ax] >= 0 && T[ax] < nt[ax] && wl[ax] < WN;
             }
-            if (ok) {
-                // every work item (chunk) of tile T
-                const uint2 ts = reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(T[0], T[1], T[2])];
-                for (uint32_t c = 0; c < ts.y; ++c) {
-                    const float4 u = wbuf[(size_t)(ts.x + c) * PV + (wl[2] * WN + wl[1]) * WN + wl[0]];
-                    av.x += u.x;
-                    av.y += u.y;
-                    av.z += u.z;
-                }
-            }
+            ts[e] = ok ? reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(T[0], T[1], T[2])]
+                       : make_uint2(0u, 0u);
+            wo[e] = (wl[2] * WN + wl[1]) * WN + wl[0];
         }
+        float4 u[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            u[e] = ts[e].y ? wbuf[(size_t)ts[e].x * PV + wo[e]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            av.x += u[e].x;
+            av.y += u[e].y;
+            av.z += u[e].z;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            for (uint32_t c = 1; c < ts[e].y; ++c) {
+                const float4 t = wbuf[(size_t)(ts[e].x + c) * PV + wo[e]];
+                av.x += t.x;
+                av.y += t.y;
+                av.z += t.z;
+            }
         const float4 q = mp[id];
         apam[id] = node_velocity_vjp(g, n3[0], n3[1], n3[2], node_bc ? node_bc[sc.goff + id] : 0u, q, av);
         agv[id] = make_float4(0.f, 0.f, 0.f, 0.f);
