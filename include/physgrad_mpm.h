@@ -146,7 +146,9 @@ int pg_mpm_backward(pg_mpm_ctx* ctx, const float* ax, const float* av, const flo
  * pg_mpm_set_profiling resets the totals.  Kernel ids: 0 tiled P2G, 1 tiled
  * fused G2P+P2G, 2 tiled G2P, 3 grid update, 4 binning, 5 plain P2G, 6 plain
  * G2P, 7 grid clear, 8 ref-compat step, 9 reverse G2P, 10 reverse grid
- * update, 11 reverse P2G. */
+ * update, 11 reverse P2G, 12 reverse P2G of a step fused with the reverse G2P
+ * of the step before it (the tiled backward runs 9 and 11 once per segment
+ * and 12 for the steps between). */
 int pg_mpm_set_profiling(pg_mpm_ctx* ctx, int on);
 int pg_mpm_kernel_time(pg_mpm_ctx* ctx, int kernel, double* total_ms, long long* launches);
 

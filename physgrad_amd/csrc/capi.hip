@@ -70,6 +70,10 @@ struct pg_mpm_ctx {
     float *aE = nullptr, *ux = nullptr, *uv = nullptr, *uC = nullptr, *uF = nullptr;
     float *aEs = nullptr, *aEp = nullptr;   // dL/dE by storage slot through a segment, and its carry
     uint32_t* adst = nullptr;               // storage slot -> output position of a reverse step
+    // second set of per-step reverse buffers (the tiled reverse alternates between the two)
+    float4* part2 = nullptr;
+    float* aEp2 = nullptr;
+    uint32_t* adst2 = nullptr;
     uint32_t* check = nullptr;              // [4] results of the state validation kernel
     bool validate = false;                  // pg_mpm_step validates the state after stepping
     // per-particle materials (pg_mpm_set_materials) and per-node boundary flags (pg_mpm_set_boundary)
@@ -997,6 +1001,8 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
         (rc = valloc(c, A, &c->agv, G, true)) || (rc = valloc(c, A, &c->apam, G, true)) ||
         (rc = valloc(c, A, &c->aE, c->cap, false)) || (rc = valloc(c, A, &c->aEs, c->cap, false)) ||
         (rc = valloc(c, A, &c->aEp, c->cap, false)) || (rc = valloc(c, A, &c->adst, c->cap, false)) ||
+        (!plain && ((rc = valloc(c, A, &c->part2, 3 * c->cap, false)) ||
+                    (rc = valloc(c, A, &c->aEp2, c->cap, false)) || (rc = valloc(c, A, &c->adst2, c->cap, false)))) ||
         (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uv, 3 * c->cap, false)) || (rc = valloc(c, A, &c->uC, 9 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uF, 9 * c->cap, false)))
@@ -1008,7 +1014,9 @@ fail:
     free_all(c->tape_allocs);
     c->abar = c->part = c->agv = c->apam = nullptr;
     c->aE = c->aEs = c->aEp = c->ux = c->uv = c->uC = c->uF = nullptr;
-    c->adst = nullptr;
+    c->adst = c->adst2 = nullptr;
+    c->part2 = nullptr;
+    c->aEp2 = nullptr;
     return rc;
 }
 
@@ -1163,9 +1171,12 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         c->tape_filled = 0;   // the reverse pass clears the grids it reads
         if (e == hipSuccess)
             e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, aE, aEs, st);
-        if (e == hipSuccess)
-            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, c->part, c->agv, c->apam, aEs, c->aEp, c->adst,
-                                    st);
+        if (e == hipSuccess) {
+            float4* const part[2] = {c->part, c->part2};
+            float* const aEp[2] = {c->aEp, c->aEp2};
+            uint32_t* const dst[2] = {c->adst, c->adst2};
+            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, part, c->agv, c->apam, aEs, aEp, dst, st);
+        }
         if (e == hipSuccess)
             e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);
     }

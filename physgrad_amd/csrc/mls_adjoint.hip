@@ -29,6 +29,13 @@
 // particle's slot in step n); A3 writes the adjoint of the state before step n
 // at its slot of step n, which is what A1 of step n-1 reads by its output
 // position.  A3 reads A1's partials at dst[slot].
+//
+// A segment of L steps runs A1(L-1), A2(L-1), then for n = L-1 .. 1 the fused
+// kernel A3(n) + A1(n-1) (k_adj_g2p<true>: per particle, in binning n-1's
+// groups, the adjoint of the state between the two steps stays in registers)
+// and A2(n-1), and last A3(0).  The partials, dst and the dL/dE carry
+// alternate between two buffer sets, since the fused kernel reads step n's
+// while it writes step n-1's.
 #include <algorithm>
 
 #include "pg_tile.h"
@@ -57,6 +64,21 @@ struct AdjArgs {
     const SceneInfo* scenes;
     size_t cap;
     MatTable mt;            // per-particle materials (recs NULL: the scene's)
+};
+
+// The step after the one A1 reverses, for the fused kernel (A3 of step n+1
+// followed by A1 of step n on each particle): its slots are step n's output
+// positions, so the particle A1 of step n takes at output position o is the
+// one in slot o of step n+1.
+struct AdjNext {
+    const float4* S;        // tape state of step n+1
+    const uint32_t* pid;
+    const float* E;
+    const float* mass;
+    const float4* apam;     // (am, ap) per node of step n+1
+    const float4* part;     // partials of step n+1 by its output position
+    const uint32_t* dst;    // slot of step n+1 -> its output position
+    const float* aEp;       // dL/dE carry of step n+1 by its output position
 };
 
 PG_DEV void load_adj(const float4* __restrict__ A, size_t cap, size_t i, float* ax, float* aF, float* av, float* aC) {
@@ -251,250 +273,6 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
     for (int c = 0; c < 3; ++c) axo[c] = ax[c] + af[c] * g.inv_dx;
 }
 
-// A1: work items of binning n, in the forward tile kernel's groups.  The
-// forward wrote the in-tile particles of a pass group by group: group grp of
-// G = max(ceil(tot / 64), ceil(maxcount / 2)) holds ceil((tot - grp) / G)
-// particles from output offset grp (tot div G) + min(grp, tot mod G), and the
-// particles of a pass whose stencil base lies outside the tile (clamped keys)
-// after them.  Counting the pass by stencil-base cell gives tot and maxcount
-// again, so the reverse runs the same groups over contiguous output offsets.
-// A group holds at most two particles of a cell, in adjacent lanes; the upper
-// one scatters into the wave's second LDS window, so within a window the lanes
-// touch distinct nodes and add with plain read-modify-writes.  The windows are
-// summed into the tile's buffer; the reverse grid update gathers them.  An
-// outside particle scatters with global atomics into agv.
-__global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a) {
-    constexpr int PT = APASS / TB;
-    static_assert(APASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
-    __shared__ float4 sv[WV];
-    __shared__ float4 pw[NW][2][WV];          // per-wave windows: lower and upper particle of a cell
-    __shared__ uint8_t lcell[APASS];          // stencil-base cell by output offset (64: outside)
-    __shared__ uint32_t lcnt[65];
-    __shared__ uint32_t s_max, s_tot;
-    const ItemStride ch(*a.nactive);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
-        const uint4 te = a.alist[ti];
-        const SceneInfo sc = a.scenes[te.w];
-        const MlsGeom g = sc.g;
-        const float4* __restrict__ gvs = a.gv + sc.goff;
-        float4* __restrict__ agvs = a.agv + sc.goff;
-        const uint32_t beg = te.y, cnt = te.z;
-        int t0[3];
-        tile_origin(te.x - sc.toff, t0);
-        static_assert(WV <= TB, "one window node per thread");
-        if (tid < WV) {
-            const int gi = t0[0] + tid % WN, gj = t0[1] + (tid / WN) % WN, gk = t0[2] + tid / (WN * WN);
-            float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
-            sv[tid] = u;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                pw[w][0][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
-                pw[w][1][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-        for (uint32_t pb = 0; pb < cnt; pb += APASS) {
-            const uint32_t pc = min((uint32_t)APASS, cnt - pb);
-            if (tid < 65) lcnt[tid] = 0;
-            __syncthreads();
-            float4 k0[PT];
-#pragma unroll
-            for (int k = 0; k < PT; ++k) {
-                const uint32_t r = tid + k * TB;
-                if (r < pc) k0[k] = a.S[a.src[beg + pb + r]];
-            }
-#pragma unroll
-            for (int k = 0; k < PT; ++k) {
-                const uint32_t r = tid + k * TB;
-                if (r < pc) {
-                    const float xx[3] = {k0[k].x, k0[k].y, k0[k].z};
-                    int l3[3];
-                    bool in = true;
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        l3[d] = stencil_base(grid_coord(xx[d], g.o[d], g.inv_dx)) - t0[d];
-                        in = in && l3[d] >= 0 && l3[d] < TT;
-                    }
-                    const int cell = in ? l3[0] + TT * l3[2] + TT * TT * l3[1] : 64;
-                    atomicAdd(&lcnt[cell], 1u);
-                    lcell[r] = (uint8_t)cell;
-                }
-            }
-            __syncthreads();
-            if (wave == 0) {
-                uint32_t sum = lcnt[lane], mx = lcnt[lane];
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    sum += (uint32_t)__shfl_xor(sum, off);
-                    mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-                }
-                if (lane == 0) {
-                    s_tot = sum;
-                    s_max = mx;
-                }
-            }
-            __syncthreads();
-            const uint32_t maxcnt = s_max, tot = s_tot;
-
-            // r = output offset in the pass; own: a group particle (its base is a cell
-            // of the tile); second: the upper particle of its cell in the group (the
-            // wave's second window)
-            auto particle = [&](uint32_t r, bool own, bool second) {
-                const uint32_t o = beg + pb + r;
-                const uint32_t s = a.src[o];
-                float x[3];
-                float F[9], ax[3], aF[9], av[3], aC[9];
-                {
-                    const float4 q0 = a.S[s], q1 = a.S[a.cap + s], q2 = a.S[2 * a.cap + s];
-                    x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
-                    F[0] = q0.w; F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
-                    F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
-                }
-                load_adj(a.abar, a.cap, o, ax, aF, av, aC);
-                Stencil st;
-                make_stencil(x, g.o, g.inv_dx, st);
-                float axo[3], aFo[9];
-                if (own) {
-                    float4* __restrict__ win = pw[wave][second ? 1 : 0];
-                    const int nb = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
-                    adj_g2p_particle<false>([&](int ix, int iy, int iz) { return sv[nb + (iz * WN + iy) * WN + ix]; },
-                                     [&](int ix, int iy, int iz, float4 c) {
-                                         float4* t = win + nb + (iz * WN + iy) * WN + ix;
-                                         float4 u = *t;
-                                         u.x += c.x;
-                                         u.y += c.y;
-                                         u.z += c.z;
-                                         *t = u;
-                                         lds_order();
-                                     },
-                                     g, st, F, ax, aF, av, aC, axo, aFo);
-                } else {
-                    adj_g2p_particle<true>(
-                        [&](int ix, int iy, int iz) {
-                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
-                            return in_grid(g, gi, gj, gk) ? gvs[node_id(g, gi, gj, gk)]
-                                                          : make_float4(0.f, 0.f, 0.f, 0.f);
-                        },
-                        [&](int ix, int iy, int iz, float4 c) {
-                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
-                            if (!in_grid(g, gi, gj, gk)) return;
-                            float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
-                            atomic_add_f32(cp + 0, c.x);
-                            atomic_add_f32(cp + 1, c.y);
-                            atomic_add_f32(cp + 2, c.z);
-                        },
-                        g, st, F, ax, aF, av, aC, axo, aFo);
-                }
-                // partials by output position (coalesced); A3 finds them through dst
-                a.part[o] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
-                a.part[a.cap + o] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
-                a.part[2 * a.cap + o] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
-                a.dst[s] = o;
-                if (a.aEs) a.aEp[o] = a.aEs[o];
-            };
-            const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
-            if (G > 0) {
-                const uint32_t gq = tot / G, gr = tot % G;
-                for (uint32_t grp = wave; grp < G; grp += NW) {
-                    const uint32_t r = grp * gq + min(grp, gr) + (uint32_t)lane;
-                    if ((uint32_t)lane < gq + (grp < gr ? 1u : 0u))
-                        particle(r, true, lane > 0 && lcell[r - 1] == lcell[r]);
-                }
-            }
-            for (uint32_t t = tid; t < pc - tot; t += TB) particle(tot + t, false, false);
-            __syncthreads();
-        }
-        // the tile's velocity-adjoint window goes to its buffer (one per active-list entry)
-        float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
-        if (tid < WV) {
-            float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int w = 0; w < NW; ++w)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    m.x += pw[w][h][tid].x;
-                    m.y += pw[w][h][tid].y;
-                    m.z += pw[w][h][tid].z;
-                }
-            wb[tid] = m;
-        }
-        __syncthreads();
-    }
-}
-
-// A2: reverse of v_i = (p_i/m_i + dt g) * mask_i over the step's 8^3 blocks.
-// The velocity adjoint of node i is the sum of the buffered windows of the
-// tiles of binning n that cover it (the window of tile T spans nodes
-// [4T, 4T + 6)) plus the global accumulator.  Clears the accumulator, the
-// step's grid and flags.
-__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes,
-                                                  const uint32_t* __restrict__ bscene,
-                                                  const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
-                                                  float4* __restrict__ agv_all, float4* __restrict__ apam_all,
-                                                  float4* __restrict__ mp_all, uint32_t* __restrict__ flags,
-                                                  const uint32_t* __restrict__ node_bc) {
-    const int b = blockIdx.x;
-    if (flags[b] == 0u) return;
-    const SceneInfo sc = scenes[bscene[b]];
-    const MlsGeom g = sc.g;
-    float4* __restrict__ agv = agv_all + sc.goff;
-    float4* __restrict__ apam = apam_all + sc.goff;
-    float4* __restrict__ mp = mp_all + sc.goff;
-    const int lb = b - (int)sc.boff;
-    const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
-    const int n3[3] = {bx * GB + (int)(threadIdx.x & 7), by * GB + (int)((threadIdx.x >> 3) & 7),
-                       bz * GB + (int)(threadIdx.x >> 6)};
-    const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
-    if (n3[0] < g.nx && n3[1] < g.ny && n3[2] < g.nz) {
-        const size_t id = node_id(g, n3[0], n3[1], n3[2]);
-        float4 av = agv[id];
-        // the entry ranges of the 8 tiles whose windows may cover the node, then
-        // their first windows, as independent loads (two round trips); the
-        // further windows of a tile split into several work items after that
-        uint2 ts[8];
-        int wo[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            int T[3], wl[3];
-            bool ok = true;
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                T[ax] = n3[ax] / TT - ((e >> ax) & 1);
-                wl[ax] = n3[ax] - TT * T[ax];
-                ok = ok && T[ax] >= 0 && T[ax] < nt[ax] && wl[ax] < WN;
-            }
-            ts[e] = ok ? reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(T[0], T[1], T[2])]
-                       : make_uint2(0u, 0u);
-            wo[e] = (wl[2] * WN + wl[1]) * WN + wl[0];
-        }
-        float4 u[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            u[e] = ts[e].y ? wbuf[(size_t)ts[e].x * PV + wo[e]] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            av.x += u[e].x;
-            av.y += u[e].y;
-            av.z += u[e].z;
-        }
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            for (uint32_t c = 1; c < ts[e].y; ++c) {
-                const float4 t = wbuf[(size_t)(ts[e].x + c) * PV + wo[e]];
-                av.x += t.x;
-                av.y += t.y;
-                av.z += t.z;
-            }
-        const float4 q = mp[id];
-        apam[id] = node_velocity_vjp(g, n3[0], n3[1], n3[2], node_bc ? node_bc[sc.goff + id] : 0u, q, av);
-        agv[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-        mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) flags[b] = 0u;
-}
-
 // A3 for one particle: the reverse of its P2G (stress included) given the
 // grid adjoints (am, ap) of its stencil nodes (load(x, y, z) -> float4, zero
 // outside the grid) and its partial adjoints (ax, aF) from A1 (read by part()
@@ -638,6 +416,298 @@ __device__ __forceinline__ void adj_p2g_particle(Load load, Part part, GetF getF
     getF(F);
     dE = 0.f;
     stress_vjp(pm, F, C, at, aF, aC, dE);
+}
+
+// A1: work items of binning n, in the forward tile kernel's groups.  The
+// forward wrote the in-tile particles of a pass group by group: group grp of
+// G = max(ceil(tot / 64), ceil(maxcount / 2)) holds ceil((tot - grp) / G)
+// particles from output offset grp (tot div G) + min(grp, tot mod G), and the
+// particles of a pass whose stencil base lies outside the tile (clamped keys)
+// after them.  Counting the pass by stencil-base cell gives tot and maxcount
+// again, so the reverse runs the same groups over contiguous output offsets.
+// A group holds at most two particles of a cell, in adjacent lanes; the upper
+// one scatters into the wave's second LDS window, so within a window the lanes
+// touch distinct nodes and add with plain read-modify-writes.  The windows are
+// summed into the tile's buffer; the reverse grid update gathers them.  An
+// outside particle scatters with global atomics into agv.
+//
+// FUSED: each particle first runs A3 of step n+1 (nx) and takes the adjoints
+// of its state after step n from that, in registers, instead of reading them
+// from abar.  The (am, ap) of step n+1 come from an 8^3 window [t0 - 1,
+// t0 + 7) of the tile of binning n, which holds the stencil of every particle
+// that moved at most one cell out of the tile in step n; a particle further
+// out reads them from global memory.
+template <bool FUSED>
+__global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a, AdjNext nx) {
+    constexpr int PT = APASS / TB;
+    static_assert(APASS % TB == 0, "the bulk phases cover the pass with whole thread strides");
+    __shared__ float4 sv[WV];
+    __shared__ float4 sa[FUSED ? PV : 1];     // (am, ap) window of step n+1
+    __shared__ float4 pw[NW][2][WV];          // per-wave windows: lower and upper particle of a cell
+    __shared__ uint8_t lcell[APASS];          // stencil-base cell by output offset (64: outside)
+    __shared__ uint32_t lcnt[65];
+    __shared__ uint32_t s_max, s_tot;
+    const ItemStride ch(*a.nactive);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t ti = ch.first; ti < ch.end; ti += ch.stride) {
+        const uint4 te = a.alist[ti];
+        const SceneInfo sc = a.scenes[te.w];
+        const MlsGeom g = sc.g;
+        const float4* __restrict__ gvs = a.gv + sc.goff;
+        float4* __restrict__ agvs = a.agv + sc.goff;
+        const float4* __restrict__ apams = FUSED ? nx.apam + sc.goff : nullptr;
+        const uint32_t beg = te.y, cnt = te.z;
+        int t0[3];
+        tile_origin(te.x - sc.toff, t0);
+        if constexpr (FUSED) {
+            for (int l = tid; l < PV; l += TB) {
+                const int gi = t0[0] - 1 + l % PN, gj = t0[1] - 1 + (l / PN) % PN, gk = t0[2] - 1 + l / (PN * PN);
+                sa[l] = in_grid(g, gi, gj, gk) ? apams[node_id(g, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        static_assert(WV <= TB, "one window node per thread");
+        if (tid < WV) {
+            const int gi = t0[0] + tid % WN, gj = t0[1] + (tid / WN) % WN, gk = t0[2] + tid / (WN * WN);
+            float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gi < g.nx && gj < g.ny && gk < g.nz) u = gvs[node_id(g, gi, gj, gk)];
+            sv[tid] = u;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                pw[w][0][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+                pw[w][1][tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        for (uint32_t pb = 0; pb < cnt; pb += APASS) {
+            const uint32_t pc = min((uint32_t)APASS, cnt - pb);
+            if (tid < 65) lcnt[tid] = 0;
+            __syncthreads();
+            float4 k0[PT];
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                const uint32_t r = tid + k * TB;
+                if (r < pc) k0[k] = a.S[a.src[beg + pb + r]];
+            }
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                const uint32_t r = tid + k * TB;
+                if (r < pc) {
+                    const float xx[3] = {k0[k].x, k0[k].y, k0[k].z};
+                    int l3[3];
+                    bool in = true;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        l3[d] = stencil_base(grid_coord(xx[d], g.o[d], g.inv_dx)) - t0[d];
+                        in = in && l3[d] >= 0 && l3[d] < TT;
+                    }
+                    const int cell = in ? l3[0] + TT * l3[2] + TT * TT * l3[1] : 64;
+                    atomicAdd(&lcnt[cell], 1u);
+                    lcell[r] = (uint8_t)cell;
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {
+                uint32_t sum = lcnt[lane], mx = lcnt[lane];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    sum += (uint32_t)__shfl_xor(sum, off);
+                    mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+                }
+                if (lane == 0) {
+                    s_tot = sum;
+                    s_max = mx;
+                }
+            }
+            __syncthreads();
+            const uint32_t maxcnt = s_max, tot = s_tot;
+
+            // r = output offset in the pass; own: a group particle (its base is a cell
+            // of the tile); second: the upper particle of its cell in the group (the
+            // wave's second window)
+            auto particle = [&](uint32_t r, bool own, bool second) {
+                const uint32_t o = beg + pb + r;
+                float ax[3], aF[9], av[3], aC[9];
+                if constexpr (FUSED) {
+                    // A3 of step n+1 on the particle in its slot o
+                    float xn[3], vn[3], Cn[9];
+                    load_xvC(nx.S, a.cap, o, xn, vn, Cn);
+                    const float mq = nx.mass ? nx.mass[o] : g.p_mass;
+                    const PMat pm = particle_mat(g, a.mt, nx.pid[o], nx.E, o, mq);
+                    Stencil sn;
+                    make_stencil(xn, g.o, g.inv_dx, sn);
+                    const int l0 = sn.b[0] - t0[0] + 1, l1 = sn.b[1] - t0[1] + 1, l2 = sn.b[2] - t0[2] + 1;
+                    const bool inw = (unsigned)l0 < (unsigned)(PN - 2) && (unsigned)l1 < (unsigned)(PN - 2) &&
+                                     (unsigned)l2 < (unsigned)(PN - 2);
+                    const uint32_t q = nx.dst[o];
+                    auto part = [&](float4& p0, float4& p1, float4& p2) {
+                        p0 = nx.part[q]; p1 = nx.part[a.cap + q]; p2 = nx.part[2 * a.cap + q];
+                    };
+                    auto getF = [&](float* Fn) { load_F(nx.S, a.cap, o, Fn); };
+                    float dE;
+                    if (inw)
+                        adj_p2g_particle<false>(
+                            [&](int ii, int jj, int kk) { return sa[((l2 + kk) * PN + (l1 + jj)) * PN + (l0 + ii)]; },
+                            part, getF, g, sn, vn, Cn, pm, mq, ax, aF, av, aC, dE);
+                    else
+                        adj_p2g_particle<true>(
+                            [&](int ii, int jj, int kk) {
+                                const int gi = sn.b[0] + ii, gj = sn.b[1] + jj, gk = sn.b[2] + kk;
+                                return in_grid(g, gi, gj, gk) ? apams[node_id(g, gi, gj, gk)]
+                                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+                            },
+                            part, getF, g, sn, vn, Cn, pm, mq, ax, aF, av, aC, dE);
+                    if (a.aEs) a.aEp[o] = nx.aEp[q] + dE;
+                }
+                const uint32_t s = a.src[o];
+                float x[3];
+                float F[9];
+                {
+                    const float4 q0 = a.S[s], q1 = a.S[a.cap + s], q2 = a.S[2 * a.cap + s];
+                    x[0] = q0.x; x[1] = q0.y; x[2] = q0.z;
+                    F[0] = q0.w; F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
+                    F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
+                }
+                if constexpr (!FUSED) load_adj(a.abar, a.cap, o, ax, aF, av, aC);
+                Stencil st;
+                make_stencil(x, g.o, g.inv_dx, st);
+                float axo[3], aFo[9];
+                if (own) {
+                    float4* __restrict__ win = pw[wave][second ? 1 : 0];
+                    const int nb = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
+                    adj_g2p_particle<false>([&](int ix, int iy, int iz) { return sv[nb + (iz * WN + iy) * WN + ix]; },
+                                     [&](int ix, int iy, int iz, float4 c) {
+                                         float4* t = win + nb + (iz * WN + iy) * WN + ix;
+                                         float4 u = *t;
+                                         u.x += c.x;
+                                         u.y += c.y;
+                                         u.z += c.z;
+                                         *t = u;
+                                         lds_order();
+                                     },
+                                     g, st, F, ax, aF, av, aC, axo, aFo);
+                } else {
+                    adj_g2p_particle<true>(
+                        [&](int ix, int iy, int iz) {
+                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+                            return in_grid(g, gi, gj, gk) ? gvs[node_id(g, gi, gj, gk)]
+                                                          : make_float4(0.f, 0.f, 0.f, 0.f);
+                        },
+                        [&](int ix, int iy, int iz, float4 c) {
+                            const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
+                            if (!in_grid(g, gi, gj, gk)) return;
+                            float* cp = reinterpret_cast<float*>(agvs + node_id(g, gi, gj, gk));
+                            atomic_add_f32(cp + 0, c.x);
+                            atomic_add_f32(cp + 1, c.y);
+                            atomic_add_f32(cp + 2, c.z);
+                        },
+                        g, st, F, ax, aF, av, aC, axo, aFo);
+                }
+                // partials by output position (coalesced); A3 finds them through dst
+                a.part[o] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
+                a.part[a.cap + o] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
+                a.part[2 * a.cap + o] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
+                a.dst[s] = o;
+                if (!FUSED && a.aEs) a.aEp[o] = a.aEs[o];
+            };
+            const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
+            if (G > 0) {
+                const uint32_t gq = tot / G, gr = tot % G;
+                for (uint32_t grp = wave; grp < G; grp += NW) {
+                    const uint32_t r = grp * gq + min(grp, gr) + (uint32_t)lane;
+                    if ((uint32_t)lane < gq + (grp < gr ? 1u : 0u))
+                        particle(r, true, lane > 0 && lcell[r - 1] == lcell[r]);
+                }
+            }
+            for (uint32_t t = tid; t < pc - tot; t += TB) particle(tot + t, false, false);
+            __syncthreads();
+        }
+        // the tile's velocity-adjoint window goes to its buffer (one per active-list entry)
+        float4* __restrict__ wb = a.wbuf + (size_t)ti * PV;
+        if (tid < WV) {
+            float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int w = 0; w < NW; ++w)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    m.x += pw[w][h][tid].x;
+                    m.y += pw[w][h][tid].y;
+                    m.z += pw[w][h][tid].z;
+                }
+            wb[tid] = m;
+        }
+        __syncthreads();
+    }
+}
+
+// A2: reverse of v_i = (p_i/m_i + dt g) * mask_i over the step's 8^3 blocks.
+// The velocity adjoint of node i is the sum of the buffered windows of the
+// tiles of binning n that cover it (the window of tile T spans nodes
+// [4T, 4T + 6)) plus the global accumulator.  Clears the accumulator, the
+// step's grid and flags.
+__global__ void __launch_bounds__(512) k_adj_grid(const SceneInfo* __restrict__ scenes,
+                                                  const uint32_t* __restrict__ bscene,
+                                                  const uint32_t* __restrict__ tslot, const float4* __restrict__ wbuf,
+                                                  float4* __restrict__ agv_all, float4* __restrict__ apam_all,
+                                                  float4* __restrict__ mp_all, uint32_t* __restrict__ flags,
+                                                  const uint32_t* __restrict__ node_bc) {
+    const int b = blockIdx.x;
+    if (flags[b] == 0u) return;
+    const SceneInfo sc = scenes[bscene[b]];
+    const MlsGeom g = sc.g;
+    float4* __restrict__ agv = agv_all + sc.goff;
+    float4* __restrict__ apam = apam_all + sc.goff;
+    float4* __restrict__ mp = mp_all + sc.goff;
+    const int lb = b - (int)sc.boff;
+    const int bx = lb % sc.gbx, by = (lb / sc.gbx) % sc.gby, bz = lb / (sc.gbx * sc.gby);
+    const int n3[3] = {bx * GB + (int)(threadIdx.x & 7), by * GB + (int)((threadIdx.x >> 3) & 7),
+                       bz * GB + (int)(threadIdx.x >> 6)};
+    const int nt[3] = {sc.ntx, sc.nty, sc.ntz};
+    if (n3[0] < g.nx && n3[1] < g.ny && n3[2] < g.nz) {
+        const size_t id = node_id(g, n3[0], n3[1], n3[2]);
+        float4 av = agv[id];
+        // the entry ranges of the 8 tiles whose windows may cover the node, then
+        // their first windows, as independent loads (two round trips); the
+        // further windows of a tile split into several work items after that
+        uint2 ts[8];
+        int wo[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            int T[3], wl[3];
+            bool ok = true;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                T[ax] = n3[ax] / TT - ((e >> ax) & 1);
+                wl[ax] = n3[ax] - TT * T[ax];
+                ok = ok && T[ax] >= 0 && T[ax] < nt[ax] && wl[ax] < WN;
+            }
+            ts[e] = ok ? reinterpret_cast<const uint2*>(tslot)[sc.toff + morton_tile(T[0], T[1], T[2])]
+                       : make_uint2(0u, 0u);
+            wo[e] = (wl[2] * WN + wl[1]) * WN + wl[0];
+        }
+        float4 u[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            u[e] = ts[e].y ? wbuf[(size_t)ts[e].x * PV + wo[e]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            av.x += u[e].x;
+            av.y += u[e].y;
+            av.z += u[e].z;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            for (uint32_t c = 1; c < ts[e].y; ++c) {
+                const float4 t = wbuf[(size_t)(ts[e].x + c) * PV + wo[e]];
+                av.x += t.x;
+                av.y += t.y;
+                av.z += t.z;
+            }
+        const float4 q = mp[id];
+        apam[id] = node_velocity_vjp(g, n3[0], n3[1], n3[2], node_bc ? node_bc[sc.goff + id] : 0u, q, av);
+        agv[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+        mp[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) flags[b] = 0u;
 }
 
 __global__ void __launch_bounds__(TB, 3) k_adj_p2g(AdjArgs a) {   // 3 waves per SIMD: latency-bound gather
@@ -814,12 +884,13 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
 }  // namespace
 
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L, float4* abar,
-                           float4* part, float4* agv, float4* apam, float* aEs, float* aEp, uint32_t* dst,
-                           hipStream_t st) {
+                           float4* const part[2], float4* agv, float4* apam, float* aEs, float* const aEp[2],
+                           uint32_t* const dst[2], hipStream_t st) {
     (void)n;
     const int nblk = w.total_blocks;
-    for (int s = L - 1; s >= 0; --s) {
-        TapeStep& t = tape[s];
+    // step s's A1 writes partials, map and dL/dE carry into set k, its A3 reads them there
+    auto args = [&](int s, int k) {
+        const TapeStep& t = tape[s];
         AdjArgs a;
         a.S = t.S; a.pid = t.pid; a.E = t.E; a.mass = t.mass;
         a.perm = t.perm;
@@ -827,27 +898,41 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.alist = reinterpret_cast<const uint4*>(t.alist);
         a.nactive = t.nactive;
         a.gv = t.gv;
-        a.abar = abar; a.part = part; a.agv = agv; a.apam = apam;
+        a.abar = abar; a.part = part[k]; a.agv = agv; a.apam = apam;
         a.aEs = aEs;
-        a.aEp = aEp;
-        a.dst = dst;
+        a.aEp = aEp[k];
+        a.dst = dst[k];
         a.wbuf = w.wbuf;
         a.scenes = w.scenes;
         a.cap = cap;
         a.mt = w.mt;
+        return a;
+    };
+    auto grid = [&](int s) {
+        ProfScope ps(w.prof, PK_ADJ_GRID, st);
+        const TapeStep& t = tape[s];
+        k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, agv, apam, t.mp, t.flags, w.node_bc);
+    };
+    int k = 0;
+    {
+        ProfScope ps(w.prof, PK_ADJ_G2P, st);
+        k_adj_g2p<false><<<w.nblocks_launch, TB, 0, st>>>(args(L - 1, k), AdjNext{});
+    }
+    grid(L - 1);
+    // A3 of step s and A1 of step s - 1 in one pass over binning s - 1
+    for (int s = L - 1; s >= 1; --s) {
+        const TapeStep& t = tape[s];
+        const AdjNext nx{t.S, t.pid, t.E, t.mass, apam, part[k], dst[k], aEp[k]};
         {
-            ProfScope ps(w.prof, PK_ADJ_G2P, st);
-            k_adj_g2p<<<w.nblocks_launch, TB, 0, st>>>(a);
+            ProfScope ps(w.prof, PK_ADJ_FUSED, st);
+            k_adj_g2p<true><<<w.nblocks_launch, TB, 0, st>>>(args(s - 1, k ^ 1), nx);
         }
-        {
-            ProfScope ps(w.prof, PK_ADJ_GRID, st);
-            k_adj_grid<<<nblk, 512, 0, st>>>(w.scenes, w.bscene, t.tslot, w.wbuf, agv, apam, t.mp, t.flags,
-                                             w.node_bc);
-        }
-        {
-            ProfScope ps(w.prof, PK_ADJ_P2G, st);
-            k_adj_p2g<<<w.nblocks_launch, TB, 0, st>>>(a);
-        }
+        k ^= 1;
+        grid(s - 1);
+    }
+    {
+        ProfScope ps(w.prof, PK_ADJ_P2G, st);
+        k_adj_p2g<<<w.nblocks_launch, TB, 0, st>>>(args(0, k));
     }
     return hipGetLastError();
 }

@@ -98,6 +98,7 @@ enum ProfKernel {
     PK_ADJ_G2P = 9,    // reverse G2P (tiled)
     PK_ADJ_GRID = 10,  // reverse grid update
     PK_ADJ_P2G = 11,   // reverse P2G (tiled)
+    PK_ADJ_FUSED = 12, // reverse P2G of step n+1 fused with the reverse G2P of step n (tiled)
     PK_COUNT = 16
 };
 
@@ -241,12 +242,14 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
                               hipStream_t st);
 
 // Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
-// aEs: dL/dE by storage slot, carried through the steps like abar (NULL: not
-// wanted); aEp: its per-step carry buffer; part/aEp are indexed by output
-// position and dst (cap) maps a step's storage slots to its output positions.
+// abar: the adjoint state by storage slot, in (after step L-1) and out
+// (before step 0).  aEs: dL/dE by storage slot, in and out like abar (NULL:
+// not wanted).  Two sets of per-step buffers, used alternately: part (3*cap)
+// and aEp (cap) by output position, dst (cap) from a step's storage slots to
+// its output positions.
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
-                           float4* abar, float4* part, float4* agv, float4* apam, float* aEs, float* aEp,
-                           uint32_t* dst, hipStream_t st);
+                           float4* abar, float4* const part[2], float4* agv, float4* apam, float* aEs,
+                           float* const aEp[2], uint32_t* const dst[2], hipStream_t st);
 // The plain port of the reverse pass over a segment recomputed with
 // mls_plain_step: tape[s].S is the state before step s (user order),
 // tape[s].mp / .gv the dense grid of step s.

@@ -285,7 +285,8 @@ class MPMContext:
         return out
 
     KERNELS = {"tile_p2g": 0, "tile_g2p2g": 1, "tile_g2p": 2, "grid": 3, "bin": 4, "plain_p2g": 5,
-               "plain_g2p": 6, "clear": 7, "ref_step": 8, "adj_g2p": 9, "adj_grid": 10, "adj_p2g": 11}
+               "plain_g2p": 6, "clear": 7, "ref_step": 8, "adj_g2p": 9, "adj_grid": 10, "adj_p2g": 11,
+               "adj_fused": 12}
 
     def set_profiling(self, on: bool):
         check(self._L.pg_mpm_set_profiling(self._h, 1 if on else 0), self._h)
