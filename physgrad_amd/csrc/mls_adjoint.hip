@@ -45,6 +45,7 @@ namespace {
 
 struct AdjArgs {
     const float4* S;        // tape state of step n
+    const float4* Snext;    // tape state of step n+1 (by its slots: step n's output positions)
     const uint32_t* pid;    // slot -> user index (step n)
     const float* E;         // per-particle E, slot order (or NULL)
     const float* mass;      // per-particle mass, slot order (or NULL)
@@ -115,13 +116,16 @@ constexpr int APASS = 1280;   // particles per LDS pass of a work item (larger w
 // the adjoints of its new state (ax, aF, av, aC):
 //   C' = 4/dx sum_n w u_n (n - f)^T,  x' = x + dt v',  F' = (I + dt C') F
 // emits the velocity adjoint g_n = w_n (anv + 4/dx^2 anc d_n) of each stencil
-// node (d_n = (n - f) dx) and returns the adjoints of x and F.
+// node (d_n = (n - f) dx) and returns the adjoints of x and F.  With VC the
+// caller passes the forward's v', C' (vq, Cq: the new state holds exactly what
+// the forward's G2P summed), else they are summed again from the nodes.
 __device__ __forceinline__ float pick3(const float* w, int i) { return i == 0 ? w[0] : (i == 1 ? w[1] : w[2]); }
 
-template <bool ROLLED, class Load, class Emit>
+template <bool ROLLED, bool VC, class Load, class Emit>
 __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const MlsGeom& g, const Stencil& st,
                                                  const float* F, const float* ax, const float* aF,
-                                                 const float* av, const float* aC, float* axo, float* aFo) {
+                                                 const float* av, const float* aC, const float* vq, const float* Cq,
+                                                 float* axo, float* aFo) {
     const float s4 = 4.f * g.inv_dx * g.inv_dx;
     // adjoints of v' and C' (through x' = x + dt v' and F' = (I + dt C') F); they do
     // not depend on C' itself
@@ -140,9 +144,9 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
     if (ROLLED) {
         // rare path: two rolled passes, stencil values picked by selects
 #pragma unroll
-        for (int c = 0; c < 9; ++c) nc[c] = 0.f;
+        for (int c = 0; c < 9; ++c) nc[c] = VC ? Cq[c] : 0.f;
 #pragma unroll 1
-        for (int n = 0; n < 27; ++n) {
+        for (int n = 0; n < (VC ? 0 : 27); ++n) {
             const int ix = n % 3, iy = (n / 3) % 3, iz = n / 9;
             const float4 u = load(ix, iy, iz);
             const float w = pick3(st.w[0], ix) * pick3(st.w[1], iy) * pick3(st.w[2], iz);
@@ -212,11 +216,12 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
                 for (int x = 0; x < 3; ++x) {
                     const float u[3] = {ur[x].x, ur[x].y, ur[x].z};
                     const float wx = st.w[0][x], wdx = wx * ((float)x - st.f[0]);
+                    if (!VC)
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        P[c] += wx * u[c];
-                        Q[c] += wdx * u[c];
-                    }
+                        for (int c = 0; c < 3; ++c) {
+                            P[c] += wx * u[c];
+                            Q[c] += wdx * u[c];
+                        }
                     const float aw = u[0] * tx[0] + u[1] * tx[1] + u[2] * tx[2];
                     Pw += wx * aw;
                     Pd += dw[0][x] * aw;
@@ -231,9 +236,11 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
                 const float wdy = wy * ((float)y - st.f[1]), dwy = pick3(dw[1], y);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    R[c] += wy * P[c];
-                    Ry[c] += wdy * P[c];
-                    Rx[c] += wy * Q[c];
+                    if (!VC) {
+                        R[c] += wy * P[c];
+                        Ry[c] += wdy * P[c];
+                        Rx[c] += wy * Q[c];
+                    }
                     ty[c] += tdy[c];
                 }
                 Rw += wy * Pw;
@@ -243,10 +250,12 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
             const float wdz = wz * ((float)z - st.f[2]), dwz = pick3(dw[2], z);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                v[c] += wz * R[c];
-                cz[c] += wdz * R[c];
-                cy[c] += wz * Ry[c];
-                cx[c] += wz * Rx[c];
+                if (!VC) {
+                    v[c] += wz * R[c];
+                    cz[c] += wdz * R[c];
+                    cy[c] += wz * Ry[c];
+                    cx[c] += wz * Rx[c];
+                }
                 tz[c] += tdz[c];
             }
             ga[2] += dwz * Rw;
@@ -256,9 +265,10 @@ __device__ __forceinline__ void adj_g2p_particle(Load load, Emit emit, const Mls
         const float s = 4.f * g.inv_dx;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            nc[3 * c + 0] = s * cx[c];
-            nc[3 * c + 1] = s * cy[c];
-            nc[3 * c + 2] = s * cz[c];
+            if (VC) v[c] = vq[c];
+            nc[3 * c + 0] = VC ? Cq[3 * c + 0] : s * cx[c];
+            nc[3 * c + 1] = VC ? Cq[3 * c + 1] : s * cy[c];
+            nc[3 * c + 2] = VC ? Cq[3 * c + 2] : s * cz[c];
         }
         // d_n = x_n - x_p also moves with x: - s4 dx anc^T sum w u
 #pragma unroll
@@ -567,13 +577,22 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a, AdjNext nx) {
                     F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
                 }
                 if constexpr (!FUSED) load_adj(a.abar, a.cap, o, ax, aF, av, aC);
+                // the forward's G2P result v', C' from the new state
+                float vq[3], Cq[9];
+                {
+                    const float4 q3 = a.Snext[3 * a.cap + o], q4 = a.Snext[4 * a.cap + o],
+                                 q5 = a.Snext[5 * a.cap + o];
+                    vq[0] = q3.x; vq[1] = q3.y; vq[2] = q3.z; Cq[0] = q3.w;
+                    Cq[1] = q4.x; Cq[2] = q4.y; Cq[3] = q4.z; Cq[4] = q4.w;
+                    Cq[5] = q5.x; Cq[6] = q5.y; Cq[7] = q5.z; Cq[8] = q5.w;
+                }
                 Stencil st;
                 make_stencil(x, g.o, g.inv_dx, st);
                 float axo[3], aFo[9];
                 if (own) {
                     float4* __restrict__ win = pw[wave][second ? 1 : 0];
                     const int nb = ((st.b[2] - t0[2]) * WN + (st.b[1] - t0[1])) * WN + (st.b[0] - t0[0]);
-                    adj_g2p_particle<false>([&](int ix, int iy, int iz) { return sv[nb + (iz * WN + iy) * WN + ix]; },
+                    adj_g2p_particle<false, true>([&](int ix, int iy, int iz) { return sv[nb + (iz * WN + iy) * WN + ix]; },
                                      [&](int ix, int iy, int iz, float4 c) {
                                          float4* t = win + nb + (iz * WN + iy) * WN + ix;
                                          float4 u = *t;
@@ -583,9 +602,9 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a, AdjNext nx) {
                                          *t = u;
                                          lds_order();
                                      },
-                                     g, st, F, ax, aF, av, aC, axo, aFo);
+                                     g, st, F, ax, aF, av, aC, vq, Cq, axo, aFo);
                 } else {
-                    adj_g2p_particle<true>(
+                    adj_g2p_particle<true, true>(
                         [&](int ix, int iy, int iz) {
                             const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
                             return in_grid(g, gi, gj, gk) ? gvs[node_id(g, gi, gj, gk)]
@@ -599,7 +618,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a, AdjNext nx) {
                             atomic_add_f32(cp + 1, c.y);
                             atomic_add_f32(cp + 2, c.z);
                         },
-                        g, st, F, ax, aF, av, aC, axo, aFo);
+                        g, st, F, ax, aF, av, aC, vq, Cq, axo, aFo);
                 }
                 // partials by output position (coalesced); A3 finds them through dst
                 a.part[o] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
@@ -781,7 +800,7 @@ __global__ void __launch_bounds__(256) k_adj_g2p_plain(int n, size_t cap, MlsGeo
     Stencil st;
     make_stencil(x, g.o, g.inv_dx, st);
     float axo[3], aFo[9];
-    adj_g2p_particle<false>(
+    adj_g2p_particle<false, false>(
         [&](int ix, int iy, int iz) {
             const int gi = st.b[0] + ix, gj = st.b[1] + iy, gk = st.b[2] + iz;
             return in_grid(g, gi, gj, gk) ? gv[node_id(g, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -794,7 +813,7 @@ __global__ void __launch_bounds__(256) k_adj_g2p_plain(int n, size_t cap, MlsGeo
             atomic_add_f32(cp + 1, c.y);
             atomic_add_f32(cp + 2, c.z);
         },
-        g, st, F, ax, aF, av, aC, axo, aFo);
+        g, st, F, ax, aF, av, aC, nullptr, nullptr, axo, aFo);
     part[i] = make_float4(axo[0], axo[1], axo[2], aFo[0]);
     part[cap + i] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
     part[2 * cap + i] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
@@ -893,6 +912,7 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         const TapeStep& t = tape[s];
         AdjArgs a;
         a.S = t.S; a.pid = t.pid; a.E = t.E; a.mass = t.mass;
+        a.Snext = tape[s + 1].S;
         a.perm = t.perm;
         a.src = t.src;
         a.alist = reinterpret_cast<const uint4*>(t.alist);
