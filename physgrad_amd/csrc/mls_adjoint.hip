@@ -900,7 +900,43 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
     }
 }
 
+// storage order -> storage order of the adjoint planes between two segments:
+// inv[u] = slot of caller index u in the old order, then each new slot takes
+// the planes (and dL/dE) of its particle's old slot.  Both orders are sorted by
+// tile, so the gather stays near the slot it writes.
+__global__ void k_slot_of(int n, const uint32_t* __restrict__ pid, uint32_t* __restrict__ inv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) {
+        const uint32_t u = pid[s];
+        if (u < (uint32_t)n) inv[u] = (uint32_t)s;
+    }
+}
+
+__global__ void k_reorder_adj(int n, size_t cap, const uint32_t* __restrict__ pid_new,
+                              const uint32_t* __restrict__ inv_old, const float4* __restrict__ abar_old,
+                              float4* __restrict__ abar_new, const float* __restrict__ aEs_old,
+                              float* __restrict__ aEs_new) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t u = pid_new[s];
+    const size_t so = u < (uint32_t)n ? inv_old[u] : (size_t)s;
+    float x[3], v[3], C[9], F[9];
+    load_state(abar_old, cap, so, x, F, v, C);
+    store_state(abar_new, cap, s, x, F, v, C);
+    if (aEs_new) aEs_new[s] = aEs_old[so];
+}
+
 }  // namespace
+
+hipError_t adjoint_reorder(int n, size_t cap, const uint32_t* pid_old, const uint32_t* pid_new, uint32_t* inv,
+                           const float4* abar_old, float4* abar_new, const float* aEs_old, float* aEs_new,
+                           hipStream_t st) {
+    if (n > 0) {
+        k_slot_of<<<(n + 255) / 256, 256, 0, st>>>(n, pid_old, inv);
+        k_reorder_adj<<<(n + 255) / 256, 256, 0, st>>>(n, cap, pid_new, inv, abar_old, abar_new, aEs_old, aEs_new);
+    }
+    return hipGetLastError();
+}
 
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L, float4* abar,
                            float4* const part[2], float4* agv, float4* apam, float* aEs, float* const aEp[2],

@@ -264,6 +264,12 @@ hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float
                              hipStream_t st);
 hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,
                            float* aC, float* aF, const float* aEs, float* aE, hipStream_t st);
+// adjoint planes (and dL/dE when aEs_old is not NULL) by the storage order
+// pid_old -> by the storage order pid_new of the same particles (inv: cap
+// scratch entries)
+hipError_t adjoint_reorder(int n, size_t cap, const uint32_t* pid_old, const uint32_t* pid_new, uint32_t* inv,
+                           const float4* abar_old, float4* abar_new, const float* aEs_old, float* aEs_new,
+                           hipStream_t st);
 
 hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t* pid, hipStream_t st);
 // Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on
