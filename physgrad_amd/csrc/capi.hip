@@ -1143,6 +1143,7 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     // checkpoint k, where segment k's reverse ends) and is reordered into the
     // end order of segment k-1's recompute
     bool carried = false;
+    const pg::TapeStep own0 = c->tape.empty() ? pg::TapeStep{} : c->tape[0];
     for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
         const int L = std::min(K, c->ck_steps - k * K);
         pg::TapeStep* T = c->tape.data();
@@ -1167,13 +1168,13 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
             continue;
         }
         const bool taped = k == c->tape_seg && c->tape_version == c->version && c->tape_filled == L;
-        if (!taped) {   // recompute the segment into the tape
-            if (e == hipSuccess) e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
-            if (e == hipSuccess) e = copy_u32(T[0].pid, ck.pid, n, st);
-            if (e == hipSuccess && c->have_E)
-                e = hipMemcpyAsync(T[0].E, ck.E, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
-            if (e == hipSuccess && c->have_mass)
-                e = hipMemcpyAsync(T[0].mass, ck.mass, 4 * (size_t)n, hipMemcpyDeviceToDevice, st);
+        if (!taped) {
+            // recompute the segment into the tape, starting from the checkpoint in
+            // place: neither the recompute nor the reverse writes step 0's state
+            T[0].S = ck.S;
+            T[0].pid = ck.pid;
+            T[0].E = ck.E;
+            T[0].mass = ck.mass;
             if (e == hipSuccess) e = pg::tiled_tape_forward(c->tw, n, c->cap, T, L, st);
         }
         c->tape_seg = -1;
@@ -1198,6 +1199,7 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         if (e == hipSuccess && k == 0)
             e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);
         carried = true;
+        T[0] = own0;   // the tape's own step-0 buffers again (the forward writes into them)
     }
     auto out = [&](float* dst, const float* src, size_t count) {
         if (e == hipSuccess && dst) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDefault, st);
