@@ -74,9 +74,7 @@ struct pg_mpm_ctx {
     float4* part2 = nullptr;
     float* aEp2 = nullptr;
     uint32_t* adst2 = nullptr;
-    // the adjoint state and dL/dE carried from one segment to the next in the next one's order
-    float4* abar2 = nullptr;
-    float* aEs2 = nullptr;
+    uint32_t* ainv = nullptr;               // caller index -> slot of the order the adjoint is carried in
     uint32_t* check = nullptr;              // [4] results of the state validation kernel
     bool validate = false;                  // pg_mpm_step validates the state after stepping
     // per-particle materials (pg_mpm_set_materials) and per-node boundary flags (pg_mpm_set_boundary)
@@ -1006,8 +1004,7 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
         (rc = valloc(c, A, &c->aEp, c->cap, false)) || (rc = valloc(c, A, &c->adst, c->cap, false)) ||
         (!plain && ((rc = valloc(c, A, &c->part2, 3 * c->cap, false)) ||
                     (rc = valloc(c, A, &c->aEp2, c->cap, false)) || (rc = valloc(c, A, &c->adst2, c->cap, false)) ||
-                    (rc = valloc(c, A, &c->abar2, 6 * c->cap, false)) ||
-                    (rc = valloc(c, A, &c->aEs2, c->cap, false)))) ||
+                    (rc = valloc(c, A, &c->ainv, c->cap, false)))) ||
         (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uv, 3 * c->cap, false)) || (rc = valloc(c, A, &c->uC, 9 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uF, 9 * c->cap, false)))
@@ -1020,8 +1017,9 @@ fail:
     c->abar = c->part = c->agv = c->apam = nullptr;
     c->aE = c->aEs = c->aEp = c->ux = c->uv = c->uC = c->uF = nullptr;
     c->adst = c->adst2 = nullptr;
-    c->part2 = c->abar2 = nullptr;
-    c->aEp2 = c->aEs2 = nullptr;
+    c->part2 = nullptr;
+    c->aEp2 = nullptr;
+    c->ainv = nullptr;
     return rc;
 }
 
@@ -1140,8 +1138,8 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     const float* pE = c->have_E ? c->E : nullptr;
     const float* pm = c->have_mass ? c->mass : nullptr;
     // tiled: between segments the adjoint stays in storage order (that of
-    // checkpoint k, where segment k's reverse ends) and is reordered into the
-    // end order of segment k-1's recompute
+    // checkpoint k, where segment k's reverse ends); segment k-1's first reverse
+    // kernel reads it through the inverse of that order
     bool carried = false;
     const pg::TapeStep own0 = c->tape.empty() ? pg::TapeStep{} : c->tape[0];
     for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
@@ -1181,20 +1179,13 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         c->tape_filled = 0;   // the reverse pass clears the grids it reads
         if (e == hipSuccess && !carried)
             e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, aE, aEs, st);
-        if (e == hipSuccess && carried) {
-            e = pg::adjoint_reorder(n, c->cap, c->ckpt[k + 1].pid, T[L].pid, c->adst, c->abar, c->abar2, aEs,
-                                    aEs ? c->aEs2 : nullptr, st);
-            std::swap(c->abar, c->abar2);
-            if (aEs) {
-                std::swap(c->aEs, c->aEs2);
-                aEs = c->aEs;
-            }
-        }
+        if (e == hipSuccess && carried) e = pg::slot_of_caller(n, c->ckpt[k + 1].pid, c->ainv, st);
         if (e == hipSuccess) {
             float4* const part[2] = {c->part, c->part2};
             float* const aEp[2] = {c->aEp, c->aEp2};
             uint32_t* const dst[2] = {c->adst, c->adst2};
-            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, part, c->agv, c->apam, aEs, aEp, dst, st);
+            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, carried ? c->ainv : nullptr, part, c->agv,
+                                    c->apam, aEs, aEp, dst, st);
         }
         if (e == hipSuccess && k == 0)
             e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);

@@ -55,6 +55,8 @@ struct AdjArgs {
     const uint32_t* nactive;
     const float4* gv;       // grid (v, m) of step n
     float4* abar;           // adjoint state planes
+    const uint32_t* in_pid; // with in_inv: A1 (not fused) reads the adjoint of output position o at slot
+    const uint32_t* in_inv; // in_inv[in_pid[o]] (it is stored in another order of the same particles)
     float4* part;           // partial adjoint planes (3*cap) by slot of step n: (ax, aF)
     float4* agv;            // grid accumulator of the velocity adjoint (x,y,z used)
     const float4* apam;     // (am, ap) per node
@@ -576,7 +578,8 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a, AdjNext nx) {
                     F[0] = q0.w; F[1] = q1.x; F[2] = q1.y; F[3] = q1.z; F[4] = q1.w;
                     F[5] = q2.x; F[6] = q2.y; F[7] = q2.z; F[8] = q2.w;
                 }
-                if constexpr (!FUSED) load_adj(a.abar, a.cap, o, ax, aF, av, aC);
+                const uint32_t oin = (!FUSED && a.in_inv) ? a.in_inv[a.in_pid[o]] : o;
+                if constexpr (!FUSED) load_adj(a.abar, a.cap, oin, ax, aF, av, aC);
                 // the forward's G2P result v', C' from the new state
                 float vq[3], Cq[9];
                 {
@@ -625,7 +628,7 @@ __global__ void __launch_bounds__(TB, 3) k_adj_g2p(AdjArgs a, AdjNext nx) {
                 a.part[a.cap + o] = make_float4(aFo[1], aFo[2], aFo[3], aFo[4]);
                 a.part[2 * a.cap + o] = make_float4(aFo[5], aFo[6], aFo[7], aFo[8]);
                 a.dst[s] = o;
-                if (!FUSED && a.aEs) a.aEp[o] = a.aEs[o];
+                if (!FUSED && a.aEs) a.aEp[o] = a.aEs[oin];
             };
             const uint32_t G = max((tot + 63u) / 64u, (maxcnt + 1u) / 2u);
             if (G > 0) {
@@ -900,10 +903,7 @@ __global__ void k_scatter_user(int n, size_t cap, const uint32_t* __restrict__ p
     }
 }
 
-// storage order -> storage order of the adjoint planes between two segments:
-// inv[u] = slot of caller index u in the old order, then each new slot takes
-// the planes (and dL/dE) of its particle's old slot.  Both orders are sorted by
-// tile, so the gather stays near the slot it writes.
+// caller index -> slot of a storage order
 __global__ void k_slot_of(int n, const uint32_t* __restrict__ pid, uint32_t* __restrict__ inv) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) {
@@ -912,35 +912,16 @@ __global__ void k_slot_of(int n, const uint32_t* __restrict__ pid, uint32_t* __r
     }
 }
 
-__global__ void k_reorder_adj(int n, size_t cap, const uint32_t* __restrict__ pid_new,
-                              const uint32_t* __restrict__ inv_old, const float4* __restrict__ abar_old,
-                              float4* __restrict__ abar_new, const float* __restrict__ aEs_old,
-                              float* __restrict__ aEs_new) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t u = pid_new[s];
-    const size_t so = u < (uint32_t)n ? inv_old[u] : (size_t)s;
-    float x[3], v[3], C[9], F[9];
-    load_state(abar_old, cap, so, x, F, v, C);
-    store_state(abar_new, cap, s, x, F, v, C);
-    if (aEs_new) aEs_new[s] = aEs_old[so];
-}
-
 }  // namespace
 
-hipError_t adjoint_reorder(int n, size_t cap, const uint32_t* pid_old, const uint32_t* pid_new, uint32_t* inv,
-                           const float4* abar_old, float4* abar_new, const float* aEs_old, float* aEs_new,
-                           hipStream_t st) {
-    if (n > 0) {
-        k_slot_of<<<(n + 255) / 256, 256, 0, st>>>(n, pid_old, inv);
-        k_reorder_adj<<<(n + 255) / 256, 256, 0, st>>>(n, cap, pid_new, inv, abar_old, abar_new, aEs_old, aEs_new);
-    }
+hipError_t slot_of_caller(int n, const uint32_t* pid, uint32_t* inv, hipStream_t st) {
+    if (n > 0) k_slot_of<<<(n + 255) / 256, 256, 0, st>>>(n, pid, inv);
     return hipGetLastError();
 }
 
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L, float4* abar,
-                           float4* const part[2], float4* agv, float4* apam, float* aEs, float* const aEp[2],
-                           uint32_t* const dst[2], hipStream_t st) {
+                           const uint32_t* in_inv, float4* const part[2], float4* agv, float4* apam, float* aEs,
+                           float* const aEp[2], uint32_t* const dst[2], hipStream_t st) {
     (void)n;
     const int nblk = w.total_blocks;
     // step s's A1 writes partials, map and dL/dE carry into set k, its A3 reads them there
@@ -955,6 +936,8 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
         a.nactive = t.nactive;
         a.gv = t.gv;
         a.abar = abar; a.part = part[k]; a.agv = agv; a.apam = apam;
+        a.in_pid = nullptr;
+        a.in_inv = nullptr;
         a.aEs = aEs;
         a.aEp = aEp[k];
         a.dst = dst[k];
@@ -972,7 +955,10 @@ hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape
     int k = 0;
     {
         ProfScope ps(w.prof, PK_ADJ_G2P, st);
-        k_adj_g2p<false><<<w.nblocks_launch, TB, 0, st>>>(args(L - 1, k), AdjNext{});
+        AdjArgs a = args(L - 1, k);
+        a.in_pid = tape[L].pid;
+        a.in_inv = in_inv;
+        k_adj_g2p<false><<<w.nblocks_launch, TB, 0, st>>>(a, AdjNext{});
     }
     grid(L - 1);
     // A3 of step s and A1 of step s - 1 in one pass over binning s - 1

@@ -244,12 +244,15 @@ hipError_t tiled_tape_forward(TiledWork& w, int n, size_t cap, TapeStep* tape, i
 // Reverse pass over a recomputed segment tape[0..L-1] (see mls_adjoint.hip).
 // abar: the adjoint state by storage slot, in (after step L-1) and out
 // (before step 0).  aEs: dL/dE by storage slot, in and out like abar (NULL:
-// not wanted).  Two sets of per-step buffers, used alternately: part (3*cap)
-// and aEp (cap) by output position, dst (cap) from a step's storage slots to
-// its output positions.
+// not wanted).  in_inv (NULL: abar and aEs come in tape[L]'s order): abar and
+// aEs come in another storage order of the same particles, whose slot of
+// caller index u is in_inv[u].  Two sets of per-step buffers, used
+// alternately: part (3*cap) and aEp (cap) by output position, dst (cap) from a
+// step's storage slots to its output positions.
 hipError_t adjoint_segment(const TiledWork& w, int n, size_t cap, TapeStep* tape, int L,
-                           float4* abar, float4* const part[2], float4* agv, float4* apam, float* aEs,
-                           float* const aEp[2], uint32_t* const dst[2], hipStream_t st);
+                           float4* abar, const uint32_t* in_inv, float4* const part[2], float4* agv,
+                           float4* apam, float* aEs, float* const aEp[2], uint32_t* const dst[2],
+                           hipStream_t st);
 // The plain port of the reverse pass over a segment recomputed with
 // mls_plain_step: tape[s].S is the state before step s (user order),
 // tape[s].mp / .gv the dense grid of step s.
@@ -264,12 +267,8 @@ hipError_t adjoint_from_user(int n, size_t cap, const uint32_t* pid, const float
                              hipStream_t st);
 hipError_t adjoint_to_user(int n, size_t cap, const uint32_t* pid, const float4* abar, float* ax, float* av,
                            float* aC, float* aF, const float* aEs, float* aE, hipStream_t st);
-// adjoint planes (and dL/dE when aEs_old is not NULL) by the storage order
-// pid_old -> by the storage order pid_new of the same particles (inv: cap
-// scratch entries)
-hipError_t adjoint_reorder(int n, size_t cap, const uint32_t* pid_old, const uint32_t* pid_new, uint32_t* inv,
-                           const float4* abar_old, float4* abar_new, const float* aEs_old, float* aEs_new,
-                           hipStream_t st);
+// inv[pid[s]] = s: the slot of each caller index in a storage order
+hipError_t slot_of_caller(int n, const uint32_t* pid, uint32_t* inv, hipStream_t st);
 
 hipError_t tiled_init_keys(TiledWork& w, int n, const float4* S, const uint32_t* pid, hipStream_t st);
 // Runs nsteps full steps.  S/S2, pid/pid2, E/E2, mass/mass2 ping-pong; on
