@@ -44,7 +44,7 @@ def oracle_reverse(O, sc, x, v, C, F, E, nsteps, seeds):
 
 @pytest.mark.parametrize("impl", [1, 0], ids=["tiled", "plain"])
 @pytest.mark.parametrize("material", [0, 1], ids=["fixed_corotated", "neo_hookean"])
-@pytest.mark.parametrize("nsteps,K", [(1, 1), (7, 3)], ids=["one_step", "three_segments"])
+@pytest.mark.parametrize("nsteps,K", [(1, 1), (7, 3), (10, 4)], ids=["one_step", "three_segments", "fused_4_4_2"])
 def test_backward_vs_oracle_reverse(gpu, oracle_lib, material, nsteps, K, impl):
     from physgrad_amd.solver import MPMContext
     O = oracle_lib
@@ -192,6 +192,28 @@ def test_checkpointed_forward_matches_steps_and_refills_tape(gpu):
         assert rel(got[key].cpu().numpy(), want[key].cpu().numpy()) < 1e-4, key
     for ctx in (a, b, c):
         ctx.close()
+
+
+def test_backward_twice(gpu):
+    """A second backward of the same forward recomputes every segment (also the
+    last, which the forward had written into the tape) from its checkpoint and
+    reads each segment's adjoint through the next one's order: the gradients
+    must equal the first backward's.  Segments of 4, 4, 3 steps."""
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(29)
+    n = 3000
+    sc = scene(rng, n, 1)
+    seeds = [torch.as_tensor(rng.normal(size=(n, c)), dtype=torch.float32) for c in (3, 3, 9, 9)]
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, E=sc.E_particles)
+    ctx.forward_checkpointed(11, 4)
+    first = {k: v.cpu().numpy() for k, v in ctx.backward(*seeds).items()}
+    second = {k: v.cpu().numpy() for k, v in ctx.backward(*seeds).items()}
+    for key in ("x", "v", "C", "F", "E"):
+        err = rel(second[key], first[key])
+        print(key, err)
+        assert err < 1e-4, (key, err)
+    ctx.close()
 
 
 def test_backward_requires_forward(gpu):
