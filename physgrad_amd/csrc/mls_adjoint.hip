@@ -8,13 +8,14 @@
 // segment tape written by tiled_tape_forward; for step n of the tape:
 //
 //   A1  k_adj_g2p   tiles of binning n, particles in the forward tile
-//                   kernel's groups (contiguous output offsets): recompute v, C
-//                   from the gv window in LDS, push the adjoint of the new
-//                   (v, C) into the wave's LDS windows by read-modify-write
-//                   (summed into the tile's window buffer), accumulate the
-//                   position adjoint from the weights and x_i - x_p;
-//                   partial = (ax, aF) by output position (coalesced), and
-//                   dst = the storage slot -> output position map of the step.
+//                   kernel's groups (contiguous output offsets): with the
+//                   forward's v', C' (the new state on the tape) and the gv
+//                   window in LDS, push the adjoint of the new (v, C) into the
+//                   wave's LDS windows by read-modify-write (summed into the
+//                   tile's window buffer), accumulate the position adjoint
+//                   from the weights and x_i - x_p; partial = (ax, aF) by
+//                   output position (coalesced), and dst = the storage slot ->
+//                   output position map of the step.
 //   A2  k_adj_grid  flagged 8^3 blocks: the node's velocity adjoint is the sum
 //                   of the window buffers that cover it plus agv; ap = agv/m,
 //                   am = -(agv.p)/m^2 (zero on sticky or empty nodes); clears
