@@ -350,6 +350,29 @@ def test_empty_and_edge_particles(gpu):
         ctx.close()
 
 
+def test_largest_grid_dense_box(gpu, oracle_lib):
+    """The largest grid the tiled path takes (256^3 nodes: 64 tiles per axis,
+    2^18 tile slots) with 600,000 particles in a dense box (~9 per cell, ~2,200
+    tiles): one step against the FP64 oracle, particles and grid."""
+    from physgrad_amd.scenes import Scene
+    O = oracle_lib
+    rng = np.random.default_rng(41)
+    side, n = 256, 600000
+    lo, hi = 0.4, 0.6
+    x = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    v = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+    sc = Scene("largest_grid", (side,) * 3, 1 / side, 1e-4, 1, 1, 2e4, 0.3, 1000.0, (hi - lo) ** 3 / n, x, v)
+    print("largest grid", side, n, flush=True)
+    want = O.mls_run(O.scene_params(sc), x.astype(np.float64), v.astype(np.float64), None, None, sc.p_mass,
+                     sc.E, 1, want_grid=True)
+    got = gpu_step(sc, 1, x, v, None, None, None)
+    for key in ("x", "v", "C", "F"):
+        err = rel(got[key].reshape(-1), want[key].reshape(-1))
+        print(key, err)
+        assert err <= TOL_STEP, (key, err)
+    check_grid(got["grid_m"], got["grid_v"], want["grid_m"], want["grid_v"])
+
+
 @pytest.mark.skip(reason="raised an illegal memory access on MI355X when the window buffers held 2^16 work "
                          "items and later items added with global atomics; no cause was found in the kernels or "
                          "the host sizing, so this case stays off the GPU (that fallback has since been removed)")
