@@ -350,6 +350,37 @@ def test_empty_and_edge_particles(gpu):
         ctx.close()
 
 
+@pytest.mark.parametrize("offsets", [(0.5,), (0.25, 0.75)], ids=["cell_centres", "quarter_lattice"])
+def test_lattice_positions_parity(gpu, oracle_lib, offsets):
+    """Particles on a regular lattice, at cell centres (x / dx - 1/2 an exact
+    integer: the stencil base sits on a floor boundary and the fraction is 0)
+    or at the quarter points (8 per cell, fractions 0.25 / 0.75), all exactly
+    representable: one step of both paths against the FP64 oracle."""
+    from physgrad_amd.scenes import Scene
+    O = oracle_lib
+    side = 32
+    dx = 1 / side
+    cells = np.arange(8, 24)
+    coords = (cells[:, None] + np.asarray(offsets)[None, :]).reshape(-1) * dx
+    z, y, xx = np.meshgrid(coords, coords, coords, indexing="ij")
+    x = np.stack([xx.reshape(-1), y.reshape(-1), z.reshape(-1)], 1).astype(np.float32)
+    n = len(x)
+    rng = np.random.default_rng(47)
+    v = rng.normal(0, 0.2, (n, 3)).astype(np.float32)
+    C = rng.normal(0, 0.3, (n, 9)).astype(np.float32)
+    F = (np.eye(3).reshape(1, 9) + rng.normal(0, 0.02, (n, 9))).astype(np.float32)
+    sc = Scene("lattice", (side,) * 3, dx, 1e-4, 1, 1, 2e4, 0.3, 1000.0, (16 * dx) ** 3 / n, x, v)
+    want = O.mls_run(O.scene_params(sc), x.astype(np.float64), v.astype(np.float64), C.astype(np.float64),
+                     F.astype(np.float64), sc.p_mass, sc.E, 1, want_grid=True)
+    for impl in (1, 0):
+        print("lattice", offsets, "impl", impl, n, flush=True)
+        got = gpu_step(sc, impl, x, v, C, F, None)
+        for key in ("x", "v", "C", "F"):
+            err = rel(got[key].reshape(-1), want[key].reshape(-1))
+            assert err <= TOL_STEP, (key, impl, err)
+        check_grid(got["grid_m"], got["grid_v"], want["grid_m"], want["grid_v"])
+
+
 def test_largest_grid_dense_box(gpu, oracle_lib):
     """The largest grid the tiled path takes (256^3 nodes: 64 tiles per axis,
     2^18 tile slots) with 600,000 particles in a dense box (~9 per cell, ~2,200
