@@ -143,6 +143,15 @@ def fwd_bwd_leg(scene, steps, ckpt, impl):
     t_f, t_b = a.elapsed_time(b) / 1e3, b2.elapsed_time(c) / 1e3
     err = closed_form_error(grads, [scene], [n], resid, steps)
     active = int((ctx.get_grid()["mass"] > 0).sum().item())
+    # per-kernel times (HIP events on the launch stream) of a shorter forward+backward
+    # over the same scene: two segments, the later one recomputed
+    ctx.set_particles(scene.x, scene.v, E=scene.E_particles)
+    ctx.set_profiling(True)
+    ctx.forward_checkpointed(2 * ckpt, ckpt)
+    ctx.backward(seed)
+    torch.cuda.synchronize()
+    kt = {k: v[0] / v[1] for k, v in ctx.kernel_times().items()}
+    ctx.set_profiling(False)
     ctx.close()
     # algorithmic bytes per step: forward, recompute, tape write, adjoint state read/read/write
     # (96 B each per particle) + per-particle E, and the grid forward (32 B) + adjoint (64 B)
@@ -154,6 +163,7 @@ def fwd_bwd_leg(scene, steps, ckpt, impl):
             "roofline": {"bound": "hbm", "achieved": fb_bytes * steps / t / 1e9, "peak": HBM_PEAK / 1e9,
                          "unit": "GB/s", "frac": fb_bytes * steps / t / HBM_PEAK,
                          "alg_bytes_per_step": fb_bytes, "active_nodes": active},
+            "kernel_ms_per_launch": kt,
             "workload": f"{scene.name}: one scene of batch_fb ({n:,} particles, 128^3, neo-Hookean, per-particle E, "
                         f"dt 5e-5), {steps} steps, checkpoint every {ckpt} (recompute), centre-of-mass loss"}
 
