@@ -412,10 +412,11 @@ public:
             push();
             // checkpoint: state, caller index, E and mass per particle; tape step:
             // the same plus the binning's permutation and source maps, and the
-            // step's grid (m, p) and (v, m)
+            // step's grid (m, p) and (v, m), on each of the two tapes (the
+            // backward recomputes one segment while it reverses the other)
             const double ckpt_bytes = 108.0 * n;
             const double nodes = static_cast<double>(grid_->dims.x) * grid_->dims.y * grid_->dims.z;
-            const double step_bytes = 116.0 * n + 32.0 * nodes;
+            const double step_bytes = 2.0 * (116.0 * n + 32.0 * nodes);
             const int K = checkpointInterval(config_, static_cast<int>(recorded_steps_), ckpt_bytes, step_bytes);
             check(pg_mpm_forward_checkpointed(ctx_, static_cast<int>(recorded_steps_), K, nullptr));
             check(pg_mpm_backward(ctx_, ax.data(), av.data(), nullptr, aF.data(), gx.data(), gv.data(), gC.data(),

@@ -57,6 +57,13 @@ struct pg_mpm_ctx {
     int tape_filled = 0;   // steps of the tape holding unreversed grids
     std::vector<pg::TapeStep> ckpt;   // S, pid, E, mass used
     std::vector<pg::TapeStep> tape;
+    // tiled: a second segment tape, so that the backward recomputes one segment
+    // on its own stream while it reverses the other, with the recompute's own
+    // binning scratch and window buffers (rw: those fields only)
+    std::vector<pg::TapeStep> tape2;
+    pg::TiledWork rw{};
+    hipStream_t bstream = nullptr;
+    hipEvent_t bev[5] = {};     // backward start, segment recomputed (per tape), segment reversed (per tape)
     std::vector<void*> ck_allocs, tape_allocs;
     // hipGraph replay of pg_mpm_step sequences: key (nsteps, state buffer) -> executable graph
     bool use_graph = false;
@@ -477,6 +484,9 @@ int pg_mpm_destroy(pg_mpm_ctx* c) {
     for (void* p : c->tape_allocs) hipFree(p);
     for (auto& ge : c->graphs) hipGraphExecDestroy(ge.exec);
     if (c->gstream) hipStreamDestroy(c->gstream);
+    if (c->bstream) hipStreamDestroy(c->bstream);
+    for (hipEvent_t ev : c->bev)
+        if (ev) hipEventDestroy(ev);
     delete c;
     return PG_OK;
 }
@@ -973,6 +983,7 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     hipDeviceSynchronize();
     free_all(c->tape_allocs);
     c->tape.assign(K + 1, pg::TapeStep{});
+    c->tape2.assign(plain ? 0 : K + 1, pg::TapeStep{});
     c->tape_len = 0;
     c->tape_filled = 0;   // the new tape's grids start cleared
     c->tape_seg = -1;
@@ -981,7 +992,9 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     const size_t nblk = (size_t)w.total_blocks;
     auto& A = c->tape_allocs;
     int rc;
-    for (auto& t : c->tape) {
+    for (int ts = 0; ts < 2 * (K + 1); ++ts) {
+        if (ts > K && plain) break;
+        pg::TapeStep& t = ts <= K ? c->tape[ts] : c->tape2[ts - K - 1];
         if (plain) {   // user order throughout: states and dense grids only
             if ((rc = valloc(c, A, &t.S, 6 * c->cap, false)) || (rc = valloc(c, A, &t.mp, G, true)) ||
                 (rc = valloc(c, A, &t.gv, G, true)))
@@ -1004,7 +1017,15 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
         (rc = valloc(c, A, &c->aEp, c->cap, false)) || (rc = valloc(c, A, &c->adst, c->cap, false)) ||
         (!plain && ((rc = valloc(c, A, &c->part2, 3 * c->cap, false)) ||
                     (rc = valloc(c, A, &c->aEp2, c->cap, false)) || (rc = valloc(c, A, &c->adst2, c->cap, false)) ||
-                    (rc = valloc(c, A, &c->ainv, c->cap, false)))) ||
+                    (rc = valloc(c, A, &c->ainv, c->cap, false)) ||
+                    (rc = valloc(c, A, &c->rw.count, (size_t)w.tile_slots, true)) ||
+                    (rc = valloc(c, A, &c->rw.start, (size_t)w.tile_slots, false)) ||
+                    (rc = valloc(c, A, &c->rw.key, c->cap, false)) || (rc = valloc(c, A, &c->rw.rank, c->cap, false)) ||
+                    (rc = valloc(c, A, &c->rw.bsum, 5 * (((size_t)w.tile_slots + 1023) / 1024), false)) ||
+                    (rc = valloc(c, A, &c->rw.bflag, (size_t)w.tile_slots, true)) ||
+                    (rc = valloc(c, A, &c->rw.blist, (size_t)w.tile_slots, false)) ||
+                    (rc = valloc(c, A, &c->rw.nblist, 1, true)) ||
+                    (rc = valloc(c, A, &c->rw.wbuf, (size_t)w.max_entries * 512, false)))) ||
         (rc = valloc(c, A, &c->ux, 3 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uv, 3 * c->cap, false)) || (rc = valloc(c, A, &c->uC, 9 * c->cap, false)) ||
         (rc = valloc(c, A, &c->uF, 9 * c->cap, false)))
@@ -1013,6 +1034,8 @@ int ensure_tape(pg_mpm_ctx* c, int K) {
     return PG_OK;
 fail:
     c->tape.clear();
+    c->tape2.clear();
+    c->rw = pg::TiledWork{};
     free_all(c->tape_allocs);
     c->abar = c->part = c->agv = c->apam = nullptr;
     c->aE = c->aEs = c->aEp = c->ux = c->uv = c->uC = c->uF = nullptr;
@@ -1140,13 +1163,11 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
     // tiled: between segments the adjoint stays in storage order (that of
     // checkpoint k, where segment k's reverse ends); segment k-1's first reverse
     // kernel reads it through the inverse of that order
-    bool carried = false;
-    const pg::TapeStep own0 = c->tape.empty() ? pg::TapeStep{} : c->tape[0];
-    for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
-        const int L = std::min(K, c->ck_steps - k * K);
-        pg::TapeStep* T = c->tape.data();
-        const pg::TapeStep& ck = c->ckpt[k];
-        if (plain) {
+    if (plain) {
+        for (int k = ncp - 1; k >= 0 && e == hipSuccess; --k) {
+            const int L = std::min(K, c->ck_steps - k * K);
+            pg::TapeStep* T = c->tape.data();
+            const pg::TapeStep& ck = c->ckpt[k];
             e = copy_planes(T[0].S, ck.S, c->cap, n, 6, st);
             // recompute the segment: tape[s].S is the state before step s, tape[s].mp/gv its grid
             for (int s = 0; s < L && e == hipSuccess; ++s) {
@@ -1163,34 +1184,85 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
                                               c->mat_table(), c->bc_flags(), st, &c->prof);
             if (e == hipSuccess)
                 e = pg::adjoint_to_user(n, c->cap, nullptr, c->abar, c->ux, c->uv, c->uC, c->uF, nullptr, nullptr, st);
-            continue;
         }
-        const bool taped = k == c->tape_seg && c->tape_version == c->version && c->tape_filled == L;
-        if (!taped) {
-            // recompute the segment into the tape, starting from the checkpoint in
-            // place: neither the recompute nor the reverse writes step 0's state
+    } else {
+        // Tiled: segment k is reversed on the caller's stream from one tape while
+        // segment k-1 is recomputed into the other on the context's backward
+        // stream, with its own binning scratch and window buffers; events hand
+        // each tape over (recomputed -> reversed -> recomputed again).  Between
+        // segments the adjoint stays in storage order (that of checkpoint k, where
+        // segment k's reverse ends); segment k-1's first reverse kernel reads it
+        // through the inverse of that order.
+        pg::TapeStep* Ts[2] = {c->tape.data(), c->tape2.data()};
+        const pg::TapeStep own0[2] = {c->tape[0], c->tape2[0]};
+        pg::TiledWork w2 = c->tw;
+        w2.count = c->rw.count;
+        w2.start = c->rw.start;
+        w2.key = c->rw.key;
+        w2.rank = c->rw.rank;
+        w2.bsum = c->rw.bsum;
+        w2.bflag = c->rw.bflag;
+        w2.blist = c->rw.blist;
+        w2.nblist = c->rw.nblist;
+        w2.wbuf = c->rw.wbuf;
+        if (!c->bstream) e = hipStreamCreateWithFlags(&c->bstream, hipStreamNonBlocking);
+        for (hipEvent_t& ev : c->bev)
+            if (e == hipSuccess && !ev) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        hipStream_t st2 = c->bstream;
+        hipEvent_t ev_start = c->bev[0], *ev_recomputed = c->bev + 1, *ev_reversed = c->bev + 3;
+        // the recompute stream starts after everything the caller queued (the forward, the seeds)
+        if (e == hipSuccess) e = hipEventRecord(ev_start, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st2, ev_start, 0);
+        auto seg_len = [&](int k) { return std::min(K, c->ck_steps - k * K); };
+        auto recompute = [&](int k, int set) {
+            // from the checkpoint in place: neither the recompute nor the reverse
+            // writes step 0's state
+            pg::TapeStep* T = Ts[set];
+            const pg::TapeStep& ck = c->ckpt[k];
             T[0].S = ck.S;
             T[0].pid = ck.pid;
             T[0].E = ck.E;
             T[0].mass = ck.mass;
-            if (e == hipSuccess) e = pg::tiled_tape_forward(c->tw, n, c->cap, T, L, st);
-        }
+            if (e == hipSuccess) e = pg::tiled_tape_forward(w2, n, c->cap, T, seg_len(k), st2);
+            if (e == hipSuccess) e = hipEventRecord(ev_recomputed[set], st2);
+        };
+        const int k0 = ncp - 1;
+        const bool taped = k0 == c->tape_seg && c->tape_version == c->version && c->tape_filled == seg_len(k0);
         c->tape_seg = -1;
         c->tape_filled = 0;   // the reverse pass clears the grids it reads
-        if (e == hipSuccess && !carried)
-            e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, aE, aEs, st);
-        if (e == hipSuccess && carried) e = pg::slot_of_caller(n, c->ckpt[k + 1].pid, c->ainv, st);
-        if (e == hipSuccess) {
-            float4* const part[2] = {c->part, c->part2};
-            float* const aEp[2] = {c->aEp, c->aEp2};
-            uint32_t* const dst[2] = {c->adst, c->adst2};
-            e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, carried ? c->ainv : nullptr, part, c->agv,
-                                    c->apam, aEs, aEp, dst, st);
+        if (!taped) recompute(k0, 0);
+        bool used[2] = {false, false};
+        int set = 0;
+        for (int k = k0; k >= 0 && e == hipSuccess; --k) {
+            const int L = seg_len(k);
+            pg::TapeStep* T = Ts[set];
+            const bool carried = k < k0;
+            if (carried || !taped) e = hipStreamWaitEvent(st, ev_recomputed[set], 0);
+            if (e == hipSuccess && k >= 1) {
+                // the other tape is free once the reverse that last read it is done
+                if (used[set ^ 1]) e = hipStreamWaitEvent(st2, ev_reversed[set ^ 1], 0);
+                recompute(k - 1, set ^ 1);
+            }
+            if (e == hipSuccess && !carried)
+                e = pg::adjoint_from_user(n, c->cap, T[L].pid, c->ux, c->uv, c->uC, c->uF, c->abar, aE, aEs, st);
+            if (e == hipSuccess && carried) e = pg::slot_of_caller(n, c->ckpt[k + 1].pid, c->ainv, st);
+            if (e == hipSuccess) {
+                float4* const part[2] = {c->part, c->part2};
+                float* const aEp[2] = {c->aEp, c->aEp2};
+                uint32_t* const dst[2] = {c->adst, c->adst2};
+                e = pg::adjoint_segment(c->tw, n, c->cap, T, L, c->abar, carried ? c->ainv : nullptr, part, c->agv,
+                                        c->apam, aEs, aEp, dst, st);
+            }
+            if (e == hipSuccess && k == 0)
+                e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);
+            if (e == hipSuccess) e = hipEventRecord(ev_reversed[set], st);
+            used[set] = true;
+            set ^= 1;
         }
-        if (e == hipSuccess && k == 0)
-            e = pg::adjoint_to_user(n, c->cap, T[0].pid, c->abar, c->ux, c->uv, c->uC, c->uF, aEs, aE, st);
-        carried = true;
-        T[0] = own0;   // the tape's own step-0 buffers again (the forward writes into them)
+        // every recompute was waited for by the reverse of its segment; the tapes'
+        // own step-0 buffers again (the forward writes into them)
+        Ts[0][0] = own0[0];
+        Ts[1][0] = own0[1];
     }
     auto out = [&](float* dst, const float* src, size_t count) {
         if (e == hipSuccess && dst) e = hipMemcpyAsync(dst, src, count * 4, hipMemcpyDefault, st);
