@@ -216,6 +216,32 @@ def test_backward_twice(gpu):
     ctx.close()
 
 
+def test_backward_on_a_side_stream(gpu):
+    """The backward recomputes on the context's own stream and reverses on the
+    caller's: called on a side stream it must give what it gives on the
+    default stream (4 segments of 3 steps, so three tape hand-overs)."""
+    from physgrad_amd.solver import MPMContext
+    rng = np.random.default_rng(31)
+    n = 2500
+    sc = scene(rng, n, 0)
+    seeds = [torch.as_tensor(rng.normal(size=(n, c)), dtype=torch.float32, device="cuda") for c in (3, 3, 9, 9)]
+    ctx = MPMContext(sc.params(impl=1))
+    ctx.set_particles(sc.x, sc.v, E=sc.E_particles)
+    ctx.forward_checkpointed(12, 3)
+    want = {k: v.cpu().numpy() for k, v in ctx.backward(*seeds).items()}
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        got = ctx.backward(*seeds)
+        got = {k: v.cpu().numpy() for k, v in got.items()}
+    torch.cuda.synchronize()
+    for key in ("x", "v", "C", "F", "E"):
+        err = rel(got[key], want[key])
+        print(key, err)
+        assert err < 1e-4, (key, err)
+    ctx.close()
+
+
 def test_backward_requires_forward(gpu):
     from physgrad_amd import PgError
     from physgrad_amd.solver import MPMContext
