@@ -63,7 +63,8 @@ struct pg_mpm_ctx {
     std::vector<pg::TapeStep> tape2;
     pg::TiledWork rw{};
     hipStream_t bstream = nullptr;
-    hipEvent_t bev[5] = {};     // backward start, segment recomputed (per tape), segment reversed (per tape)
+    hipEvent_t bev[7] = {};     // backward start, segment recomputed (per tape), segment reversed (per tape);
+                                // forward: state ready for its checkpoint, checkpoint copied
     std::vector<void*> ck_allocs, tape_allocs;
     // hipGraph replay of pg_mpm_step sequences: key (nsteps, state buffer) -> executable graph
     bool use_graph = false;
@@ -975,6 +976,15 @@ int ensure_checkpoints(pg_mpm_ctx* c, int ncp) {
     return PG_OK;
 }
 
+// the context's second stream and its events (created on first use)
+hipError_t side_stream(pg_mpm_ctx* c) {
+    hipError_t e = hipSuccess;
+    if (!c->bstream) e = hipStreamCreateWithFlags(&c->bstream, hipStreamNonBlocking);
+    for (hipEvent_t& ev : c->bev)
+        if (e == hipSuccess && !ev) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    return e;
+}
+
 int ensure_tape(pg_mpm_ctx* c, int K) {
     const bool plain = c->p.impl == PG_IMPL_PLAIN;   // plain tapes hold no E / mass copies
     if (c->tape_len >= K + 1 && c->abar && (plain || !c->have_E || c->tape[0].E) &&
@@ -1070,20 +1080,32 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
     c->tape_seg = -1;
     for (int k = 0; k < ncp && e == hipSuccess; ++k) {
         pg::TapeStep& ck = c->ckpt[k];
-        e = copy_planes(ck.S, c->S, c->cap, c->n, 6, st);
+        const int L = std::min(K, nsteps - k * K);
         if (plain) {   // E and mass stay in user order: only the state moves
-            const int L = std::min(K, nsteps - k * K);
+            e = copy_planes(ck.S, c->S, c->cap, c->n, 6, st);
             for (int s = 0; s < L && e == hipSuccess; ++s)
                 e = pg::mls_plain_step(g, c->n, c->cap, c->S, c->have_mass ? c->mass : nullptr,
                                        c->have_E ? c->E : nullptr, c->mat_table(), c->bc_flags(), c->mp, c->gv, st,
                                        &c->prof);
             continue;
         }
-        if (e == hipSuccess) e = copy_u32(ck.pid, c->pid, c->n, st);
-        if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(ck.E, c->E, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
+        // tiled: a checkpoint before a segment of two or more steps that is not the
+        // last is copied on the context's second stream, beside the segment's
+        // first step (which does not write the state it reads)
+        const bool side = k + 1 < ncp && L >= 2;
+        hipStream_t cs = st;
+        if (side) {
+            if (e == hipSuccess) e = side_stream(c);
+            if (e == hipSuccess) e = hipEventRecord(c->bev[5], st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->bstream, c->bev[5], 0);
+            cs = c->bstream;
+        }
+        if (e == hipSuccess) e = copy_planes(ck.S, c->S, c->cap, c->n, 6, cs);
+        if (e == hipSuccess) e = copy_u32(ck.pid, c->pid, c->n, cs);
+        if (e == hipSuccess && c->have_E) e = hipMemcpyAsync(ck.E, c->E, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, cs);
         if (e == hipSuccess && c->have_mass)
-            e = hipMemcpyAsync(ck.mass, c->mass, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, st);
-        const int L = std::min(K, nsteps - k * K);
+            e = hipMemcpyAsync(ck.mass, c->mass, 4 * (size_t)c->n, hipMemcpyDeviceToDevice, cs);
+        if (e == hipSuccess && side) e = hipEventRecord(c->bev[6], cs);
         float* E = c->have_E ? c->E : nullptr;
         float* E2 = c->have_E ? c->E2 : nullptr;
         float* m = c->have_mass ? c->mass : nullptr;
@@ -1112,8 +1134,19 @@ int pg_mpm_forward_checkpointed(pg_mpm_ctx* c, int nsteps, int checkpoint_every,
             c->tape_seg = k;
             continue;
         }
-        if (e == hipSuccess)
+        if (e == hipSuccess && L >= 2) {
+            // the first step reads the state and writes the other buffer; the
+            // second overwrites it, so it waits for the checkpoint copy, which runs
+            // on the context's second stream beside the first step
+            e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv, 1,
+                                st, false, true);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, c->bev[6], 0);
+            if (e == hipSuccess)
+                e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv,
+                                    L - 1, st, true, false);
+        } else if (e == hipSuccess) {
             e = pg::tiled_steps(c->tw, c->n, c->cap, c->S, c->S2, c->pid, c->pid2, E, E2, m, m2, c->mp, c->gv, L, st);
+        }
         if (c->have_E) { c->E = E; c->E2 = E2; }
         if (c->have_mass) { c->mass = m; c->mass2 = m2; }
     }
@@ -1205,9 +1238,7 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         w2.blist = c->rw.blist;
         w2.nblist = c->rw.nblist;
         w2.wbuf = c->rw.wbuf;
-        if (!c->bstream) e = hipStreamCreateWithFlags(&c->bstream, hipStreamNonBlocking);
-        for (hipEvent_t& ev : c->bev)
-            if (e == hipSuccess && !ev) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = side_stream(c);
         hipStream_t st2 = c->bstream;
         hipEvent_t ev_start = c->bev[0], *ev_recomputed = c->bev + 1, *ev_reversed = c->bev + 3;
         // the recompute stream starts after everything the caller queued (the forward, the seeds)
