@@ -1220,7 +1220,7 @@ int pg_mpm_backward(pg_mpm_ctx* c, const float* ax, const float* av, const float
         }
     } else {
         // Tiled: segment k is reversed on the caller's stream from one tape while
-        // segment k-1 is recomputed into the other on the context's backward
+        // segment k-1 is recomputed into the other on the context's second
         // stream, with its own binning scratch and window buffers; events hand
         // each tape over (recomputed -> reversed -> recomputed again).  Between
         // segments the adjoint stays in storage order (that of checkpoint k, where
